@@ -1,0 +1,206 @@
+/*
+ * wf.h — C-ABI of the MI355X wavefront path tracer.
+ *
+ * The entry points mirror the reference renderer's front-end
+ * (source/renderer/renderer.h:13-55) so that a caller of that API can switch
+ * to this library; the global singleton `g_renderer` (renderer.cpp:25)
+ * becomes an explicit wf_renderer handle.  Every entry point returns a
+ * wf_status instead of asserting (core/assertion.h:51-67), copies its inputs
+ * before returning, and never retains caller pointers.  Output buffers are
+ * caller-owned host memory unless the name ends in _device.
+ *
+ * Wire formats are the reference's (shared.hlsl.h:58-161): wf_vertex is
+ * vertex_t (48 B), wf_ray is RayDesc2 (32 B), wf_hit is hit_result_t (20 B),
+ * wf_bvh_node is bvh_node_t (32 B), wf_tlas_node is tlas_node_t (32 B),
+ * wf_bvh_instance is bvh_instance_t (92 B).  All structs are 4-byte packed.
+ *
+ * Threading: one wf_renderer per thread; a renderer is not internally
+ * synchronised.  All GPU work of a renderer is ordered on its own HIP stream;
+ * readbacks synchronise that stream.
+ */
+#ifndef WF_H
+#define WF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WF_API __attribute__((visibility("default")))
+
+typedef enum wf_status {
+    WF_OK = 0,
+    WF_ERROR_INVALID_ARGUMENT = 1,
+    WF_ERROR_CAPACITY = 2,          /* an instance/texture/mesh/TLAS limit was exceeded */
+    WF_ERROR_OUT_OF_MEMORY = 3,
+    WF_ERROR_DEVICE = 4,            /* a HIP runtime call failed */
+    WF_ERROR_STACK_OVERFLOW = 5,    /* a traversal stack overflowed (result for that ray is a miss) */
+    WF_ERROR_INVALID_STATE = 6,     /* call order violated (e.g. render without begin_scene) */
+    WF_ERROR_NOT_FOUND = 7          /* stale or unknown handle */
+} wf_status;
+
+typedef struct wf_renderer wf_renderer;
+
+/* resource_handle_t (core/api_types.h:4-25): low 32 bits = index, high 32 bits
+ * = version; WF_INVALID_HANDLE = UINT64_MAX. */
+typedef uint64_t wf_mesh_handle;
+typedef uint64_t wf_texture_handle;
+#define WF_INVALID_HANDLE UINT64_MAX
+
+/* vertex_t (shared.hlsl.h:58-64) */
+typedef struct wf_vertex { float position[3]; float normal[3]; float tangent[4]; float uv[2]; } wf_vertex;
+/* RayDesc2 (shared.hlsl.h:155-161).  The software path ignores tmin (the
+ * origin is already nudged by direction*1e-8, common.hlsl:145-154) and starts
+ * the closest-hit search with ray.t = tmax. */
+typedef struct wf_ray { float origin[3]; float tmin; float direction[3]; float tmax; } wf_ray;
+/* hit_result_t (shared.hlsl.h:144-150); a miss is {~0u, ~0u, FLT_MAX, {0,0}} */
+typedef struct wf_hit { uint32_t instance_idx; uint32_t primitive_idx; float t; float bary[2]; } wf_hit;
+/* bvh_header_t / bvh_node_t / bvh_triangle_t / bvh_instance_t / tlas_header_t / tlas_node_t */
+typedef struct wf_bvh_header { uint32_t nodes_offset, triangles_offset, indices_offset; } wf_bvh_header;
+typedef struct wf_bvh_node { float aabb_min[3]; uint32_t left_first; float aabb_max[3]; uint32_t prim_count; } wf_bvh_node;
+typedef struct wf_bvh_triangle { float p0[3], p1[3], p2[3]; } wf_bvh_triangle;
+typedef struct wf_bvh_instance { float world_to_local[16]; float aabb_min[3]; float aabb_max[3]; uint32_t bvh_index; } wf_bvh_instance;
+typedef struct wf_tlas_header { uint32_t nodes_offset, instances_offset; } wf_tlas_header;
+typedef struct wf_tlas_node { float aabb_min[3]; uint32_t left_right; float aabb_max[3]; uint32_t instance_idx; } wf_tlas_node;
+
+/* renderer::init_params_t (renderer.h:13-19) */
+typedef struct wf_init_params { uint32_t render_width; uint32_t render_height; uint32_t backbuffer_count; uint32_t vsync; } wf_init_params;
+
+/* camera_t (core/camera/camera.h:5-13): column-major glm matrices */
+typedef struct wf_camera { float inv_view_matrix[16]; float view_matrix[16]; float vfov_deg; float yaw; float pitch; } wf_camera;
+
+/* view_t (shared.hlsl.h:46-56): the per-frame camera constants handed to the kernels */
+typedef struct wf_view {
+    float world_to_view[16]; float view_to_world[16]; float view_to_clip[16]; float clip_to_view[16];
+    float render_dim[2]; float near_plane; float far_plane;
+} wf_view;
+
+/* TEXTURE_FORMAT subset accepted by create_render_texture */
+enum { WF_TEXTURE_FORMAT_RGBA8 = 1, WF_TEXTURE_FORMAT_RGBA8_SRGB = 2, WF_TEXTURE_FORMAT_RGB32F = 3, WF_TEXTURE_FORMAT_RGBA32F = 4 };
+
+/* renderer::render_texture_params_t (renderer.h:33-42) */
+typedef struct wf_render_texture_params { uint32_t width; uint32_t height; uint32_t bits_per_pixel; uint32_t format; const void* ptr_data; } wf_render_texture_params;
+
+/* renderer::render_mesh_params_t (renderer.h:45-53) */
+typedef struct wf_render_mesh_params { uint32_t vertex_count; const wf_vertex* vertices; uint32_t index_count; const uint32_t* indices; } wf_render_mesh_params;
+
+/* material_asset_t (core/assets/asset_types.h:10-21) */
+typedef struct wf_material_asset {
+    float base_color_factor[4]; float metallic_factor; float roughness_factor;
+    float emissive_factor[3]; float emissive_strength;
+    wf_texture_handle base_color_texture; wf_texture_handle normal_texture;
+    wf_texture_handle metallic_roughness_texture; wf_texture_handle emissive_texture;
+} wf_material_asset;
+
+/* render_settings_t (shared.hlsl.h:34-44) followed by this library's additions.
+ * Defaults (renderer.cpp:32-45 plus additions): wavefront on, software RT on
+ * (the only traversal this library has), view mode none, max_bounces 3,
+ * cosine-weighted diffuse, accumulate, env strength 1, spp 1, Russian
+ * roulette from bounce 3 with survival 0.75, next-event estimation off,
+ * frame_seed 0 (0 = advance an internal xorshift per frame, renderer.cpp:739). */
+typedef struct wf_render_settings {
+    uint32_t use_wavefront_pathtracing;
+    uint32_t use_software_rt;
+    uint32_t render_view_mode;
+    uint32_t max_bounces;
+    uint32_t cosine_weighted_diffuse;
+    uint32_t accumulate;
+    float    hdr_env_strength;
+    /* additions (no reference counterpart) */
+    uint32_t spp;
+    uint32_t rr_start_bounce;
+    float    rr_survival;
+    uint32_t next_event_estimation;
+    uint32_t frame_seed;
+} wf_render_settings;
+
+/* bvh_builder_t::build_options_t (bvh/bvh_builder.h:17-21); the reference
+ * renderer uses interval_count 8, subdivide_single_prim false (renderer.cpp:95-96) */
+typedef struct wf_build_options { uint32_t interval_count; uint32_t subdivide_single_prim; } wf_build_options;
+
+/* Per-frame stage timings (ms) from HIP events, the reference's GPU profiler
+ * scopes (gpu_profiler.h:13-28) for the wavefront stages. */
+typedef struct wf_frame_stats {
+    float ms_clear, ms_generate, ms_extend, ms_shade, ms_connect, ms_post, ms_total;
+    uint32_t rays_extended;      /* closest-hit rays traced this frame */
+    uint32_t shadow_rays;        /* occlusion rays traced this frame */
+    uint32_t stack_overflows;
+} wf_frame_stats;
+
+/* ---- lifecycle: renderer::init / exit (renderer.cpp:222-541) ---------- */
+WF_API wf_status wf_renderer_init(const wf_init_params* params, wf_renderer** out_renderer);
+WF_API wf_status wf_renderer_exit(wf_renderer* r);
+WF_API const char* wf_renderer_last_error(const wf_renderer* r);
+
+/* ---- frame / scene: renderer::begin_frame .. end_scene (renderer.cpp:543-1055) */
+WF_API wf_status wf_renderer_begin_frame(wf_renderer* r);
+WF_API wf_status wf_renderer_end_frame(wf_renderer* r);
+WF_API wf_status wf_renderer_begin_scene(wf_renderer* r, const wf_camera* camera, wf_texture_handle env_texture);
+WF_API wf_status wf_renderer_render(wf_renderer* r);
+WF_API wf_status wf_renderer_end_scene(wf_renderer* r);
+
+/* ---- resources: create_render_texture / create_render_mesh / submit_render_mesh
+ *      (renderer.cpp:1150-1350) ---------------------------------------- */
+WF_API wf_status wf_renderer_create_render_texture(wf_renderer* r, const wf_render_texture_params* params, wf_texture_handle* out_handle);
+WF_API wf_status wf_renderer_create_render_mesh(wf_renderer* r, const wf_render_mesh_params* params, wf_mesh_handle* out_handle);
+WF_API wf_status wf_renderer_submit_render_mesh(wf_renderer* r, wf_mesh_handle mesh, const float transform[16], const wf_material_asset* material);
+
+/* ---- settings (render_settings_t, editable through the reference's UI, renderer.cpp:1120-1148) */
+WF_API wf_status wf_renderer_get_settings(const wf_renderer* r, wf_render_settings* out);
+WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_render_settings* settings);
+/* options for meshes created after the call (renderer.cpp:93-96) */
+WF_API wf_status wf_renderer_set_build_options(wf_renderer* r, const wf_build_options* options);
+
+/* ---- camera::create (core/camera/camera.cpp:8-16): lookAtLH, up = +y ---- */
+WF_API wf_status wf_camera_create(wf_camera* out, const float view_origin[3], const float target_pos[3], float vfov_deg);
+/* the view_t that begin_scene derives (renderer.cpp:614-629) */
+WF_API wf_status wf_renderer_get_view(const wf_renderer* r, wf_view* out);
+/* the same derivation without a renderer (host only) */
+WF_API wf_status wf_make_view(const wf_camera* camera, uint32_t width, uint32_t height, wf_view* out);
+
+/* ---- readbacks --------------------------------------------------------- */
+/* linear path-traced energy of the last render(): width*height RGBA float
+ * (the reference's texture_energy, renderer.cpp:470-474) */
+WF_API wf_status wf_renderer_read_energy(wf_renderer* r, float* out_rgba);
+/* progressive accumulator (post_process.hlsl:40-51): width*height RGBA float */
+WF_API wf_status wf_renderer_read_accumulator(wf_renderer* r, float* out_rgba);
+/* tone-mapped sRGB output (post_process.hlsl:53-71): width*height RGBA8 */
+WF_API wf_status wf_renderer_read_final_color(wf_renderer* r, uint8_t* out_rgba8);
+WF_API wf_status wf_renderer_get_frame_stats(const wf_renderer* r, wf_frame_stats* out);
+
+/* ---- the extend stage on caller rays (extend.hlsl:25-40) -------------- */
+/* Builds the TLAS from the instances submitted since begin_scene and uploads
+ * it; render() does this itself (renderer.cpp:637-670). */
+WF_API wf_status wf_renderer_build_tlas(wf_renderer* r);
+/* Closest hit for n rays (host arrays).  Returns WF_ERROR_STACK_OVERFLOW if
+ * any ray overflowed its traversal stack (that ray reports a miss). */
+WF_API wf_status wf_trace_closest(wf_renderer* r, const wf_ray* rays, uint32_t n, wf_hit* hits);
+/* Same, on device arrays, ordered on the renderer's stream (no host sync). */
+WF_API wf_status wf_trace_closest_device(wf_renderer* r, const wf_ray* d_rays, uint32_t n, wf_hit* d_hits);
+/* Occlusion: visible[i] = 1 if nothing is hit with t in [0, tmax). */
+WF_API wf_status wf_trace_occluded(wf_renderer* r, const wf_ray* rays, uint32_t n, uint32_t* visible);
+/* Per-ray traversal counters of the reference algorithm (4 x u32 per ray:
+ * TLAS nodes fetched, BLAS nodes fetched, instances visited, triangles
+ * tested), for the algorithmic-bytes roofline. */
+WF_API wf_status wf_trace_closest_counters(wf_renderer* r, const wf_ray* rays, uint32_t n, wf_hit* hits, uint32_t* counters);
+
+/* ---- host builders (bvh_builder_t / tlas_builder_t build + extract) ---- */
+/* SAH binned BLAS over n triangles given as 9 floats each (p0,p1,p2).
+ * Writes the reference's GPU blob (bvh_header_t ‖ nodes ‖ triangles ‖
+ * indices, renderer.cpp:114-144) into out_blob when capacity suffices;
+ * *out_size always receives the blob size, *out_node_count node_at. */
+WF_API wf_status wf_bvh_build(const float* tri_positions, uint32_t n, const wf_build_options* options,
+                              void* out_blob, uint64_t capacity, uint64_t* out_size, uint32_t* out_node_count);
+/* Agglomerative TLAS blob (tlas_header_t ‖ nodes ‖ instances). */
+WF_API wf_status wf_tlas_build(const wf_bvh_instance* instances, uint32_t m,
+                               void* out_blob, uint64_t capacity, uint64_t* out_size, uint32_t* out_node_count);
+/* The software instance record submit_render_mesh derives (renderer.cpp:1318-1336). */
+WF_API wf_status wf_make_bvh_instance(const float transform[16], const float blas_min[3], const float blas_max[3],
+                                      uint32_t bvh_index, wf_bvh_instance* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,145 @@
+/*
+ * CPU oracle for the wavefront path-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's
+ * algorithms, used by tests/, by the smoke check and by bench.py's CPU
+ * baseline leg as the thing results are checked against.  Nothing in the
+ * product library links, loads or calls it.
+ *
+ * Reference files restated (paths relative to the reference root):
+ *   source/renderer/bvh/bvh_builder.cpp, bvh/as_util.h     -> orc_blas_build
+ *   source/renderer/bvh/tlas_builder.cpp                   -> orc_tlas_build
+ *   source/renderer/renderer.cpp:1318-1336                 -> orc_instance_record
+ *   source/renderer/shaders/intersect.hlsl:7-76            -> orc_intersect_*
+ *   source/renderer/shaders/accelstruct.hlsl:58-218        -> orc_trace_closest
+ *   source/renderer/shaders/common.hlsl:145-169            -> orc_primary_rays
+ *   extern/glm (v1.0.2) lookAtLH / perspectiveFovLH_ZO /
+ *     inverse / mat4*vec4                                   -> orc_mat4_*, orc_camera_*
+ *   source/renderer/shaders/wavefront/{generate,extend,shade}.hlsl,
+ *     renderer.cpp:742-939                                  -> orc_render (wavefront loop)
+ *
+ * Floating point: FP32, no contraction (built with -ffp-contract=off), no
+ * fast-math, SSE scalar arithmetic, correctly rounded divide and sqrt.
+ *
+ * Parity status per function is listed in the pinning table of the test
+ * suite (tests/test_oracle_*.py); functions without an independent check say
+ * "parity unpinned" in their comment below.
+ */
+#ifndef WF_ORACLE_H
+#define WF_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reference wire layouts (shared.hlsl.h:106-161), 4-byte packed. */
+typedef struct { float aabb_min[3]; uint32_t left_first; float aabb_max[3]; uint32_t prim_count; } orc_node;      /* bvh_node_t 32 B */
+typedef struct { float p0[3]; float p1[3]; float p2[3]; } orc_tri;                                                  /* bvh_triangle_t 36 B */
+typedef struct { float aabb_min[3]; uint32_t left_right; float aabb_max[3]; uint32_t instance_idx; } orc_tnode;   /* tlas_node_t 32 B */
+typedef struct { float world_to_local[16]; float aabb_min[3]; float aabb_max[3]; uint32_t bvh_index; } orc_instance; /* bvh_instance_t 92 B */
+typedef struct { uint32_t instance_idx; uint32_t primitive_idx; float t; float bary[2]; } orc_hit;                 /* hit_result_t 20 B */
+typedef struct { float origin[3]; float tmin; float dir[3]; float tmax; } orc_ray;                                 /* RayDesc2 32 B */
+typedef struct { uint32_t tlas_nodes, blas_nodes, instances, tri_tests; } orc_counters;
+
+enum { ORC_OK = 0, ORC_EINVAL = 1, ORC_ECAP = 2, ORC_ESTACK = 5 };
+
+/* ---- builders ---------------------------------------------------------- */
+/* SAH binned BLAS.  pos9 = N triangles x (p0,p1,p2).  nodes_out holds 2N
+ * entries, idx_out N entries.  *node_count = nodes written (node_at). */
+int orc_blas_build(const float* pos9, uint32_t n, uint32_t bins, int single_prim,
+                   orc_node* nodes_out, uint32_t* idx_out, uint32_t* node_count);
+
+/* Agglomerative TLAS over M instance records; nodes_out holds 2M entries. */
+int orc_tlas_build(const orc_instance* inst, uint32_t m, orc_tnode* nodes_out, uint32_t* node_count);
+
+/* glm restatements (column-major m[col*4+row]) */
+void orc_mat4_inverse(const float m[16], float out[16]);
+void orc_mat4_mul_vec4(const float m[16], const float v[4], float out[4]);
+void orc_mat4_mul(const float a[16], const float b[16], float out[16]);
+
+/* Software instance record from a local_to_world transform and the BLAS
+ * root bounds (renderer.cpp:1318-1336). */
+void orc_instance_record(const float transform[16], const float blas_min[3], const float blas_max[3],
+                         uint32_t bvh_index, orc_instance* out);
+
+/* ---- intersection primitives ------------------------------------------- */
+int   orc_intersect_triangle(const float v0[3], const float v1[3], const float v2[3],
+                             const float o[3], const float d[3], float* t_inout, float bary[2]);
+float orc_intersect_aabb(const float bmin[3], const float bmax[3], const float o[3],
+                         const float inv_dir[3], float t);
+
+/* ---- scene + traversal -------------------------------------------------- */
+typedef struct {
+    const orc_tnode*    tnodes;     uint32_t n_tnodes;
+    const orc_instance* inst;       uint32_t n_inst;
+    const orc_node*     nodes;      const uint32_t* node_off;   /* per BLAS */
+    const orc_tri*      tris;       const uint32_t* idx;        const uint32_t* tri_off; /* per BLAS */
+    uint32_t            n_blas;
+} orc_scene;
+
+/* Closest hit for each ray (extend.hlsl:25-40).  counters may be NULL
+ * (else one orc_counters per ray).  nthreads <= 1 runs on the caller. */
+int orc_trace_closest(const orc_scene* s, const orc_ray* rays, uint32_t n, orc_hit* hits,
+                      orc_counters* counters, int nthreads);
+
+/* Occlusion (any hit with t in [0, tmax)); visible[i] = 1 when unoccluded. */
+int orc_trace_any(const orc_scene* s, const orc_ray* rays, uint32_t n, uint32_t* visible, int nthreads);
+
+/* Brute force closest t over every instance and triangle (no BVH). */
+int orc_brute_closest(const orc_scene* s, const orc_ray* rays, uint32_t n, orc_hit* hits);
+
+/* ---- camera + generate --------------------------------------------------- */
+typedef struct {
+    float world_to_view[16]; float view_to_world[16]; float view_to_clip[16]; float clip_to_view[16];
+    float render_dim[2]; float near_plane; float far_plane;
+} orc_view;  /* view_t 272 B */
+
+void orc_camera_view(const float eye[3], const float target[3], float vfov_deg,
+                     uint32_t width, uint32_t height, orc_view* out);
+void orc_primary_rays(const orc_view* v, uint32_t width, uint32_t height, orc_ray* rays);
+
+/* ---- wavefront render (generate/extend/shade/connect) -------------------- */
+typedef struct {
+    float    local_to_world[16];
+    float    base_color[3];
+    float    emissive[3];          /* emissive_factor * emissive_strength */
+    int32_t  base_color_tex;       /* -1: none (white) */
+    uint32_t blas;                 /* mesh id */
+} orc_material_instance;
+
+typedef struct {
+    const float* verts;            /* per mesh triangle_t data: 3 vertices x (pos3, nrm3, uv2) = 24 floats per tri */
+    const uint32_t* vert_off;      /* per mesh offset in triangles */
+    const orc_material_instance* mi;   /* per instance */
+    const float* tex;              /* RGB float textures, concatenated */
+    const uint32_t* tex_off;       /* per texture, offset in floats */
+    const uint32_t* tex_dim;       /* per texture (w, h) */
+    const uint32_t* emitters;      /* (instance, primitive) pairs */
+    uint32_t n_emitters;
+    float    emitter_area_total;   /* sum of world-space emitter areas */
+    const float* emitter_cdf;      /* n_emitters, cumulative area / total */
+} orc_shading;
+
+typedef struct {
+    uint32_t max_bounces;
+    uint32_t spp;
+    uint32_t rr_start;             /* bounce index at which Russian roulette starts */
+    uint32_t nee;                  /* 1: next-event estimation through connect */
+    uint32_t frame_seed;
+    float    rr_survive;
+} orc_render_params;
+
+int orc_render(const orc_scene* s, const orc_shading* sh, const orc_view* v, uint32_t width, uint32_t height,
+               const orc_render_params* p, float* energy_rgb, int nthreads);
+
+uint32_t orc_pcg_stream_first(uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t frame_seed);
+float    orc_sin_turns(float u);
+float    orc_cos_turns(float u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

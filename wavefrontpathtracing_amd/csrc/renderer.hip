@@ -1,0 +1,473 @@
+// Host runtime: the renderer object behind the C-ABI in include/wf.h.
+//
+// Mirrors renderer::init / create_render_mesh / submit_render_mesh /
+// begin_scene / render / end_scene (renderer.cpp:222-1350) with HIP device
+// memory in place of D3D12 resources and an explicit handle in place of the
+// g_renderer singleton.  Software ray tracing is the only traversal mode.
+#include "renderer_impl.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+using namespace wf;
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+wf_status wf_renderer::fail(wf_status s, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return s;
+}
+
+wf_status wf_renderer::hip_fail(hipError_t e, const char* what)
+{
+    const wf_status s = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? WF_ERROR_OUT_OF_MEMORY : WF_ERROR_DEVICE;
+    return fail(s, "%s: %s", what, hipGetErrorString(e));
+}
+
+hipError_t DevBuf::ensure(size_t n)
+{
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) { p = nullptr; bytes = 0; return e; }
+    bytes = n;
+    return hipSuccess;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+static inline uint64_t make_handle(uint32_t index, uint32_t version) { return ((uint64_t)version << 32) | index; }
+static inline uint32_t handle_index(uint64_t h) { return (uint32_t)(h & 0xFFFFFFFFu); }
+static inline uint32_t handle_version(uint64_t h) { return (uint32_t)(h >> 32); }
+
+Mesh* wf_renderer::find_mesh(uint64_t h)
+{
+    if (h == WF_INVALID_HANDLE) return nullptr;
+    const uint32_t i = handle_index(h);
+    if (i >= meshes.size() || !meshes[i].alive || meshes[i].version != handle_version(h)) return nullptr;
+    return &meshes[i];
+}
+
+Texture* wf_renderer::find_texture(uint64_t h)
+{
+    if (h == WF_INVALID_HANDLE) return nullptr;
+    const uint32_t i = handle_index(h);
+    if (i >= textures.size() || !textures[i].alive || textures[i].version != handle_version(h)) return nullptr;
+    return &textures[i];
+}
+
+// get_default_render_settings (renderer.cpp:32-45) + this library's additions
+static wf_render_settings default_settings()
+{
+    wf_render_settings s{};
+    s.use_wavefront_pathtracing = 1;
+    s.use_software_rt = 1;
+    s.render_view_mode = 0;
+    s.max_bounces = 3;
+    s.cosine_weighted_diffuse = 1;
+    s.accumulate = 1;
+    s.hdr_env_strength = 1.0f;
+    s.spp = 1;
+    s.rr_start_bounce = 3;
+    s.rr_survival = 0.75f;
+    s.next_event_estimation = 0;
+    s.frame_seed = 0;
+    return s;
+}
+
+wf_status wf_renderer::ensure_trace_scratch()
+{
+    // persistent grid: as many workgroups as stay resident on every CU
+    int per_cu = 0;
+    hipError_t e = extend_occupancy(ext_cfg, &per_cu);
+    if (e != hipSuccess) return hip_fail(e, "occupancy query");
+    if (per_cu < 1) per_cu = 1;
+    ext_grid = (uint32_t)per_cu * (uint32_t)cu_count;
+    spill_per_thread = ext_cfg.lds_stack > 0 ? (kStackTotal > ext_cfg.lds_stack ? kStackTotal - ext_cfg.lds_stack : 0) : 0;
+    const size_t threads = (size_t)ext_grid * ext_cfg.block;
+    if ((e = d_spill.ensure(std::max<size_t>(threads * spill_per_thread * sizeof(uint2), 16))) != hipSuccess)
+        return hip_fail(e, "spill stack allocation");
+    if ((e = d_work.ensure(64)) != hipSuccess) return hip_fail(e, "work counter allocation");
+    return WF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// lifecycle
+// ---------------------------------------------------------------------------
+
+extern "C" WF_API wf_status wf_renderer_init(const wf_init_params* params, wf_renderer** out)
+{
+    if (!params || !out) return WF_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (params->render_width == 0 || params->render_height == 0 ||
+        (uint64_t)params->render_width * params->render_height > (1ull << 28))
+        return WF_ERROR_INVALID_ARGUMENT;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return WF_ERROR_DEVICE;
+    wf_renderer* r = new (std::nothrow) wf_renderer();
+    if (!r) return WF_ERROR_OUT_OF_MEMORY;
+    r->width = params->render_width;
+    r->height = params->render_height;
+    r->settings = default_settings();
+    r->build_opts = wf_build_options{8, 0};   // renderer.cpp:95-96
+    hipError_t e = hipGetDevice(&r->device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, r->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete r; return WF_ERROR_DEVICE; }
+    r->cu_count = prop.multiProcessorCount;
+    for (int i = 0; i < kNumStageEvents; ++i) {
+        if (hipEventCreate(&r->ev[i]) != hipSuccess) { wf_renderer_exit(r); return WF_ERROR_DEVICE; }
+    }
+    if (r->ensure_trace_scratch() != WF_OK) { wf_renderer_exit(r); return WF_ERROR_DEVICE; }
+    *out = r;
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_renderer_exit(wf_renderer* r)
+{
+    if (!r) return WF_ERROR_INVALID_ARGUMENT;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    for (auto& m : r->meshes) { m.nodes.release(); m.tris.release(); m.shade.release(); }
+    for (auto& t : r->textures) t.texels.release();
+    r->release_frame_buffers();
+    r->d_tnodes.release(); r->d_inst.release(); r->d_shade_inst.release(); r->d_textures.release();
+    r->d_emitters.release();
+    r->d_work.release(); r->d_spill.release(); r->d_tmp_rays.release(); r->d_tmp_hits.release(); r->d_tmp_aux.release();
+    for (int i = 0; i < kNumStageEvents; ++i) if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+    return WF_OK;
+}
+
+extern "C" WF_API const char* wf_renderer_last_error(const wf_renderer* r)
+{
+    return r ? r->err.c_str() : "null renderer";
+}
+
+// ---------------------------------------------------------------------------
+// settings / camera
+// ---------------------------------------------------------------------------
+
+extern "C" WF_API wf_status wf_renderer_get_settings(const wf_renderer* r, wf_render_settings* out)
+{
+    if (!r || !out) return WF_ERROR_INVALID_ARGUMENT;
+    *out = r->settings;
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_render_settings* s)
+{
+    if (!r || !s) return WF_ERROR_INVALID_ARGUMENT;
+    if (s->max_bounces > kMaxBounces) return r->fail(WF_ERROR_INVALID_ARGUMENT, "max_bounces %u > %u", s->max_bounces, kMaxBounces);
+    if (s->spp == 0 || s->spp > 4096) return r->fail(WF_ERROR_INVALID_ARGUMENT, "spp must be in [1, 4096]");
+    if (!(s->rr_survival > 0.0f && s->rr_survival <= 1.0f)) return r->fail(WF_ERROR_INVALID_ARGUMENT, "rr_survival must be in (0, 1]");
+    if (s->use_software_rt == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only software ray tracing is available");
+    if (s->render_view_mode != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "debug view modes are not supported");
+    if (s->cosine_weighted_diffuse == 0 && s->next_event_estimation)
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "next-event estimation requires cosine-weighted sampling");
+    const bool reset = s->max_bounces != r->settings.max_bounces || s->spp != r->settings.spp ||
+                       s->next_event_estimation != r->settings.next_event_estimation;
+    r->settings = *s;
+    if (reset) r->accum_count = 0;
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_renderer_set_build_options(wf_renderer* r, const wf_build_options* o)
+{
+    if (!r || !o) return WF_ERROR_INVALID_ARGUMENT;
+    if (o->interval_count < 2 || o->interval_count > 1024) return r->fail(WF_ERROR_INVALID_ARGUMENT, "interval_count must be in [2, 1024]");
+    r->build_opts = *o;
+    return WF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// resources
+// ---------------------------------------------------------------------------
+
+extern "C" WF_API wf_status wf_renderer_create_render_texture(wf_renderer* r, const wf_render_texture_params* p, wf_texture_handle* out)
+{
+    if (!r || !p || !out || !p->ptr_data || p->width == 0 || p->height == 0) return WF_ERROR_INVALID_ARGUMENT;
+    if ((uint64_t)p->width * p->height > (1ull << 28)) return r->fail(WF_ERROR_CAPACITY, "texture too large");
+    const size_t texels = (size_t)p->width * p->height;
+    std::vector<float> rgba(texels * 4);
+    auto srgb_to_linear = [](float c) {   // common.hlsl:261-269
+        return c <= 0.04045f ? c / 12.92f : std::pow((c + 0.055f) / 1.055f, 2.4f);
+    };
+    switch (p->format) {
+    case WF_TEXTURE_FORMAT_RGBA8:
+    case WF_TEXTURE_FORMAT_RGBA8_SRGB: {
+        const uint8_t* s = (const uint8_t*)p->ptr_data;
+        for (size_t i = 0; i < texels * 4; ++i) {
+            float c = (float)s[i] / 255.0f;
+            if (p->format == WF_TEXTURE_FORMAT_RGBA8_SRGB && (i & 3) != 3) c = srgb_to_linear(c);
+            rgba[i] = c;
+        }
+        break;
+    }
+    case WF_TEXTURE_FORMAT_RGB32F: {
+        const float* s = (const float*)p->ptr_data;
+        for (size_t i = 0; i < texels; ++i) { rgba[4 * i] = s[3 * i]; rgba[4 * i + 1] = s[3 * i + 1]; rgba[4 * i + 2] = s[3 * i + 2]; rgba[4 * i + 3] = 1.0f; }
+        break;
+    }
+    case WF_TEXTURE_FORMAT_RGBA32F:
+        std::memcpy(rgba.data(), p->ptr_data, texels * 16);
+        break;
+    default:
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "unsupported texture format %u", p->format);
+    }
+    for (float v : rgba) if (!std::isfinite(v)) return r->fail(WF_ERROR_INVALID_ARGUMENT, "texture holds a non-finite value");
+    if (r->textures.size() >= kMaxTextures) return r->fail(WF_ERROR_CAPACITY, "too many textures");
+    Texture t;
+    t.alive = true;
+    t.version = 1;
+    t.width = p->width;
+    t.height = p->height;
+    hipError_t e = t.texels.ensure(texels * 16);
+    if (e == hipSuccess) e = hipMemcpy(t.texels.p, rgba.data(), texels * 16, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { t.texels.release(); return r->hip_fail(e, "texture upload"); }
+    r->textures.push_back(std::move(t));
+    r->textures_dirty = true;
+    *out = make_handle((uint32_t)r->textures.size() - 1, 1);
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_renderer_create_render_mesh(wf_renderer* r, const wf_render_mesh_params* p, wf_mesh_handle* out)
+{
+    if (!r || !p || !out || !p->vertices || !p->indices) return WF_ERROR_INVALID_ARGUMENT;
+    if (p->index_count == 0 || p->index_count % 3 != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "index_count must be a positive multiple of 3");
+    const uint32_t n = p->index_count / 3;
+    if (n > kMaxBlasTriangles) return r->fail(WF_ERROR_CAPACITY, "mesh has %u triangles (limit %u)", n, kMaxBlasTriangles);
+    for (uint32_t i = 0; i < p->index_count; ++i)
+        if (p->indices[i] >= p->vertex_count) return r->fail(WF_ERROR_INVALID_ARGUMENT, "index %u out of range", i);
+
+    // de-index into triangles (renderer.cpp:1251-1258)
+    Mesh m;
+    m.pos9.resize(9ull * n);
+    std::vector<GpuShadeTri> shade(n);
+    for (uint32_t t = 0; t < n; ++t) {
+        for (int v = 0; v < 3; ++v) {
+            const wf_vertex& vx = p->vertices[p->indices[3 * t + v]];
+            std::memcpy(&m.pos9[9ull * t + 3 * v], vx.position, 12);
+            std::memcpy(shade[t].p[v], vx.position, 12);
+            std::memcpy(shade[t].n[v], vx.normal, 12);
+            std::memcpy(shade[t].uv[v], vx.uv, 8);
+        }
+    }
+    BlasData b;
+    wf_status s = build_blas(m.pos9.data(), n, r->build_opts, b);
+    if (s != WF_OK) return r->fail(s, "BLAS build failed (non-finite vertex or bad options)");
+
+    // device layouts: nodes as built, triangles permuted into leaf order
+    std::vector<GpuTri> perm(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t ti = b.idx[i];
+        const float* q = &m.pos9[9ull * ti];
+        GpuTri& g = perm[i];
+        for (int k = 0; k < 3; ++k) {
+            g.v0[k] = q[k];
+            g.e1[k] = q[3 + k] - q[k];
+            g.e2[k] = q[6 + k] - q[k];
+        }
+        g.prim = ti;
+        g.pad0 = g.pad1 = 0;
+    }
+    hipError_t e = m.nodes.ensure(sizeof(wf_bvh_node) * b.node_count);
+    if (e == hipSuccess) e = m.tris.ensure(sizeof(GpuTri) * n);
+    if (e == hipSuccess) e = m.shade.ensure(sizeof(GpuShadeTri) * n);
+    if (e == hipSuccess) e = hipMemcpy(m.nodes.p, b.nodes.data(), sizeof(wf_bvh_node) * b.node_count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m.tris.p, perm.data(), sizeof(GpuTri) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m.shade.p, shade.data(), sizeof(GpuShadeTri) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { m.nodes.release(); m.tris.release(); m.shade.release(); return r->hip_fail(e, "mesh upload"); }
+    m.tri_count = n;
+    m.node_count = b.node_count;
+    m.max_leaf = b.max_leaf;
+    std::memcpy(m.blas_min, b.nodes[0].aabb_min, 12);   // renderer.cpp:107-111
+    std::memcpy(m.blas_max, b.nodes[0].aabb_max, 12);
+    m.alive = true;
+    m.version = 1;
+    r->meshes.push_back(std::move(m));
+    *out = make_handle((uint32_t)r->meshes.size() - 1, 1);
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_renderer_submit_render_mesh(wf_renderer* r, wf_mesh_handle mh, const float transform[16], const wf_material_asset* mat)
+{
+    if (!r || !transform || !mat) return WF_ERROR_INVALID_ARGUMENT;
+    Mesh* m = r->find_mesh(mh);
+    if (!m) return r->fail(WF_ERROR_NOT_FOUND, "mesh handle {index %u, version %u} is not valid", handle_index(mh), handle_version(mh));
+    if (r->instances.size() >= kMaxInstances) return r->fail(WF_ERROR_CAPACITY, "exceeded capacity of instances (%u)", kMaxInstances);
+    for (int i = 0; i < 16; ++i) if (!std::isfinite(transform[i])) return r->fail(WF_ERROR_INVALID_ARGUMENT, "non-finite transform");
+    InstanceRec rec;
+    rec.mesh = handle_index(mh);
+    std::memcpy(rec.transform, transform, 64);
+    rec.material = *mat;
+    // software instance record (renderer.cpp:1318-1336)
+    make_bvh_instance(transform, m->blas_min, m->blas_max, rec.mesh, rec.sw);
+    r->instances.push_back(rec);
+    r->tlas_valid = false;
+    return WF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// TLAS build + upload (renderer.cpp:637-670)
+// ---------------------------------------------------------------------------
+
+wf_status wf_renderer::upload_scene()
+{
+    if (tlas_valid) return WF_OK;
+    if (instances.empty()) return fail(WF_ERROR_INVALID_STATE, "no instances submitted");
+    if (instances.size() > kMaxTlasInstances) return fail(WF_ERROR_CAPACITY, "%zu instances exceed the TLAS limit %u", instances.size(), kMaxTlasInstances);
+    std::vector<wf_bvh_instance> sw(instances.size());
+    for (size_t i = 0; i < instances.size(); ++i) sw[i] = instances[i].sw;
+    TlasData t;
+    wf_status s = build_tlas(sw.data(), (uint32_t)sw.size(), t);
+    if (s != WF_OK) return fail(s, "TLAS build failed");
+    tlas_host = t;
+
+    std::vector<GpuInstance> gi(instances.size());
+    std::vector<GpuShadeInstance> si(instances.size());
+    for (size_t i = 0; i < instances.size(); ++i) {
+        const InstanceRec& rec = instances[i];
+        const Mesh& m = meshes[rec.mesh];
+        for (int c = 0; c < 4; ++c)
+            for (int rr = 0; rr < 3; ++rr) gi[i].m[c * 3 + rr] = rec.sw.world_to_local[c * 4 + rr];
+        gi[i].nodes = (const wf_bvh_node*)m.nodes.p;
+        gi[i].tris = (const GpuTri*)m.tris.p;
+        GpuShadeInstance& sh = si[i];
+        std::memset(&sh, 0, sizeof(sh));
+        std::memcpy(sh.local_to_world, rec.transform, 64);
+        for (int k = 0; k < 3; ++k) {
+            sh.base_color[k] = rec.material.base_color_factor[k];
+            sh.emissive[k] = rec.material.emissive_factor[k] * rec.material.emissive_strength;
+        }
+        const Texture* bt = find_texture(rec.material.base_color_texture);
+        const Texture* et = find_texture(rec.material.emissive_texture);
+        sh.base_color_tex = bt ? (int32_t)handle_index(rec.material.base_color_texture) : -1;
+        sh.emissive_tex = et ? (int32_t)handle_index(rec.material.emissive_texture) : -1;
+        sh.tris = (const GpuShadeTri*)m.shade.p;
+    }
+    hipError_t e = d_tnodes.ensure(sizeof(wf_tlas_node) * t.node_count);
+    if (e == hipSuccess) e = d_inst.ensure(sizeof(GpuInstance) * gi.size());
+    if (e == hipSuccess) e = d_shade_inst.ensure(sizeof(GpuShadeInstance) * si.size());
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tnodes.p, t.nodes.data(), sizeof(wf_tlas_node) * t.node_count, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_inst.p, gi.data(), sizeof(GpuInstance) * gi.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_shade_inst.p, si.data(), sizeof(GpuShadeInstance) * si.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "TLAS upload");
+    s = upload_materials_and_emitters();
+    if (s != WF_OK) return s;
+    tlas_valid = true;
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_renderer_build_tlas(wf_renderer* r)
+{
+    if (!r) return WF_ERROR_INVALID_ARGUMENT;
+    r->tlas_valid = false;
+    return r->upload_scene();
+}
+
+// ---------------------------------------------------------------------------
+// extend on caller rays
+// ---------------------------------------------------------------------------
+
+wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
+                                  uint32_t* d_visible, uint32_t* d_counters, bool any)
+{
+    hipError_t e = launch_fill_u32((uint32_t*)d_work.p, 0u, 16, stream);
+    if (e != hipSuccess) return hip_fail(e, "work counter reset");
+    ExtendLaunch L{};
+    L.scene.tnodes = (const wf_tlas_node*)d_tnodes.p;
+    L.scene.inst = (const GpuInstance*)d_inst.p;
+    L.rays = d_rays;
+    L.d_count = d_count;
+    L.n = n;
+    L.hits = d_hits;
+    L.visible = d_visible;
+    L.counters = d_counters;
+    L.work = (uint32_t*)d_work.p;
+    L.spill = (uint2*)d_spill.p;
+    L.spill_per_thread = spill_per_thread;
+    L.grid = ext_grid;
+    L.any_hit = any ? 1 : 0;
+    e = launch_extend(L, ext_cfg, stream);
+    if (e != hipSuccess) return hip_fail(e, "extend launch");
+    return WF_OK;
+}
+
+wf_status wf_renderer::read_overflows(uint32_t* out)
+{
+    hipError_t e = hipMemcpyAsync(out, (uint32_t*)d_work.p + 1, 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "overflow counter readback");
+    return WF_OK;
+}
+
+static wf_status trace_host(wf_renderer* r, const wf_ray* rays, uint32_t n, wf_hit* hits, uint32_t* visible, uint32_t* counters)
+{
+    if (!r || (n && !rays) || (n && !hits && !visible)) return WF_ERROR_INVALID_ARGUMENT;
+    wf_status s = r->upload_scene();
+    if (s != WF_OK) return s;
+    if (n == 0) return WF_OK;
+    const bool any = visible != nullptr;
+    const size_t out_bytes = any ? 4ull * n : sizeof(wf_hit) * (size_t)n;
+    hipError_t e = r->d_tmp_rays.ensure(sizeof(wf_ray) * (size_t)n);
+    if (e == hipSuccess) e = r->d_tmp_hits.ensure(out_bytes);
+    if (e == hipSuccess && counters) e = r->d_tmp_aux.ensure(16ull * n);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->d_tmp_rays.p, rays, sizeof(wf_ray) * (size_t)n, hipMemcpyHostToDevice, r->stream);
+    if (e != hipSuccess) return r->hip_fail(e, "ray upload");
+    s = r->run_extend((const wf_ray*)r->d_tmp_rays.p, nullptr, n, any ? nullptr : (wf_hit*)r->d_tmp_hits.p,
+                      any ? (uint32_t*)r->d_tmp_hits.p : nullptr, counters ? (uint32_t*)r->d_tmp_aux.p : nullptr, any);
+    if (s != WF_OK) return s;
+    e = hipMemcpyAsync(any ? (void*)visible : (void*)hits, r->d_tmp_hits.p, out_bytes, hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess && counters) e = hipMemcpyAsync(counters, r->d_tmp_aux.p, 16ull * n, hipMemcpyDeviceToHost, r->stream);
+    if (e != hipSuccess) return r->hip_fail(e, "hit readback");
+    uint32_t ov = 0;
+    s = r->read_overflows(&ov);
+    if (s != WF_OK) return s;
+    if (ov) return r->fail(WF_ERROR_STACK_OVERFLOW, "%u rays overflowed the traversal stack", ov);
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_trace_closest(wf_renderer* r, const wf_ray* rays, uint32_t n, wf_hit* hits)
+{
+    if (!hits && n) return WF_ERROR_INVALID_ARGUMENT;
+    return trace_host(r, rays, n, hits, nullptr, nullptr);
+}
+
+extern "C" WF_API wf_status wf_trace_occluded(wf_renderer* r, const wf_ray* rays, uint32_t n, uint32_t* visible)
+{
+    if (!visible && n) return WF_ERROR_INVALID_ARGUMENT;
+    return trace_host(r, rays, n, nullptr, visible, nullptr);
+}
+
+extern "C" WF_API wf_status wf_trace_closest_counters(wf_renderer* r, const wf_ray* rays, uint32_t n, wf_hit* hits, uint32_t* counters)
+{
+    if ((!hits || !counters) && n) return WF_ERROR_INVALID_ARGUMENT;
+    return trace_host(r, rays, n, hits, nullptr, counters);
+}
+
+extern "C" WF_API wf_status wf_trace_closest_device(wf_renderer* r, const wf_ray* d_rays, uint32_t n, wf_hit* d_hits)
+{
+    if (!r || (n && (!d_rays || !d_hits))) return WF_ERROR_INVALID_ARGUMENT;
+    wf_status s = r->upload_scene();
+    if (s != WF_OK) return s;
+    if (n == 0) return WF_OK;
+    return r->run_extend(d_rays, nullptr, n, d_hits, nullptr, nullptr, false);
+}

@@ -1,0 +1,85 @@
+// Internal declarations shared by the host runtime and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "wf_host.h"
+
+namespace wf {
+
+// ---- device layouts ------------------------------------------------------------
+// Triangle in leaf order (pre-permuted through the index array), with the two
+// edges precomputed in FP32 (v1-v0 and v2-v0 are exactly what the
+// intersection test computes first, intersect.hlsl:9-10): three 16-byte rows.
+struct alignas(16) GpuTri {
+    float v0[3]; uint32_t prim;    // prim = original triangle id (indices[i])
+    float e1[3]; uint32_t pad0;
+    float e2[3]; uint32_t pad1;
+};
+static_assert(sizeof(GpuTri) == 48, "GpuTri is 48 B");
+
+// Instance record read at every TLAS leaf: rows 0..2 of world_to_local
+// (column-major, m[c*3+r]) and the mesh's device arrays.
+struct alignas(16) GpuInstance {
+    float m[12];
+    const wf_bvh_node* nodes;
+    const GpuTri* tris;
+};
+static_assert(sizeof(GpuInstance) == 64, "GpuInstance is 64 B");
+
+// Per-triangle shading data (triangle_t without tangents): positions,
+// normals and uvs of the three vertices.
+struct GpuShadeTri {
+    float p[3][3];
+    float n[3][3];
+    float uv[3][2];
+};
+static_assert(sizeof(GpuShadeTri) == 96, "GpuShadeTri is 96 B");
+
+// Per-instance shading record (instance_data_t, shared.hlsl.h:87-95).
+struct alignas(16) GpuShadeInstance {
+    float local_to_world[16];
+    float base_color[3]; int32_t base_color_tex;   // texture slot, -1 = none
+    float emissive[3];   int32_t emissive_tex;     // emissive_factor * strength
+    const GpuShadeTri* tris;
+    uint64_t pad;
+};
+static_assert(sizeof(GpuShadeInstance) == 112, "GpuShadeInstance is 112 B");
+
+struct GpuTexture {
+    const float* texels;   // RGBA float, row-major
+    uint32_t width, height;
+};
+
+struct TraceScene {
+    const wf_tlas_node* tnodes;
+    const GpuInstance* inst;
+};
+
+// Traversal configuration (the autotuner's axes).
+struct ExtendConfig {
+    uint32_t block = 128;       // workgroup size: 64, 128, 256
+    uint32_t lds_stack = 16;    // LDS stack entries per lane (0 = private/scratch stack)
+    uint32_t skip_popped = 1;   // skip popped interior nodes whose entry distance >= t
+};
+
+// Launch the closest-hit kernel.  d_count (device) overrides n when non-null.
+struct ExtendLaunch {
+    TraceScene scene;
+    const wf_ray* rays;
+    const uint32_t* d_count;
+    uint32_t n;
+    wf_hit* hits;              // AoS output (may be null when hit_* SoA used)
+    uint32_t* visible;         // occlusion output (any-hit mode)
+    uint32_t* counters;        // 4 x u32 per ray, or null
+    uint32_t* work;            // [0] next ray, [1] stack overflows
+    uint2* spill;              // global overflow stack
+    uint32_t spill_per_thread;
+    uint32_t grid;             // persistent blocks
+    int any_hit;
+};
+hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStream_t stream);
+hipError_t extend_occupancy(const ExtendConfig& cfg, int* blocks_per_cu);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t stream);
+
+} // namespace wf
