@@ -133,6 +133,9 @@ typedef struct wf_frame_stats {
 WF_API wf_status wf_renderer_init(const wf_init_params* params, wf_renderer** out_renderer);
 WF_API wf_status wf_renderer_exit(wf_renderer* r);
 WF_API const char* wf_renderer_last_error(const wf_renderer* r);
+/* The hipStream_t all of the renderer's GPU work is ordered on (for callers
+ * that record events or enqueue work around it). */
+WF_API wf_status wf_renderer_get_stream(const wf_renderer* r, void** out_stream);
 
 /* ---- frame / scene: renderer::begin_frame .. end_scene (renderer.cpp:543-1055) */
 WF_API wf_status wf_renderer_begin_frame(wf_renderer* r);
@@ -152,6 +155,17 @@ WF_API wf_status wf_renderer_get_settings(const wf_renderer* r, wf_render_settin
 WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_render_settings* settings);
 /* options for meshes created after the call (renderer.cpp:93-96) */
 WF_API wf_status wf_renderer_set_build_options(wf_renderer* r, const wf_build_options* options);
+
+/* Traversal kernel configuration (no reference counterpart: the autotuner's
+ * axes).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
+ * scratch), 8, 16 or 32 (LDS, spilling to global memory past that depth);
+ * skip_popped_interior drops popped interior nodes whose entry distance is
+ * not below the current hit distance; bvh_layout 0 = BVH2. */
+typedef struct wf_traversal_config {
+    uint32_t workgroup_size; uint32_t lds_stack_entries; uint32_t skip_popped_interior; uint32_t bvh_layout;
+} wf_traversal_config;
+WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, const wf_traversal_config* config);
+WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* r, wf_traversal_config* out);
 
 /* ---- camera::create (core/camera/camera.cpp:8-16): lookAtLH, up = +y ---- */
 WF_API wf_status wf_camera_create(wf_camera* out, const float view_origin[3], const float target_pos[3], float vfov_deg);
@@ -174,6 +188,8 @@ WF_API wf_status wf_renderer_get_frame_stats(const wf_renderer* r, wf_frame_stat
 /* Builds the TLAS from the instances submitted since begin_scene and uploads
  * it; render() does this itself (renderer.cpp:637-670). */
 WF_API wf_status wf_renderer_build_tlas(wf_renderer* r);
+/* World bounds of the built TLAS (its root node). */
+WF_API wf_status wf_renderer_get_scene_bounds(const wf_renderer* r, float out_min[3], float out_max[3]);
 /* Closest hit for n rays (host arrays).  Returns WF_ERROR_STACK_OVERFLOW if
  * any ray overflowed its traversal stack (that ray reports a miss). */
 WF_API wf_status wf_trace_closest(wf_renderer* r, const wf_ray* rays, uint32_t n, wf_hit* hits);

@@ -102,25 +102,26 @@ void orc_camera_view(const float eye[3], const float target[3], float vfov_deg,
 void orc_primary_rays(const orc_view* v, uint32_t width, uint32_t height, orc_ray* rays);
 
 /* ---- wavefront render (generate/extend/shade/connect) -------------------- */
+/* Per-instance shading record (instance_data_t, shared.hlsl.h:87-95). */
 typedef struct {
     float    local_to_world[16];
     float    base_color[3];
     float    emissive[3];          /* emissive_factor * emissive_strength */
     int32_t  base_color_tex;       /* -1: none (white) */
-    uint32_t blas;                 /* mesh id */
+    uint32_t mesh;
 } orc_material_instance;
 
 typedef struct {
-    const float* verts;            /* per mesh triangle_t data: 3 vertices x (pos3, nrm3, uv2) = 24 floats per tri */
-    const uint32_t* vert_off;      /* per mesh offset in triangles */
+    const float*    shade_tris;    /* all meshes: 24 floats per triangle: p0 p1 p2 (9), n0 n1 n2 (9), uv0 uv1 uv2 (6) */
+    const uint32_t* shade_off;     /* per mesh, first triangle */
+    const uint32_t* tri_count;     /* per mesh */
     const orc_material_instance* mi;   /* per instance */
-    const float* tex;              /* RGB float textures, concatenated */
+    uint32_t        n_inst;
+    const float*    tex_rgba;      /* RGBA float textures, concatenated */
     const uint32_t* tex_off;       /* per texture, offset in floats */
-    const uint32_t* tex_dim;       /* per texture (w, h) */
-    const uint32_t* emitters;      /* (instance, primitive) pairs */
-    uint32_t n_emitters;
-    float    emitter_area_total;   /* sum of world-space emitter areas */
-    const float* emitter_cdf;      /* n_emitters, cumulative area / total */
+    const uint32_t* tex_w;
+    const uint32_t* tex_h;
+    float           env[3];        /* constant environment radiance (hdr_env_strength x white) */
 } orc_shading;
 
 typedef struct {
@@ -129,15 +130,24 @@ typedef struct {
     uint32_t rr_start;             /* bounce index at which Russian roulette starts */
     uint32_t nee;                  /* 1: next-event estimation through connect */
     uint32_t frame_seed;
-    float    rr_survive;
+    float    rr_survival;
 } orc_render_params;
 
+/* energy_rgba: width*height*4 floats (the resolved per-pixel mean over spp). */
 int orc_render(const orc_scene* s, const orc_shading* sh, const orc_view* v, uint32_t width, uint32_t height,
-               const orc_render_params* p, float* energy_rgb, int nthreads);
+               const orc_render_params* p, float* energy_rgba, int nthreads);
 
-uint32_t orc_pcg_stream_first(uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t frame_seed);
+/* The emitter table next-event estimation samples: per emissive triangle
+ * (instance, primitive) and the cumulative area fraction; returns the count
+ * (call with NULL outputs for the count) and the total world area. */
+uint32_t orc_emitters(const orc_shading* sh, uint32_t* ids, float* cdf, float* total_area);
+
+/* RNG and trigonometry used by shade (exposed for known-answer tests). */
+uint32_t orc_pcg32_first(uint64_t initstate, uint64_t initseq);
+void     orc_path_rng(uint32_t path, uint32_t bounce, uint32_t frame_seed, float out[6]);
 float    orc_sin_turns(float u);
 float    orc_cos_turns(float u);
+void     orc_cosine_sample(const float n[3], float r0, float r1, float out[3]);
 
 #ifdef __cplusplus
 }

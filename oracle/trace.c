@@ -187,6 +187,11 @@ static int trace_one(const orc_scene* s, const orc_ray* r, orc_hit* h, orc_count
     return ORC_OK;
 }
 
+int orc_trace_one(const orc_scene* s, const orc_ray* r, orc_hit* h, int any)
+{
+    return trace_one(s, r, h, NULL, any);
+}
+
 typedef struct {
     const orc_scene* s; const orc_ray* rays; orc_hit* hits; orc_counters* cnt; uint32_t* vis;
     uint32_t n; int tid, nt; int any; int rc;

@@ -30,6 +30,22 @@ class OrcView(C.Structure):
                 ("clip_to_view", C.c_float * 16), ("render_dim", C.c_float * 2), ("near_plane", C.c_float), ("far_plane", C.c_float)]
 
 
+class OrcMaterialInstance(C.Structure):
+    _fields_ = [("local_to_world", C.c_float * 16), ("base_color", C.c_float * 3), ("emissive", C.c_float * 3),
+                ("base_color_tex", C.c_int32), ("mesh", C.c_uint32)]
+
+
+class OrcShading(C.Structure):
+    _fields_ = [("shade_tris", C.c_void_p), ("shade_off", C.c_void_p), ("tri_count", C.c_void_p), ("mi", C.c_void_p),
+                ("n_inst", C.c_uint32), ("tex_rgba", C.c_void_p), ("tex_off", C.c_void_p), ("tex_w", C.c_void_p),
+                ("tex_h", C.c_void_p), ("env", C.c_float * 3)]
+
+
+class OrcRenderParams(C.Structure):
+    _fields_ = [("max_bounces", C.c_uint32), ("spp", C.c_uint32), ("rr_start", C.c_uint32), ("nee", C.c_uint32),
+                ("frame_seed", C.c_uint32), ("rr_survival", C.c_float)]
+
+
 _o = None
 
 
@@ -53,6 +69,14 @@ def olib():
             "orc_brute_closest": (C.c_int, [C.POINTER(OrcScene), vp, u32, vp]),
             "orc_camera_view": (None, [vp, vp, f32, u32, u32, C.POINTER(OrcView)]),
             "orc_primary_rays": (None, [C.POINTER(OrcView), u32, u32, vp]),
+            "orc_render": (C.c_int, [C.POINTER(OrcScene), C.POINTER(OrcShading), C.POINTER(OrcView), u32, u32,
+                                     C.POINTER(OrcRenderParams), vp, C.c_int]),
+            "orc_emitters": (u32, [C.POINTER(OrcShading), vp, vp, C.POINTER(f32)]),
+            "orc_pcg32_first": (u32, [C.c_uint64, C.c_uint64]),
+            "orc_path_rng": (None, [u32, u32, u32, vp]),
+            "orc_sin_turns": (f32, [f32]),
+            "orc_cos_turns": (f32, [f32]),
+            "orc_cosine_sample": (None, [vp, f32, f32, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(o, name)
@@ -177,6 +201,60 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError(f"orc_trace_any rc={rc}")
         return vis
+
+    def shading(self, env: float = 1.0):
+        """The oracle's shading inputs for the scene (kept alive on self)."""
+        sc = self.scene
+        tris, off, cnt = [], [], []
+        at = 0
+        for v, i in sc.meshes:
+            tri = i.reshape(-1, 3)
+            t = np.concatenate([v["position"][tri].reshape(-1, 9), v["normal"][tri].reshape(-1, 9),
+                                v["uv"][tri].reshape(-1, 6)], axis=1).astype(np.float32)
+            tris.append(t)
+            off.append(at)
+            cnt.append(len(t))
+            at += len(t)
+        self._sh_tris = np.ascontiguousarray(np.concatenate(tris))
+        self._sh_off = np.array(off, dtype=np.uint32)
+        self._sh_cnt = np.array(cnt, dtype=np.uint32)
+        mi = (OrcMaterialInstance * len(sc.instances))()
+        for k, ins in enumerate(sc.instances):
+            mi[k].local_to_world[:] = [float(x) for x in np.asarray(ins.transform, np.float32)]
+            mi[k].base_color[:] = [float(np.float32(x)) for x in ins.base_color]
+            mi[k].emissive[:] = [float(np.float32(np.float32(e) * np.float32(ins.emissive_strength))) for e in ins.emissive]
+            mi[k].base_color_tex = ins.texture
+            mi[k].mesh = ins.mesh
+        self._mi = mi
+        texs, toff, tw, th = [], [], [], []
+        at = 0
+        for (w, h, rgb) in sc.textures:
+            rgba = np.concatenate([rgb.reshape(-1, 3), np.ones((w * h, 1), np.float32)], axis=1).astype(np.float32)
+            texs.append(rgba.reshape(-1))
+            toff.append(at)
+            tw.append(w)
+            th.append(h)
+            at += rgba.size
+        self._tex = np.ascontiguousarray(np.concatenate(texs)) if texs else np.zeros(4, np.float32)
+        self._toff = np.array(toff or [0], dtype=np.uint32)
+        self._tw = np.array(tw or [1], dtype=np.uint32)
+        self._th = np.array(th or [1], dtype=np.uint32)
+        sh = OrcShading(p(self._sh_tris), p(self._sh_off), p(self._sh_cnt), C.cast(mi, C.c_void_p), len(sc.instances),
+                        p(self._tex), p(self._toff), p(self._tw), p(self._th))
+        sh.env[:] = [float(np.float32(env))] * 3
+        self._sh = sh
+        return sh
+
+    def render(self, max_bounces=3, spp=1, rr_start=3, rr_survival=0.75, nee=0, frame_seed=1, env=1.0, nthreads=8):
+        sc = self.scene
+        sh = self.shading(env)
+        view = camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
+        prm = OrcRenderParams(max_bounces, spp, rr_start, nee, frame_seed, rr_survival)
+        out = np.zeros((sc.height, sc.width, 4), dtype=np.float32)
+        rc = olib().orc_render(C.byref(self.s), C.byref(sh), C.byref(view), sc.width, sc.height, C.byref(prm), p(out), nthreads)
+        if rc != 0:
+            raise RuntimeError(f"orc_render rc={rc}")
+        return out
 
     def brute_closest(self, rays: np.ndarray) -> np.ndarray:
         rays = np.ascontiguousarray(rays, dtype=L.RAY)

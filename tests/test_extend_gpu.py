@@ -1,0 +1,205 @@
+"""GPU parity of the extend stage (closest hit / occlusion) against the oracle.
+
+Bar: hit_result_t bit-exact (instance id, primitive id, t and barycentrics as
+raw 32-bit words) for every ray; traversal counters equal the oracle's.
+"""
+import numpy as np
+import pytest
+
+from wavefrontpathtracing_amd import scenes
+from wavefrontpathtracing_amd import _lib as L
+from wavefrontpathtracing_amd.renderer import Renderer, WfError
+
+import oracle_api as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(scene, bins=8):
+    r = Renderer(scene.width, scene.height)
+    r.set_build_options(bins)
+    scenes.load(r, scene)
+    r.build_tlas()
+    return r
+
+
+def _assert_bitexact(gpu, ref, what):
+    bad = O.hits_equal(gpu, ref)
+    assert len(bad) == 0, f"{what}: {len(bad)} of {len(ref)} hits differ; first {bad[:5]}: gpu={gpu[bad[:3]]} ref={ref[bad[:3]]}"
+
+
+@pytest.fixture(scope="module")
+def soup():
+    sc = scenes.soup_scene(65536, seed=1)
+    r = _setup(sc, bins=16)
+    orc = O.OracleScene(sc, bins=16)
+    yield sc, r, orc
+    r.exit()
+
+
+@pytest.fixture(scope="module")
+def small():
+    sc = scenes.small_tlas_scene()
+    r = _setup(sc)
+    orc = O.OracleScene(sc)
+    yield sc, r, orc
+    r.exit()
+
+
+def test_soup_primary_rays_bitexact(soup):
+    """65,536-triangle soup, 16 bins, 512x512 primary rays, closest hit."""
+    sc, r, orc = soup
+    view = O.camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
+    rays = O.primary_rays(view, sc.width, sc.height)
+    ref = orc.trace_closest(rays, nthreads=8)
+    assert (ref["instance_idx"] != L.MISS).mean() > 0.3
+    _assert_bitexact(r.trace_closest(rays), ref, "soup primary")
+
+
+def test_soup_counters_match_oracle(soup):
+    sc, r, orc = soup
+    rays = scenes.sphere_rays(31, (-1.5, -1.5, -1.5), (1.5, 1.5, 1.5), 20000)
+    ref, cnt = orc.trace_closest(rays, nthreads=8, counters=True)
+    hits, gcnt = r.trace_closest_counters(rays)
+    _assert_bitexact(hits, ref, "soup sphere rays")
+    ref_c = np.stack([cnt[f] for f in cnt.dtype.names], axis=1)
+    mism = np.nonzero((gcnt != ref_c).any(axis=1))[0]
+    assert len(mism) == 0, f"counters differ on {len(mism)} rays, first {mism[:3]}: gpu {gcnt[mism[:3]]} ref {ref_c[mism[:3]]}"
+
+
+def test_tlas_small_primary_and_incoherent(small):
+    sc, r, orc = small
+    view = O.camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
+    rays = O.primary_rays(view, sc.width, sc.height)
+    _assert_bitexact(r.trace_closest(rays), orc.trace_closest(rays, nthreads=8), "tlas primary")
+    bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
+    sr = scenes.sphere_rays(3, bmin, bmax, 50000)
+    ref, cnt = orc.trace_closest(sr, nthreads=8, counters=True)
+    hits, gcnt = r.trace_closest_counters(sr)
+    _assert_bitexact(hits, ref, "tlas sphere rays")
+    ref_c = np.stack([cnt[f] for f in cnt.dtype.names], axis=1)
+    assert np.array_equal(gcnt, ref_c)
+
+
+def test_tlas_occlusion_matches_oracle(small):
+    sc, r, orc = small
+    bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
+    sr = scenes.sphere_rays(5, bmin, bmax, 30000)
+    # finite segments: half the rays end at 3 units
+    sr["tmax"][::2] = 3.0
+    assert np.array_equal(r.trace_occluded(sr), orc.trace_any(sr, nthreads=8))
+
+
+def test_tmax_limits_hits(small):
+    sc, r, orc = small
+    bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
+    sr = scenes.sphere_rays(7, bmin, bmax, 20000)
+    sr["tmax"] = np.float32(2.0)
+    ref = orc.trace_closest(sr, nthreads=8)
+    got = r.trace_closest(sr)
+    _assert_bitexact(got, ref, "tmax 2")
+    hit = got["instance_idx"] != L.MISS
+    assert np.all(got["t"][hit] < 2.0)
+
+
+def test_empty_and_degenerate_rays(small):
+    sc, r, orc = small
+    assert len(r.trace_closest(np.zeros(0, dtype=L.RAY))) == 0
+    rays = np.zeros(8, dtype=L.RAY)
+    rays["tmax"] = 3.4028235e38
+    rays["direction"][0] = (0.0, 0.0, 0.0)                 # zero direction: inv_dir = inf
+    rays["direction"][1] = (np.nan, 0.0, 1.0)               # NaN component
+    rays["direction"][2] = (0.0, -1.0, 0.0)                 # axis-parallel
+    rays["origin"][2] = (0.0, 20.0, 0.0)
+    rays["direction"][3] = (1.0, 0.0, 0.0)
+    rays["origin"][3] = orc.tnodes[0]["aabb_min"]           # origin on the root box corner
+    rays["direction"][4:] = (0.0, 0.0, 1.0)
+    rays["origin"][4:] = [(0, 0, -30), (1e30, 0, 0), (0, 0, 0), (-0.0, -0.0, -0.0)]
+    _assert_bitexact(r.trace_closest(rays), orc.trace_closest(rays), "degenerate rays")
+
+
+def test_single_triangle_single_instance():
+    v = np.zeros(3, dtype=L.VERTEX)
+    v["position"] = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    sc = scenes.Scene(meshes=[(v, np.arange(3, dtype=np.uint32))], instances=[scenes.Instance(0, scenes.identity())],
+                      eye=(0.25, 0.25, 1.0), target=(0.25, 0.25, 0.0), vfov_deg=60.0, width=8, height=8)
+    r = _setup(sc)
+    try:
+        rays = np.zeros(3, dtype=L.RAY)
+        rays["origin"] = [(0.25, 0.25, 1.0), (0.25, 0.25, -1.0), (2.0, 2.0, 1.0)]
+        rays["direction"] = [(0, 0, -1), (0, 0, 1), (0, 0, -1)]
+        rays["tmax"] = 3.4028235e38
+        h = r.trace_closest(rays)
+        # front face hit at t = 1, bary (0.25, 0.25); back face culled; outside misses
+        assert h[0]["instance_idx"] == 0 and h[0]["primitive_idx"] == 0
+        assert h[0]["t"] == np.float32(1.0) and tuple(h[0]["bary"]) == (0.25, 0.25)
+        assert h[1]["instance_idx"] == L.MISS and h[1]["t"] == np.float32(3.4028235e38)
+        assert h[2]["instance_idx"] == L.MISS
+        _assert_bitexact(h, O.OracleScene(sc).trace_closest(rays), "single triangle")
+    finally:
+        r.exit()
+
+
+def test_stack_overflow_reports_status():
+    """A 1-D chain of geometrically shrinking triangles built with
+    subdivide_single_prim peels one triangle off per level (a BVH ~240 deep);
+    a ray along the chain pushes one far child per level and must end with
+    WF_ERROR_STACK_OVERFLOW (stack capacity 128 entries per lane), not a fault."""
+    n = 240
+    v = np.zeros(3 * n, dtype=L.VERTEX)
+    for k in range(n):
+        x = np.float32(2.0) ** np.float32(-k * 0.5)
+        s = x * np.float32(0.01)
+        v["position"][3 * k: 3 * k + 3] = [(x, -s, 0.0), (x, s, s), (x, s, -s)]
+    sc = scenes.Scene(meshes=[(v, np.arange(3 * n, dtype=np.uint32))], instances=[scenes.Instance(0, scenes.identity())],
+                      eye=(2, 0, 0), target=(0, 0, 0), vfov_deg=60.0, width=8, height=8)
+    r = Renderer(8, 8)
+    try:
+        r.set_build_options(8, subdivide_single_prim=True)
+        scenes.load(r, sc)
+        r.build_tlas()
+        rays = np.zeros(64, dtype=L.RAY)
+        rays["origin"] = (-1.0, 0.0, 0.0)
+        rays["direction"] = (1.0, 0.0, 0.0)
+        rays["tmax"] = 3.4028235e38
+        with pytest.raises(WfError) as ei:
+            r.trace_closest(rays)
+        assert ei.value.status == 5
+        # a ray that misses the chain is unaffected
+        ok = rays.copy()
+        ok["origin"] = (-1.0, 5.0, 0.0)
+        assert np.all(r.trace_closest(ok)["instance_idx"] == L.MISS)
+    finally:
+        r.exit()
+
+
+@pytest.mark.parametrize("block,lds,skip", [(64, 8, 1), (128, 0, 1), (256, 32, 0), (128, 16, 0)])
+def test_traversal_configs_agree(small, block, lds, skip):
+    sc, r, orc = small
+    from wavefrontpathtracing_amd import tuning
+    bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
+    sr = scenes.sphere_rays(11, bmin, bmax, 20000)
+    ref = orc.trace_closest(sr, nthreads=8)
+    tuning.set_extend_config(r, block=block, lds_stack=lds, skip_popped=skip)
+    try:
+        _assert_bitexact(r.trace_closest(sr), ref, f"config {block}/{lds}/{skip}")
+    finally:
+        tuning.set_extend_config(r)
+
+
+@pytest.mark.slow
+def test_full_tlas_scene_subset():
+    """The 256-instance scene at full size: a strided subset of the 1920x1080
+    primary rays plus incoherent rays, bit-exact."""
+    sc = scenes.tlas_scene()
+    r = _setup(sc)
+    try:
+        orc = O.OracleScene(sc)
+        view = O.camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
+        rays = O.primary_rays(view, sc.width, sc.height)[::37]
+        _assert_bitexact(r.trace_closest(rays), orc.trace_closest(rays, nthreads=16), "full tlas primary subset")
+        bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
+        sr = scenes.sphere_rays(3, bmin, bmax, 30000)
+        _assert_bitexact(r.trace_closest(sr), orc.trace_closest(sr, nthreads=16), "full tlas sphere subset")
+    finally:
+        r.exit()

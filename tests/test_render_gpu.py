@@ -1,0 +1,92 @@
+"""GPU parity of the whole wavefront frame (clear, generate, extend, shade,
+connect, resolve) against the oracle render loop.
+
+Bar: relative L1 error of the linear energy image <= 1e-3 (the tolerance of
+the image comparison); the per-path arithmetic is the same operation
+sequence on both sides, so most pixels are expected to agree bit for bit.
+"""
+import numpy as np
+import pytest
+
+from wavefrontpathtracing_amd import Renderer, scenes
+from wavefrontpathtracing_amd import _lib as L
+
+import oracle_api as O
+from test_oracle_render import furnace_scene
+
+pytestmark = pytest.mark.gpu
+
+REL_L1_TOL = 1e-3
+
+
+def render_gpu(sc, max_bounces=3, spp=1, nee=0, env=1.0, seed=1, rr_start=3, rr_survival=0.75, frames=1):
+    r = Renderer(sc.width, sc.height)
+    try:
+        scenes.load(r, sc)
+        r.update_settings(max_bounces=max_bounces, spp=spp, next_event_estimation=nee, hdr_env_strength=env,
+                          frame_seed=seed, rr_start_bounce=rr_start, rr_survival=rr_survival, accumulate=1)
+        for _ in range(frames):
+            r.render()
+        return r.read_energy(), r.frame_stats(), r.read_accumulator(), r.read_final_color()
+    finally:
+        r.exit()
+
+
+def rel_l1(a, b):
+    a = a[..., :3].astype(np.float64)
+    b = b[..., :3].astype(np.float64)
+    return np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-30)
+
+
+@pytest.mark.parametrize("nee,env", [(0, 1.0), (1, 0.0), (1, 0.5)])
+def test_small_tlas_frame_matches_oracle(nee, env):
+    sc = scenes.small_tlas_scene()
+    sc.eye = (0.0, 9.0, -14.0)
+    gpu, stats, _, _ = render_gpu(sc, max_bounces=5, spp=4, nee=nee, env=env, seed=3)
+    ref = O.OracleScene(sc).render(max_bounces=5, spp=4, nee=nee, env=env, frame_seed=3, nthreads=8)
+    err = rel_l1(gpu, ref)
+    same = np.mean(np.all(gpu[..., :3] == ref[..., :3], axis=-1))
+    assert err <= REL_L1_TOL, f"relative L1 {err:.3e}, bit-identical pixels {same:.4f}"
+    assert same > 0.99, f"bit-identical pixels {same:.4f} (relative L1 {err:.3e})"
+    assert stats.rays_extended >= sc.width * sc.height * 4
+    assert (stats.shadow_rays > 0) == bool(nee)
+
+
+def test_textured_frame_matches_oracle():
+    sc = scenes.tlas_scene(seed=5, width=96, height=64, n_instances=32, hf_quads=16, ico_subdiv=2, soup_tris=500, texture=True)
+    sc.textures = [(64, 48, scenes.value_noise_texture(8, 64, 48, cells=8))]
+    sc.eye = (0.0, 9.0, -14.0)
+    gpu, _, _, _ = render_gpu(sc, max_bounces=4, spp=2, nee=1, env=0.2, seed=9)
+    ref = O.OracleScene(sc).render(max_bounces=4, spp=2, nee=1, env=0.2, frame_seed=9, nthreads=8)
+    assert rel_l1(gpu, ref) <= REL_L1_TOL
+
+
+def test_furnace_on_gpu():
+    sc = furnace_scene()
+    gpu, _, _, _ = render_gpu(sc, max_bounces=4, spp=2, nee=1, env=1.5, seed=4)
+    orc = O.OracleScene(sc)
+    view = O.camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
+    hit = (orc.trace_closest(O.primary_rays(view, sc.width, sc.height))["instance_idx"] != L.MISS).reshape(sc.height, sc.width)
+    expect = np.float32(1.5) * np.array([0.6, 0.5, 0.4], np.float32)
+    assert np.allclose(gpu[hit][:, :3], expect, rtol=2e-6, atol=0)
+    assert np.all(gpu[~hit][:, :3] == np.float32(1.5))
+
+
+def test_frame_is_deterministic_and_accumulates():
+    sc = scenes.small_tlas_scene(width=64, height=48)
+    a = render_gpu(sc, max_bounces=4, spp=2, nee=1, env=0.3, seed=11)
+    b = render_gpu(sc, max_bounces=4, spp=2, nee=1, env=0.3, seed=11)
+    assert np.array_equal(a[0], b[0])
+    # the accumulator of a single frame equals that frame's energy (weight 1)
+    assert np.array_equal(a[2][..., :3], a[0][..., :3])
+    assert a[3].dtype == np.uint8 and a[3][..., 3].min() == 255
+
+
+def test_render_without_scene_is_an_error():
+    r = Renderer(16, 16)
+    try:
+        with pytest.raises(L.WfError) as e:
+            r.render()
+        assert e.value.status == 6
+    finally:
+        r.exit()

@@ -87,6 +87,11 @@ class FrameStats(C.Structure):
                 ("rays_extended", C.c_uint32), ("shadow_rays", C.c_uint32), ("stack_overflows", C.c_uint32)]
 
 
+class TraversalConfig(C.Structure):
+    _fields_ = [("workgroup_size", C.c_uint32), ("lds_stack_entries", C.c_uint32), ("skip_popped_interior", C.c_uint32),
+                ("bvh_layout", C.c_uint32)]
+
+
 class Pcg32(C.Structure):
     _fields_ = [("state", C.c_uint64), ("inc", C.c_uint64)]
 
@@ -102,6 +107,7 @@ SIGNATURES = {
     "wf_renderer_init": (_st, [C.POINTER(InitParams), C.POINTER(_vp)]),
     "wf_renderer_exit": (_st, [_vp]),
     "wf_renderer_last_error": (C.c_char_p, [_vp]),
+    "wf_renderer_get_stream": (_st, [_vp, C.POINTER(_vp)]),
     "wf_renderer_begin_frame": (_st, [_vp]),
     "wf_renderer_end_frame": (_st, [_vp]),
     "wf_renderer_begin_scene": (_st, [_vp, C.POINTER(Camera), _u64]),
@@ -113,6 +119,8 @@ SIGNATURES = {
     "wf_renderer_get_settings": (_st, [_vp, C.POINTER(RenderSettings)]),
     "wf_renderer_set_settings": (_st, [_vp, C.POINTER(RenderSettings)]),
     "wf_renderer_set_build_options": (_st, [_vp, C.POINTER(BuildOptions)]),
+    "wf_renderer_set_traversal_config": (_st, [_vp, C.POINTER(TraversalConfig)]),
+    "wf_renderer_get_traversal_config": (_st, [_vp, C.POINTER(TraversalConfig)]),
     "wf_camera_create": (_st, [C.POINTER(Camera), _vp, _vp, _f32]),
     "wf_renderer_get_view": (_st, [_vp, C.POINTER(View)]),
     "wf_make_view": (_st, [C.POINTER(Camera), _u32, _u32, C.POINTER(View)]),
@@ -121,6 +129,7 @@ SIGNATURES = {
     "wf_renderer_read_final_color": (_st, [_vp, _vp]),
     "wf_renderer_get_frame_stats": (_st, [_vp, C.POINTER(FrameStats)]),
     "wf_renderer_build_tlas": (_st, [_vp]),
+    "wf_renderer_get_scene_bounds": (_st, [_vp, _vp, _vp]),
     "wf_trace_closest": (_st, [_vp, _vp, _u32, _vp]),
     "wf_trace_closest_device": (_st, [_vp, _vp, _u32, _vp]),
     "wf_trace_occluded": (_st, [_vp, _vp, _u32, _vp]),

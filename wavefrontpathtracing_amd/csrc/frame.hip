@@ -1,6 +1,7 @@
 // Frame / scene lifecycle (renderer.cpp:543-630, 1051-1055) and the
 // material / emitter tables the shade and connect stages read.
 #include "renderer_impl.h"
+#include "stages.h"
 
 #include <cfloat>
 #include <cmath>
@@ -70,8 +71,148 @@ void wf_renderer::release_frame_buffers()
 {
     for (auto& b : d_rays) b.release();
     for (auto& b : d_paths) b.release();
-    d_hits.release(); d_shadow.release(); d_counts.release(); d_energy.release(); d_accum.release(); d_final.release();
+    d_hits.release(); d_thr.release(); d_path_energy.release(); d_counts.release();
+    d_shadow_rays.release(); d_shadow_path.release(); d_shadow_contrib.release(); d_shadow_visible.release();
+    d_image.release(); d_accum.release(); d_final.release();
     frame_capacity = 0;
+    image_valid = false;
+}
+
+// Queues sized for every path of the frame (the reference sizes them to
+// W*H, renderer.cpp:432-469); the accumulator and output images per pixel.
+wf_status wf_renderer::ensure_frame_buffers(size_t n_paths, size_t n_pixels)
+{
+    hipError_t e = hipSuccess;
+    auto need = [&](DevBuf& b, size_t bytes) { if (e == hipSuccess) e = b.ensure(std::max<size_t>(bytes, 64)); };
+    for (auto& b : d_rays) need(b, sizeof(wf_ray) * n_paths);
+    for (auto& b : d_paths) need(b, 4 * n_paths);
+    need(d_hits, sizeof(wf_hit) * n_paths);
+    need(d_thr, 16 * n_paths);
+    need(d_path_energy, 16 * n_paths);
+    need(d_counts, 4 * kCountSlots);
+    need(d_shadow_rays, sizeof(wf_ray) * n_paths);
+    need(d_shadow_path, 4 * n_paths);
+    need(d_shadow_contrib, 16 * n_paths);
+    need(d_shadow_visible, 4 * n_paths);
+    const bool fresh_accum = d_accum.bytes < 16 * n_pixels;
+    need(d_image, 16 * n_pixels);
+    need(d_accum, 16 * n_pixels);
+    need(d_final, 4 * n_pixels);
+    if (e != hipSuccess) return hip_fail(e, "frame buffer allocation");
+    if (fresh_accum) accum_count = 0;
+    frame_capacity = n_paths;
+    return WF_OK;
+}
+
+// renderer::render (renderer.cpp:632-1049), wavefront branch: clear, then
+// for every depth d <= max_bounces: (generate at d = 0), extend, shade,
+// connect; then resolve + post-process.  All counts stay on the device.
+wf_status wf_renderer::render_frame()
+{
+    if (!view_valid) return fail(WF_ERROR_INVALID_STATE, "render called before begin_scene");
+    if (env_texture != WF_INVALID_HANDLE && find_texture(env_texture))
+        return fail(WF_ERROR_INVALID_ARGUMENT, "environment textures are not supported (only the constant default)");
+    wf_status s = upload_scene();
+    if (s != WF_OK) return s;
+    const uint32_t npix = width * height;
+    const uint64_t npath64 = (uint64_t)npix * settings.spp;
+    if (npath64 > (1ull << 30)) return fail(WF_ERROR_CAPACITY, "width*height*spp exceeds 2^30 paths");
+    const uint32_t npath = (uint32_t)npath64;
+    if ((s = ensure_frame_buffers(npath, npix)) != WF_OK) return s;
+    if (accum_count == 0) accum_count = 1;   // reset_color_accumulator (renderer.cpp:47-51)
+
+    // renderer.cpp:739: a new frame seed per frame unless the caller fixes it
+    uint32_t seed = settings.frame_seed;
+    if (seed == 0) {
+        xorshift_seed ^= xorshift_seed << 13;
+        xorshift_seed ^= xorshift_seed >> 17;
+        xorshift_seed ^= xorshift_seed << 5;
+        seed = xorshift_seed;
+    }
+
+    FrameArgs A{};
+    A.width = width; A.height = height; A.spp = settings.spp; A.n_paths = npath;
+    A.view = view;
+    A.frame_seed = seed;
+    A.max_bounces = settings.max_bounces;
+    A.rr_start = settings.rr_start_bounce;
+    A.nee = settings.next_event_estimation;
+    A.rr_survival = settings.rr_survival;
+    for (int k = 0; k < 3; ++k) A.env[k] = settings.hdr_env_strength;   // default 1x1 white environment
+    A.n_emitters = n_emitters;
+    A.emitter_area = emitter_area;
+    A.accumulate = settings.accumulate;
+    A.sample_count = accum_count;
+    A.thr = (float4*)d_thr.p;
+    A.energy = (float4*)d_path_energy.p;
+    for (int k = 0; k < 2; ++k) { A.rays[k] = (wf_ray*)d_rays[k].p; A.path_of[k] = (uint32_t*)d_paths[k].p; }
+    A.hits = (wf_hit*)d_hits.p;
+    A.counts = (uint32_t*)d_counts.p;
+    A.shadow_rays = (wf_ray*)d_shadow_rays.p;
+    A.shadow_path = (uint32_t*)d_shadow_path.p;
+    A.shadow_contrib = (float4*)d_shadow_contrib.p;
+    A.shadow_visible = (uint32_t*)d_shadow_visible.p;
+    A.shade_inst = (const GpuShadeInstance*)d_shade_inst.p;
+    A.textures = (const GpuTexture*)d_textures.p;
+    A.emitters = (const uint4*)d_emitters.p;
+    A.image = (float4*)d_image.p;
+    A.accum = (float4*)d_accum.p;
+    A.final_rgba8 = (uint32_t*)d_final.p;
+
+    const uint32_t grid = (uint32_t)cu_count * 8u;
+    hipError_t e;
+    int ei = 0;
+    auto mark = [&]() { return hipEventRecord(ev[ei++], stream); };
+    if ((s = reset_overflows()) != WF_OK) return s;
+    if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+    if ((e = launch_clear(A, grid, stream)) != hipSuccess) return hip_fail(e, "clear launch");
+    if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+    if ((e = launch_generate(A, grid, stream)) != hipSuccess) return hip_fail(e, "generate launch");
+    if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+    // events: 0 start, 1 after clear, 2 after generate, then per depth
+    // (after extend, after shade, after connect), then after post
+    for (uint32_t d = 0; d <= settings.max_bounces; ++d) {
+        s = run_extend(A.rays[d & 1u], A.counts + d, npath, A.hits, nullptr, nullptr, false);
+        if (s != WF_OK) return s;
+        if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+        if ((e = launch_shade(A, d, grid, stream)) != hipSuccess) return hip_fail(e, "shade launch");
+        if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+        if (settings.next_event_estimation && d < settings.max_bounces) {
+            s = run_extend(A.shadow_rays, A.counts + kShadowCountBase + d, npath, nullptr, A.shadow_visible, nullptr, true);
+            if (s != WF_OK) return s;
+            if ((e = launch_connect_accumulate(A, d, grid, stream)) != hipSuccess) return hip_fail(e, "connect launch");
+        }
+        if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+    }
+    if ((e = launch_resolve_post(A, grid, stream)) != hipSuccess) return hip_fail(e, "post-process launch");
+    if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+
+    uint32_t counts[kCountSlots];
+    e = hipMemcpyAsync(counts, d_counts.p, sizeof(counts), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "frame");
+    wf_frame_stats st{};
+    auto ms = [&](int a, int b) { float t = 0.0f; (void)hipEventElapsedTime(&t, ev[a], ev[b]); return t; };
+    st.ms_clear = ms(0, 1);
+    st.ms_generate = ms(1, 2);
+    for (uint32_t d = 0; d <= settings.max_bounces; ++d) {
+        const int b = 2 + 3 * (int)d;
+        st.ms_extend += ms(b, b + 1);
+        st.ms_shade += ms(b + 1, b + 2);
+        st.ms_connect += ms(b + 2, b + 3);
+        st.rays_extended += counts[d];
+        if (d < kMaxBounces) st.shadow_rays += counts[kShadowCountBase + d];
+    }
+    st.ms_post = ms(ei - 2, ei - 1);
+    st.ms_total = ms(0, ei - 1);
+    uint32_t ov = 0;
+    if ((s = read_overflows(&ov)) != WF_OK) return s;
+    st.stack_overflows = ov;
+    stats = st;
+    image_valid = true;
+    accum_count++;
+    if (ov) return fail(WF_ERROR_STACK_OVERFLOW, "%u rays overflowed the traversal stack this frame", ov);
+    return WF_OK;
 }
 
 // Texture table and the emitter list for next-event estimation: every
@@ -138,23 +279,32 @@ wf_status wf_renderer::upload_materials_and_emitters()
 extern "C" WF_API wf_status wf_renderer_render(wf_renderer* r)
 {
     if (!r) return WF_ERROR_INVALID_ARGUMENT;
-    return r->fail(WF_ERROR_INVALID_STATE, "render: wavefront loop not built yet");
+    return r->render_frame();
+}
+
+static wf_status read_image(wf_renderer* r, const DevBuf& b, void* out, size_t bytes)
+{
+    if (!r->image_valid) return r->fail(WF_ERROR_INVALID_STATE, "no frame rendered");
+    hipError_t e = hipMemcpyAsync(out, b.p, bytes, hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return r->hip_fail(e, "image readback");
+    return WF_OK;
 }
 
 extern "C" WF_API wf_status wf_renderer_read_energy(wf_renderer* r, float* out)
 {
     if (!r || !out) return WF_ERROR_INVALID_ARGUMENT;
-    return r->fail(WF_ERROR_INVALID_STATE, "no frame rendered");
+    return read_image(r, r->d_image, out, 16ull * r->width * r->height);
 }
 
 extern "C" WF_API wf_status wf_renderer_read_accumulator(wf_renderer* r, float* out)
 {
     if (!r || !out) return WF_ERROR_INVALID_ARGUMENT;
-    return r->fail(WF_ERROR_INVALID_STATE, "no frame rendered");
+    return read_image(r, r->d_accum, out, 16ull * r->width * r->height);
 }
 
 extern "C" WF_API wf_status wf_renderer_read_final_color(wf_renderer* r, uint8_t* out)
 {
     if (!r || !out) return WF_ERROR_INVALID_ARGUMENT;
-    return r->fail(WF_ERROR_INVALID_STATE, "no frame rendered");
+    return read_image(r, r->d_final, out, 4ull * r->width * r->height);
 }

@@ -1,0 +1,425 @@
+// Wavefront stages around extend: clear, generate, shade (+ shadow-ray
+// emission for connect), connect accumulation, resolve and post-process.
+//
+// Reference stage kernels: clear_buffers.hlsl:21-39, init_args.hlsl:15-22,
+// generate.hlsl:12-20, shade.hlsl:42-174, post_process.hlsl:25-72.  HIP has no
+// ExecuteIndirect, so every stage is a grid-stride kernel over a queue whose
+// length it reads from device memory; the host never reads a count inside
+// the bounce loop.  Queue appends are aggregated per wave64 (ballot +
+// popcount, one atomic per wave), and the ray queues are double-buffered
+// (the reference compacts in place, shade.hlsl:52 vs :167).
+//
+// Per-path state (throughput, energy) is FP32 and indexed by path id, so a
+// path's arithmetic does not depend on where the compaction placed it; the
+// random numbers are PCG32 keyed on (path, bounce).
+#include "stages.h"
+
+#include <cfloat>
+
+#pragma clang fp contract(off)
+
+namespace wf {
+namespace {
+
+constexpr float kInvPi = 0.31830988f;   // common.hlsl:23
+constexpr float kRayMinT = 1e-8f;
+
+// ---- RNG: PCG32 (O'Neill 2014) seeded per (path, bounce) ------------------
+struct Pcg { uint64_t state, inc; };
+__device__ __forceinline__ uint32_t pcg_next(Pcg& r)
+{
+    const uint64_t old = r.state;
+    r.state = old * 6364136223846793005ull + r.inc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+__device__ __forceinline__ void path_rng(uint32_t path, uint32_t bounce, uint32_t seed, float u[6])
+{
+    Pcg r;
+    r.state = 0u;
+    r.inc = ((uint64_t)bounce << 1u) | 1u;
+    pcg_next(r);
+    r.state += ((uint64_t)seed << 32) | path;
+    pcg_next(r);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) u[k] = (float)(pcg_next(r) >> 8) * (1.0f / 16777216.0f);
+}
+
+// ---- sin/cos of 2*pi*u (quadrant + Taylor, the same operation sequence as
+// the host restatement) ----------------------------------------------------------
+__device__ __forceinline__ void sincos_turns(float u, float& s, float& c)
+{
+    const float q4 = u * 4.0f;
+    const float qf = floorf(q4);
+    const float f = q4 - qf;
+    const int q = ((int)qf) & 3;
+    const float x = f * 1.57079632679489662f;
+    const float y = x * x;
+    float p = (float)(1.0 / 6227020800.0);
+    p = p * y - (float)(1.0 / 39916800.0);
+    p = p * y + (float)(1.0 / 362880.0);
+    p = p * y - (float)(1.0 / 5040.0);
+    p = p * y + (float)(1.0 / 120.0);
+    p = p * y - (float)(1.0 / 6.0);
+    p = p * y + 1.0f;
+    const float sx = x * p;
+    float r = -(float)(1.0 / 87178291200.0);
+    r = r * y + (float)(1.0 / 479001600.0);
+    r = r * y - (float)(1.0 / 3628800.0);
+    r = r * y + (float)(1.0 / 40320.0);
+    r = r * y - (float)(1.0 / 720.0);
+    r = r * y + (float)(1.0 / 24.0);
+    r = r * y - 0.5f;
+    r = r * y + 1.0f;
+    const float cx = r;
+    switch (q) {
+    case 0: s = sx; c = cx; break;
+    case 1: s = cx; c = -sx; break;
+    case 2: s = -sx; c = -cx; break;
+    default: s = -cx; c = sx; break;
+    }
+}
+
+struct V3 { float x, y, z; };
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3 normalize3(V3 v)
+{
+    const float s = 1.0f / sqrtf(dot3(v, v));
+    return V3{v.x * s, v.y * s, v.z * s};
+}
+__device__ __forceinline__ V3 cross3(V3 a, V3 b)
+{
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// glm mat4 * vec4 association: (c0*x + c1*y) + (c2*z + c3*w), xyz only
+__device__ __forceinline__ V3 xform(const float* m, V3 v, float w)
+{
+    V3 o;
+    o.x = (m[0] * v.x + m[4] * v.y) + (m[8] * v.z + m[12] * w);
+    o.y = (m[1] * v.x + m[5] * v.y) + (m[9] * v.z + m[13] * w);
+    o.z = (m[2] * v.x + m[6] * v.y) + (m[10] * v.z + m[14] * w);
+    return o;
+}
+__device__ __forceinline__ float interp(float a, float b, float c, float bx, float by) { return (a + bx * (b - a)) + by * (c - a); }
+
+// cosine_weighted_hemisphere_sample + create_orthonormal_basis (sample.hlsl:13-27,45-54)
+__device__ __forceinline__ V3 cosine_sample(V3 n, float r0, float r1)
+{
+    const float sin_theta = sqrtf(r0);
+    float sp, cp;
+    sincos_turns(r1, sp, cp);
+    const V3 v = normalize3(V3{sin_theta * cp, sqrtf(1.0f - r0), sin_theta * sp});
+    const V3 t0 = (fabsf(n.x) > fabsf(n.z)) ? V3{-n.y, n.x, 0.0f} : V3{0.0f, -n.z, n.y};
+    const V3 t = normalize3(t0);
+    const V3 b = cross3(n, t);
+    return V3{(v.x * t.x + v.y * n.x) + v.z * b.x, (v.x * t.y + v.y * n.y) + v.z * b.y, (v.x * t.z + v.y * n.z) + v.z * b.z};
+}
+
+__device__ __forceinline__ V3 tex_sample(const GpuTexture& tx, float u, float v)
+{
+    float fx = u * (float)tx.width - 0.5f, fy = v * (float)tx.height - 0.5f;
+    fx = fminf(fmaxf(fx, -1e9f), 1e9f);
+    fy = fminf(fmaxf(fy, -1e9f), 1e9f);
+    const float fx0 = floorf(fx), fy0 = floorf(fy);
+    const float ax = fx - fx0, ay = fy - fy0;
+    const int64_t ix = (int64_t)fx0, iy = (int64_t)fy0;
+    const int64_t w = tx.width, h = tx.height;
+    const uint32_t x0 = (uint32_t)(((ix % w) + w) % w), y0 = (uint32_t)(((iy % h) + h) % h);
+    const uint32_t x1 = (x0 + 1u == (uint32_t)w) ? 0u : x0 + 1u, y1 = (y0 + 1u == (uint32_t)h) ? 0u : y0 + 1u;
+    const float4* t = reinterpret_cast<const float4*>(tx.texels);
+    const float4 a = t[(size_t)y0 * w + x0], b = t[(size_t)y0 * w + x1];
+    const float4 c = t[(size_t)y1 * w + x0], d = t[(size_t)y1 * w + x1];
+    V3 o;
+    float top, bot;
+    top = a.x + ax * (b.x - a.x); bot = c.x + ax * (d.x - c.x); o.x = top + ay * (bot - top);
+    top = a.y + ax * (b.y - a.y); bot = c.y + ax * (d.y - c.y); o.y = top + ay * (bot - top);
+    top = a.z + ax * (b.z - a.z); bot = c.z + ax * (d.z - c.z); o.z = top + ay * (bot - top);
+    return o;
+}
+
+// one atomic per wave64 for all lanes appending to the same queue
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
+{
+    const uint64_t mask = __ballot(pred);
+    uint32_t slot = 0;
+    if (pred) {
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, (int)leader, 64);
+        slot = base + rank;
+    }
+    return slot;
+}
+
+__device__ __forceinline__ void store_ray(wf_ray* q, uint32_t slot, V3 o, V3 d, float tmin, float tmax)
+{
+    float4* p = reinterpret_cast<float4*>(q + slot);
+    p[0] = make_float4(o.x, o.y, o.z, tmin);
+    p[1] = make_float4(d.x, d.y, d.z, tmax);
+}
+
+// ---- kernels ---------------------------------------------------------------------
+
+__global__ void k_clear(FrameArgs A)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < A.n_paths; p += stride) {
+        A.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        A.energy[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kCountSlots) A.counts[threadIdx.x] = (threadIdx.x == 0) ? A.n_paths : 0u;
+}
+
+// generate.hlsl + make_primary_ray (common.hlsl:156-169): path p covers
+// pixel p % (W*H); queue slot p holds path p.
+__global__ void k_generate(FrameArgs A)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t npix = A.width * A.height;
+    const float* c2v = A.view.clip_to_view;
+    const float* v2w = A.view.view_to_world;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < A.n_paths; p += stride) {
+        const uint32_t pixel = p % npix;
+        const uint32_t px = pixel % A.width, py = pixel / A.width;
+        const float uvx = ((float)px + 0.5f) / A.view.render_dim[0];
+        float uvy = ((float)py + 0.5f) / A.view.render_dim[1];
+        uvy = 1.0f - uvy;
+        const V3 cv = xform(c2v, V3{2.0f * uvx - 1.0f, 2.0f * uvy - 1.0f, 1.0f}, 1.0f);
+        const V3 dv = normalize3(cv);
+        const V3 dw = xform(v2w, dv, 0.0f);
+        const V3 ow = xform(v2w, V3{0.0f, 0.0f, 0.0f}, 1.0f);
+        const V3 o = V3{ow.x + dw.x * kRayMinT, ow.y + dw.y * kRayMinT, ow.z + dw.z * kRayMinT};
+        store_ray(A.rays[0], p, o, dw, kRayMinT, FLT_MAX);
+        A.path_of[0][p] = p;
+    }
+}
+
+// shade.hlsl:42-174 with the build's RNG, Russian roulette and next-event
+// estimation (see the host restatement for the exact operation order).
+__global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
+{
+    const uint32_t n = A.counts[d];
+    const uint32_t cur = d & 1u, nxt = cur ^ 1u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // iterate whole waves so that the ballot-based appends see converged lanes
+    const uint32_t n_round = (n + 63u) & ~63u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
+        const bool active = i < n;
+        bool spawn = false, shadow = false;
+        V3 new_o{0, 0, 0}, new_d{0, 0, 0}, sh_o{0, 0, 0}, sh_d{0, 0, 0};
+        float sh_tmax = 0.0f;
+        float4 contrib = make_float4(0, 0, 0, 0);
+        uint32_t p = 0;
+        if (active) {
+            p = A.path_of[cur][i];
+            const float4 r1 = reinterpret_cast<const float4*>(A.rays[cur] + i)[1];
+            const wf_hit hit = A.hits[i];
+            const float4 t4 = A.thr[p];
+            float4 e4 = A.energy[p];
+            V3 thr{t4.x, t4.y, t4.z};
+            float u[6];
+            path_rng(p, d, A.frame_seed, u);
+            (void)r1;
+            bool done = false;
+            if (hit.instance_idx == 0xFFFFFFFFu) {
+                e4.x = e4.x + thr.x * A.env[0];
+                e4.y = e4.y + thr.y * A.env[1];
+                e4.z = e4.z + thr.z * A.env[2];
+                done = true;
+            }
+            if (!done) {
+                const GpuShadeInstance* si = A.shade_inst + hit.instance_idx;
+                const float* l2w = si->local_to_world;
+                const GpuShadeTri* tri = si->tris + hit.primitive_idx;
+                const float bx = hit.bary[0], by = hit.bary[1];
+                const V3 lp{interp(tri->p[0][0], tri->p[1][0], tri->p[2][0], bx, by),
+                            interp(tri->p[0][1], tri->p[1][1], tri->p[2][1], bx, by),
+                            interp(tri->p[0][2], tri->p[1][2], tri->p[2][2], bx, by)};
+                const V3 ln{interp(tri->n[0][0], tri->n[1][0], tri->n[2][0], bx, by),
+                            interp(tri->n[0][1], tri->n[1][1], tri->n[2][1], bx, by),
+                            interp(tri->n[0][2], tri->n[1][2], tri->n[2][2], bx, by)};
+                const float tu = interp(tri->uv[0][0], tri->uv[1][0], tri->uv[2][0], bx, by);
+                const float tv = interp(tri->uv[0][1], tri->uv[1][1], tri->uv[2][1], bx, by);
+                const V3 P = xform(l2w, lp, 1.0f);
+                const V3 N = normalize3(xform(l2w, ln, 0.0f));
+                V3 albedo{si->base_color[0], si->base_color[1], si->base_color[2]};
+                if (si->base_color_tex >= 0) {
+                    const V3 tc = tex_sample(A.textures[si->base_color_tex], tu, tv);
+                    albedo = V3{albedo.x * tc.x, albedo.y * tc.y, albedo.z * tc.z};
+                }
+                const V3 em{si->emissive[0], si->emissive[1], si->emissive[2]};
+                if (em.x != 0.0f || em.y != 0.0f || em.z != 0.0f) {
+                    if (!A.nee || d == 0) {
+                        e4.x = e4.x + thr.x * em.x;
+                        e4.y = e4.y + thr.y * em.y;
+                        e4.z = e4.z + thr.z * em.z;
+                    }
+                    done = true;
+                }
+                if (!done && A.nee && A.n_emitters > 0 && d < A.max_bounces) {
+                    uint32_t lo = 0, hi = A.n_emitters - 1u;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (__uint_as_float(A.emitters[mid].z) > u[3]) hi = mid; else lo = mid + 1u;
+                    }
+                    const uint4 er = A.emitters[lo];
+                    const GpuShadeInstance* li = A.shade_inst + er.x;
+                    const GpuShadeTri* lt = li->tris + er.y;
+                    const V3 W0 = xform(li->local_to_world, V3{lt->p[0][0], lt->p[0][1], lt->p[0][2]}, 1.0f);
+                    const V3 W1 = xform(li->local_to_world, V3{lt->p[1][0], lt->p[1][1], lt->p[1][2]}, 1.0f);
+                    const V3 W2 = xform(li->local_to_world, V3{lt->p[2][0], lt->p[2][1], lt->p[2][2]}, 1.0f);
+                    const float su = sqrtf(u[4]);
+                    const float bv = su * (1.0f - u[5]), bw = su * u[5];
+                    const V3 PL{interp(W0.x, W1.x, W2.x, bv, bw), interp(W0.y, W1.y, W2.y, bv, bw), interp(W0.z, W1.z, W2.z, bv, bw)};
+                    const V3 dl{PL.x - P.x, PL.y - P.y, PL.z - P.z};
+                    const V3 e1{W1.x - W0.x, W1.y - W0.y, W1.z - W0.z};
+                    const V3 e2{W2.x - W0.x, W2.y - W0.y, W2.z - W0.z};
+                    const float dist2 = dot3(dl, dl);
+                    const float dist = sqrtf(dist2);
+                    const float inv_d = 1.0f / dist;
+                    const V3 wi{dl.x * inv_d, dl.y * inv_d, dl.z * inv_d};
+                    const V3 nl = normalize3(cross3(e1, e2));
+                    const float cos_s = dot3(N, wi);
+                    const float cos_l = -dot3(nl, wi);
+                    if (cos_s > 0.0f && cos_l > 0.0f && dist2 > 0.0f) {
+                        const float G = (cos_s * cos_l) / dist2;
+                        const float wgt = G * A.emitter_area;
+                        contrib.x = (((thr.x * albedo.x) * kInvPi) * li->emissive[0]) * wgt;
+                        contrib.y = (((thr.y * albedo.y) * kInvPi) * li->emissive[1]) * wgt;
+                        contrib.z = (((thr.z * albedo.z) * kInvPi) * li->emissive[2]) * wgt;
+                        sh_o = V3{P.x + wi.x * kRayMinT, P.y + wi.y * kRayMinT, P.z + wi.z * kRayMinT};
+                        sh_d = wi;
+                        sh_tmax = dist * 0.9999f;
+                        shadow = true;
+                    }
+                }
+                if (!done) {
+                    const V3 L = cosine_sample(N, u[0], u[1]);
+                    float NoL = dot3(N, L);
+                    if (NoL < 0.0f) NoL = 0.0f;
+                    if (!(NoL > 0.0f)) {
+                        done = true;
+                    } else {
+                        const float pdf = NoL * kInvPi;
+                        const float inv_pdf = 1.0f / pdf;
+                        thr.x = thr.x * ((NoL * (albedo.x * kInvPi)) * inv_pdf);
+                        thr.y = thr.y * ((NoL * (albedo.y * kInvPi)) * inv_pdf);
+                        thr.z = thr.z * ((NoL * (albedo.z * kInvPi)) * inv_pdf);
+                        if (d >= A.rr_start) {
+                            if (u[2] >= A.rr_survival) done = true;
+                            else { thr.x = thr.x / A.rr_survival; thr.y = thr.y / A.rr_survival; thr.z = thr.z / A.rr_survival; }
+                        }
+                        if (!done && d < A.max_bounces) {
+                            spawn = true;
+                            new_o = V3{P.x + L.x * kRayMinT, P.y + L.y * kRayMinT, P.z + L.z * kRayMinT};
+                            new_d = L;
+                        }
+                    }
+                }
+            }
+            A.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+            A.energy[p] = e4;
+        }
+        const uint32_t slot = wave_append(&A.counts[d + 1], spawn);
+        if (spawn) {
+            store_ray(A.rays[nxt], slot, new_o, new_d, kRayMinT, FLT_MAX);
+            A.path_of[nxt][slot] = p;
+        }
+        const uint32_t sslot = wave_append(&A.counts[kShadowCountBase + d], shadow);
+        if (shadow) {
+            store_ray(A.shadow_rays, sslot, sh_o, sh_d, kRayMinT, sh_tmax);
+            A.shadow_path[sslot] = p;
+            A.shadow_contrib[sslot] = contrib;
+        }
+    }
+}
+
+// post_process.hlsl:25-72 after averaging the spp paths of each pixel.
+__global__ void k_resolve_post(FrameArgs A)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t npix = A.width * A.height;
+    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += stride) {
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        for (uint32_t s = 0; s < A.spp; ++s) {
+            const float4 e = A.energy[(size_t)s * npix + px];
+            ax = ax + e.x; ay = ay + e.y; az = az + e.z;
+        }
+        const float inv = (float)A.spp;
+        float4 e = make_float4(ax / inv, ay / inv, az / inv, 1.0f);
+        A.image[px] = e;
+        // badness detector: NaN / Inf -> magenta
+        if (!(isfinite(e.x) && isfinite(e.y) && isfinite(e.z))) e = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
+        float4 acc = A.accum[px];
+        const float w = 1.0f / (float)A.sample_count;
+        if (A.accumulate) {
+            acc.x = acc.x * (1.0f - w) + e.x * w;
+            acc.y = acc.y * (1.0f - w) + e.y * w;
+            acc.z = acc.z * (1.0f - w) + e.z * w;
+            acc.w = acc.w * (1.0f - w) + 1.0f * w;
+        } else {
+            acc = e;
+        }
+        A.accum[px] = acc;
+        // Reinhard, then linear -> sRGB (common.hlsl:251-259)
+        float c[3] = {acc.x / (1.0f + acc.x), acc.y / (1.0f + acc.y), acc.z / (1.0f + acc.z)};
+        uint32_t rgba = 0;
+        for (int k = 0; k < 3; ++k) {
+            const float x = fminf(fmaxf(c[k], 0.0f), 1.0f);
+            const float sv = (x <= 0.0031308f) ? x * 12.92f : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+            const uint32_t b = (uint32_t)fminf(fmaxf(sv * 255.0f + 0.5f, 0.0f), 255.0f);
+            rgba |= b << (8 * k);
+        }
+        rgba |= 255u << 24;
+        A.final_rgba8[px] = rgba;
+    }
+}
+
+// connect: add the contribution of every unoccluded shadow ray of bounce d
+__global__ void k_connect_accumulate(FrameArgs A, uint32_t d)
+{
+    const uint32_t n = A.counts[kShadowCountBase + d];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (!A.shadow_visible[i]) continue;
+        const uint32_t p = A.shadow_path[i];
+        const float4 c = A.shadow_contrib[i];
+        float4 e = A.energy[p];
+        e.x = e.x + c.x; e.y = e.y + c.y; e.z = e.z + c.z;
+        A.energy[p] = e;
+    }
+}
+
+} // namespace
+
+hipError_t launch_clear(const FrameArgs& A, uint32_t grid, hipStream_t s)
+{
+    k_clear<<<grid, 256, 0, s>>>(A);
+    return hipGetLastError();
+}
+hipError_t launch_generate(const FrameArgs& A, uint32_t grid, hipStream_t s)
+{
+    k_generate<<<grid, 256, 0, s>>>(A);
+    return hipGetLastError();
+}
+hipError_t launch_shade(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s)
+{
+    k_shade<<<grid, 256, 0, s>>>(A, d);
+    return hipGetLastError();
+}
+hipError_t launch_connect_accumulate(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s)
+{
+    k_connect_accumulate<<<grid, 256, 0, s>>>(A, d);
+    return hipGetLastError();
+}
+hipError_t launch_resolve_post(const FrameArgs& A, uint32_t grid, hipStream_t s)
+{
+    k_resolve_post<<<grid, 256, 0, s>>>(A);
+    return hipGetLastError();
+}
+
+} // namespace wf

@@ -160,6 +160,13 @@ extern "C" WF_API const char* wf_renderer_last_error(const wf_renderer* r)
     return r ? r->err.c_str() : "null renderer";
 }
 
+extern "C" WF_API wf_status wf_renderer_get_stream(const wf_renderer* r, void** out)
+{
+    if (!r || !out) return WF_ERROR_INVALID_ARGUMENT;
+    *out = (void*)r->stream;
+    return WF_OK;
+}
+
 // ---------------------------------------------------------------------------
 // settings / camera
 // ---------------------------------------------------------------------------
@@ -185,6 +192,33 @@ extern "C" WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_re
                        s->next_event_estimation != r->settings.next_event_estimation;
     r->settings = *s;
     if (reset) r->accum_count = 0;
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, const wf_traversal_config* c)
+{
+    if (!r || !c) return WF_ERROR_INVALID_ARGUMENT;
+    if (c->workgroup_size != 64 && c->workgroup_size != 128 && c->workgroup_size != 256)
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "workgroup_size must be 64, 128 or 256");
+    if (c->lds_stack_entries != 0 && c->lds_stack_entries != 8 && c->lds_stack_entries != 16 && c->lds_stack_entries != 32)
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "lds_stack_entries must be 0, 8, 16 or 32");
+    if (c->bvh_layout != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "bvh_layout must be 0 (BVH2)");
+    const ExtendConfig old = r->ext_cfg;
+    r->ext_cfg.block = c->workgroup_size;
+    r->ext_cfg.lds_stack = c->lds_stack_entries;
+    r->ext_cfg.skip_popped = c->skip_popped_interior ? 1u : 0u;
+    const wf_status s = r->ensure_trace_scratch();
+    if (s != WF_OK) { r->ext_cfg = old; return s; }
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* r, wf_traversal_config* out)
+{
+    if (!r || !out) return WF_ERROR_INVALID_ARGUMENT;
+    out->workgroup_size = r->ext_cfg.block;
+    out->lds_stack_entries = r->ext_cfg.lds_stack;
+    out->skip_popped_interior = r->ext_cfg.skip_popped;
+    out->bvh_layout = 0;
     return WF_OK;
 }
 
@@ -383,6 +417,15 @@ extern "C" WF_API wf_status wf_renderer_build_tlas(wf_renderer* r)
     return r->upload_scene();
 }
 
+extern "C" WF_API wf_status wf_renderer_get_scene_bounds(const wf_renderer* r, float out_min[3], float out_max[3])
+{
+    if (!r || !out_min || !out_max) return WF_ERROR_INVALID_ARGUMENT;
+    if (!r->tlas_valid || r->tlas_host.nodes.empty()) return WF_ERROR_INVALID_STATE;
+    std::memcpy(out_min, r->tlas_host.nodes[0].aabb_min, 12);
+    std::memcpy(out_max, r->tlas_host.nodes[0].aabb_max, 12);
+    return WF_OK;
+}
+
 // ---------------------------------------------------------------------------
 // extend on caller rays
 // ---------------------------------------------------------------------------
@@ -390,7 +433,7 @@ extern "C" WF_API wf_status wf_renderer_build_tlas(wf_renderer* r)
 wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
                                   uint32_t* d_visible, uint32_t* d_counters, bool any)
 {
-    hipError_t e = launch_fill_u32((uint32_t*)d_work.p, 0u, 16, stream);
+    hipError_t e = launch_fill_u32((uint32_t*)d_work.p, 0u, 1, stream);   // work cursor; word 1 counts overflows
     if (e != hipSuccess) return hip_fail(e, "work counter reset");
     ExtendLaunch L{};
     L.scene.tnodes = (const wf_tlas_node*)d_tnodes.p;
@@ -408,6 +451,13 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     L.any_hit = any ? 1 : 0;
     e = launch_extend(L, ext_cfg, stream);
     if (e != hipSuccess) return hip_fail(e, "extend launch");
+    return WF_OK;
+}
+
+wf_status wf_renderer::reset_overflows()
+{
+    hipError_t e = launch_fill_u32((uint32_t*)d_work.p, 0u, 16, stream);
+    if (e != hipSuccess) return hip_fail(e, "overflow counter reset");
     return WF_OK;
 }
 
@@ -432,6 +482,7 @@ static wf_status trace_host(wf_renderer* r, const wf_ray* rays, uint32_t n, wf_h
     if (e == hipSuccess && counters) e = r->d_tmp_aux.ensure(16ull * n);
     if (e == hipSuccess) e = hipMemcpyAsync(r->d_tmp_rays.p, rays, sizeof(wf_ray) * (size_t)n, hipMemcpyHostToDevice, r->stream);
     if (e != hipSuccess) return r->hip_fail(e, "ray upload");
+    if ((s = r->reset_overflows()) != WF_OK) return s;
     s = r->run_extend((const wf_ray*)r->d_tmp_rays.p, nullptr, n, any ? nullptr : (wf_hit*)r->d_tmp_hits.p,
                       any ? (uint32_t*)r->d_tmp_hits.p : nullptr, counters ? (uint32_t*)r->d_tmp_aux.p : nullptr, any);
     if (s != WF_OK) return s;

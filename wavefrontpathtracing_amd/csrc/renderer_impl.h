@@ -12,7 +12,7 @@ namespace wf {
 
 constexpr uint32_t kMaxTextures = 65536;   // renderer.cpp:232 texture slotmap size
 constexpr uint32_t kStackTotal = 128;      // LDS + spill entries per lane (reference: 64 TLAS + 64 BLAS)
-constexpr int kNumStageEvents = 16;
+constexpr int kNumStageEvents = 48;
 
 struct DevBuf {
     void* p = nullptr;
@@ -86,8 +86,11 @@ struct wf_renderer {
     wf::DevBuf d_work, d_spill, d_tmp_rays, d_tmp_hits, d_tmp_aux;
 
     // wavefront buffers (allocated on first render)
-    wf::DevBuf d_rays[2], d_paths[2], d_hits, d_shadow, d_counts, d_energy, d_accum, d_final;
+    wf::DevBuf d_rays[2], d_paths[2], d_hits, d_thr, d_path_energy, d_counts;
+    wf::DevBuf d_shadow_rays, d_shadow_path, d_shadow_contrib, d_shadow_visible;
+    wf::DevBuf d_image, d_accum, d_final;
     size_t frame_capacity = 0;
+    bool image_valid = false;
 
     wf_status fail(wf_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
     wf_status hip_fail(hipError_t e, const char* what);
@@ -99,5 +102,8 @@ struct wf_renderer {
     wf_status run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
                          uint32_t* d_visible, uint32_t* d_counters, bool any);
     wf_status read_overflows(uint32_t* out);
+    wf_status reset_overflows();
+    wf_status ensure_frame_buffers(size_t n_paths, size_t n_pixels);
+    wf_status render_frame();
     void release_frame_buffers();
 };

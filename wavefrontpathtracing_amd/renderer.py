@@ -102,6 +102,13 @@ class Renderer:
     def _check(self, status, what):
         check(status, self.handle, what)
 
+    @property
+    def stream(self) -> int:
+        """Address of the hipStream_t the renderer orders its GPU work on."""
+        s = C.c_void_p()
+        self._check(lib().wf_renderer_get_stream(self.handle, C.byref(s)), "get_stream")
+        return s.value
+
     def begin_frame(self):
         self._check(lib().wf_renderer_begin_frame(self.handle), "begin_frame")
 
@@ -171,6 +178,11 @@ class Renderer:
     # ---- extend on caller rays ----
     def build_tlas(self) -> None:
         self._check(lib().wf_renderer_build_tlas(self.handle), "build_tlas")
+
+    def scene_bounds(self):
+        mn, mx = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        self._check(lib().wf_renderer_get_scene_bounds(self.handle, ptr(mn), ptr(mx)), "get_scene_bounds")
+        return mn, mx
 
     def trace_closest(self, rays: np.ndarray) -> np.ndarray:
         rays = np.ascontiguousarray(rays, dtype=RAY)

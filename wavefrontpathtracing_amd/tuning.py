@@ -1,0 +1,46 @@
+"""Traversal kernel configuration and the autotuner over it.
+
+Axes: workgroup size {64, 128, 256} x stack placement {LDS 8/16/32 entries
+per lane with global spill, private (scratch) stack} x popped-interior skip.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+import time
+
+from . import _lib as L
+
+DEFAULT = dict(block=128, lds_stack=16, skip_popped=1)
+
+
+def set_extend_config(renderer, block: int = DEFAULT["block"], lds_stack: int = DEFAULT["lds_stack"],
+                      skip_popped: int = DEFAULT["skip_popped"], bvh_layout: int = 0) -> None:
+    c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout)
+    L.check(L.lib().wf_renderer_set_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "set_traversal_config")
+
+
+def get_extend_config(renderer) -> dict:
+    c = L.TraversalConfig()
+    L.check(L.lib().wf_renderer_get_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "get_traversal_config")
+    return dict(block=c.workgroup_size, lds_stack=c.lds_stack_entries, skip_popped=c.skip_popped_interior, bvh_layout=c.bvh_layout)
+
+
+def candidates():
+    for block, lds, skip in itertools.product((64, 128, 256), (0, 8, 16, 32), (1, 0)):
+        yield dict(block=block, lds_stack=lds, skip_popped=skip)
+
+
+def autotune(renderer, time_fn, configs=None, repeats: int = 3):
+    """Time every configuration with ``time_fn()`` (returns seconds for one
+    measured pass, run after the config is set) and keep the fastest.
+    Returns (best_config, [(config, seconds)])."""
+    results = []
+    for cfg in (configs or list(candidates())):
+        set_extend_config(renderer, **cfg)
+        time_fn()   # warm-up
+        best = min(time_fn() for _ in range(repeats))
+        results.append((cfg, best))
+    results.sort(key=lambda x: x[1])
+    set_extend_config(renderer, **results[0][0])
+    return results[0][0], results
