@@ -114,6 +114,7 @@ typedef struct wf_render_settings {
     float    rr_survival;
     uint32_t next_event_estimation;
     uint32_t frame_seed;
+    uint32_t use_hip_graph;      /* 1: the frame's launches replay as one captured hipGraph */
 } wf_render_settings;
 
 /* bvh_builder_t::build_options_t (bvh/bvh_builder.h:17-21); the reference
@@ -143,6 +144,21 @@ WF_API wf_status wf_renderer_end_frame(wf_renderer* r);
 WF_API wf_status wf_renderer_begin_scene(wf_renderer* r, const wf_camera* camera, wf_texture_handle env_texture);
 WF_API wf_status wf_renderer_render(wf_renderer* r);
 WF_API wf_status wf_renderer_end_scene(wf_renderer* r);
+
+/* ---- batches (no reference counterpart): many scenes, each with its own
+ * camera and resolution, rendered by one set of launches over shared queues.
+ * begin_batch replaces begin_scene; batch_add_scene opens scene slot k
+ * (returned) and submit_render_mesh adds instances to the open slot;
+ * end_scene closes the batch.  render() draws every slot; render_scenes
+ * draws the slots [first, first+count) only (the per-scene loop).  A scene
+ * renders the same image alone or inside a batch (paths are numbered over
+ * all slots). */
+WF_API wf_status wf_renderer_begin_batch(wf_renderer* r);
+WF_API wf_status wf_renderer_batch_add_scene(wf_renderer* r, const wf_camera* camera, uint32_t width, uint32_t height, uint32_t* out_scene);
+WF_API wf_status wf_renderer_get_scene_count(const wf_renderer* r, uint32_t* out_count);
+WF_API wf_status wf_renderer_render_scenes(wf_renderer* r, uint32_t first, uint32_t count);
+/* linear energy of one scene slot: width*height RGBA float */
+WF_API wf_status wf_renderer_read_scene_energy(wf_renderer* r, uint32_t scene, float* out_rgba);
 
 /* ---- resources: create_render_texture / create_render_mesh / submit_render_mesh
  *      (renderer.cpp:1150-1350) ---------------------------------------- */

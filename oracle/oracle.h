@@ -131,6 +131,7 @@ typedef struct {
     uint32_t nee;                  /* 1: next-event estimation through connect */
     uint32_t frame_seed;
     float    rr_survival;
+    uint32_t path_base;            /* id of the first path (scenes of a batch number paths consecutively) */
 } orc_render_params;
 
 /* energy_rgba: width*height*4 floats (the resolved per-pixel mean over spp). */

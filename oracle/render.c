@@ -254,6 +254,7 @@ static int trace_path(render_job* J, uint32_t path, float energy[3])
     const orc_shading* sh = J->sh;
     uint32_t npix = J->w * J->h;
     uint32_t pixel = path % npix;
+    path += P->path_base;
     orc_ray ray;
     primary_ray(J->v, pixel % J->w, pixel / J->w, &ray);
     float thr[3] = { 1.0f, 1.0f, 1.0f };

@@ -43,7 +43,7 @@ class OrcShading(C.Structure):
 
 class OrcRenderParams(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("spp", C.c_uint32), ("rr_start", C.c_uint32), ("nee", C.c_uint32),
-                ("frame_seed", C.c_uint32), ("rr_survival", C.c_float)]
+                ("frame_seed", C.c_uint32), ("rr_survival", C.c_float), ("path_base", C.c_uint32)]
 
 
 _o = None
@@ -245,11 +245,12 @@ class OracleScene:
         self._sh = sh
         return sh
 
-    def render(self, max_bounces=3, spp=1, rr_start=3, rr_survival=0.75, nee=0, frame_seed=1, env=1.0, nthreads=8):
+    def render(self, max_bounces=3, spp=1, rr_start=3, rr_survival=0.75, nee=0, frame_seed=1, env=1.0, nthreads=8,
+               path_base=0):
         sc = self.scene
         sh = self.shading(env)
         view = camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
-        prm = OrcRenderParams(max_bounces, spp, rr_start, nee, frame_seed, rr_survival)
+        prm = OrcRenderParams(max_bounces, spp, rr_start, nee, frame_seed, rr_survival, path_base)
         out = np.zeros((sc.height, sc.width, 4), dtype=np.float32)
         rc = olib().orc_render(C.byref(self.s), C.byref(sh), C.byref(view), sc.width, sc.height, C.byref(prm), p(out), nthreads)
         if rc != 0:

@@ -74,7 +74,8 @@ class RenderSettings(C.Structure):
     _fields_ = [("use_wavefront_pathtracing", C.c_uint32), ("use_software_rt", C.c_uint32), ("render_view_mode", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("cosine_weighted_diffuse", C.c_uint32), ("accumulate", C.c_uint32),
                 ("hdr_env_strength", C.c_float), ("spp", C.c_uint32), ("rr_start_bounce", C.c_uint32),
-                ("rr_survival", C.c_float), ("next_event_estimation", C.c_uint32), ("frame_seed", C.c_uint32)]
+                ("rr_survival", C.c_float), ("next_event_estimation", C.c_uint32), ("frame_seed", C.c_uint32),
+                ("use_hip_graph", C.c_uint32)]
 
 
 class BuildOptions(C.Structure):
@@ -113,6 +114,11 @@ SIGNATURES = {
     "wf_renderer_begin_scene": (_st, [_vp, C.POINTER(Camera), _u64]),
     "wf_renderer_render": (_st, [_vp]),
     "wf_renderer_end_scene": (_st, [_vp]),
+    "wf_renderer_begin_batch": (_st, [_vp]),
+    "wf_renderer_batch_add_scene": (_st, [_vp, C.POINTER(Camera), _u32, _u32, C.POINTER(_u32)]),
+    "wf_renderer_get_scene_count": (_st, [_vp, C.POINTER(_u32)]),
+    "wf_renderer_render_scenes": (_st, [_vp, _u32, _u32]),
+    "wf_renderer_read_scene_energy": (_st, [_vp, _u32, _vp]),
     "wf_renderer_create_render_texture": (_st, [_vp, C.POINTER(RenderTextureParams), C.POINTER(_u64)]),
     "wf_renderer_create_render_mesh": (_st, [_vp, C.POINTER(RenderMeshParams), C.POINTER(_u64)]),
     "wf_renderer_submit_render_mesh": (_st, [_vp, _u64, _vp, C.POINTER(MaterialAsset)]),

@@ -141,7 +141,7 @@ enum TraceResult { kTraceDone = 0, kTraceOverflow = 1 };
 __device__ __forceinline__ float4 ld4(const void* p) { return *reinterpret_cast<const float4*>(p); }
 
 template <bool SKIP, bool COUNT, bool ANY, class Stack>
-__device__ __forceinline__ int trace_ray(const TraceScene& sc, const wf_ray& ray, Stack& st, HitState& h, uint32_t* cnt)
+__device__ __forceinline__ int trace_ray(const TraceScene& sc, const wf_ray& ray, uint32_t root, Stack& st, HitState& h, uint32_t* cnt)
 {
     h.inst = ~0u; h.prim = ~0u; h.t = kRayMaxT; h.u = 0.0f; h.v = 0.0f;
     float t = ray.tmax;
@@ -151,14 +151,12 @@ __device__ __forceinline__ int trace_ray(const TraceScene& sc, const wf_ray& ray
     wr.ix = 1.0f / wr.dx; wr.iy = 1.0f / wr.dy; wr.iz = 1.0f / wr.dz;
 
     const wf_tlas_node* tn = sc.tnodes;
-    {
-        const float4 a = ld4(&tn[0]);
-        const float4 b = ld4(reinterpret_cast<const char*>(&tn[0]) + 16);
-        if (COUNT) cnt[0]++;
-        if (slab(a.x, a.y, a.z, b.x, b.y, b.z, wr, t) == kRayMaxT) return kTraceDone;
-    }
-    uint32_t cur_lr = tn[0].left_right;
-    uint32_t cur_ii = tn[0].instance_idx;
+    const float4 ra = ld4(&tn[root]);
+    const float4 rb = ld4(reinterpret_cast<const char*>(&tn[root]) + 16);
+    if (COUNT) cnt[0]++;
+    if (slab(ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, wr, t) == kRayMaxT) return kTraceDone;
+    uint32_t cur_lr = __float_as_uint(ra.w);
+    uint32_t cur_ii = __float_as_uint(rb.w);
 
     bool in_blas = false;
     uint32_t sp = 0, base_sp = 0;
@@ -307,15 +305,16 @@ __global__ __launch_bounds__(BLOCK) void k_extend(ExtendLaunch L)
         const uint32_t i = base + lane;
         if (i >= n) continue;
         const wf_ray ray = L.rays[i];
+        const uint32_t root = L.ray_root ? L.ray_root[i] : 0u;
         HitState h;
         uint32_t cnt[4] = {0, 0, 0, 0};
         int rc;
         if constexpr (LDS > 0) {
             LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + (size_t)gtid * L.spill_per_thread, L.spill_per_thread};
-            rc = trace_ray<SKIP, COUNT, ANY>(L.scene, ray, st, h, cnt);
+            rc = trace_ray<SKIP, COUNT, ANY>(L.scene, ray, root, st, h, cnt);
         } else {
             PrivStack<128> st;
-            rc = trace_ray<SKIP, COUNT, ANY>(L.scene, ray, st, h, cnt);
+            rc = trace_ray<SKIP, COUNT, ANY>(L.scene, ray, root, st, h, cnt);
         }
         if (rc == kTraceOverflow) {
             atomicAdd(&L.work[1], 1u);

@@ -164,47 +164,76 @@ __device__ __forceinline__ void store_ray(wf_ray* q, uint32_t slot, V3 o, V3 d, 
 
 // ---- kernels ---------------------------------------------------------------------
 
+// scene slot holding global path id p (slots are sorted by path_off)
+__device__ __forceinline__ uint32_t scene_of_path(const FrameArgs& A, uint32_t p)
+{
+    uint32_t lo = A.first_scene, hi = A.first_scene + A.n_scenes - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (A.scenes[mid].path_off <= p) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t scene_of_pixel(const FrameArgs& A, uint32_t q)
+{
+    uint32_t lo = A.first_scene, hi = A.first_scene + A.n_scenes - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (A.scenes[mid].pix_off <= q) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
+// clear_buffers.hlsl:21-39: throughput 1, energy 0, queue 0 holds every path
 __global__ void k_clear(FrameArgs A)
 {
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < A.n_paths; p += stride) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n_paths; i += stride) {
+        const uint32_t p = A.path_base + i;
         A.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         A.energy[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     if (blockIdx.x == 0 && threadIdx.x < kCountSlots) A.counts[threadIdx.x] = (threadIdx.x == 0) ? A.n_paths : 0u;
 }
 
-// generate.hlsl + make_primary_ray (common.hlsl:156-169): path p covers
-// pixel p % (W*H); queue slot p holds path p.
+// generate.hlsl + make_primary_ray (common.hlsl:156-169): the local path q of
+// a scene covers pixel q % (W*H) (sample q / (W*H)); queue slot i holds path
+// path_base + i.
 __global__ void k_generate(FrameArgs A)
 {
     const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t npix = A.width * A.height;
-    const float* c2v = A.view.clip_to_view;
-    const float* v2w = A.view.view_to_world;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < A.n_paths; p += stride) {
-        const uint32_t pixel = p % npix;
-        const uint32_t px = pixel % A.width, py = pixel / A.width;
-        const float uvx = ((float)px + 0.5f) / A.view.render_dim[0];
-        float uvy = ((float)py + 0.5f) / A.view.render_dim[1];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n_paths; i += stride) {
+        const uint32_t p = A.path_base + i;
+        const uint32_t s = scene_of_path(A, p);
+        const GpuScene& sc = A.scenes[s];
+        const uint32_t pixel = (p - sc.path_off) % sc.n_pix;
+        const uint32_t px = pixel % sc.width, py = pixel / sc.width;
+        const float* c2v = sc.view.clip_to_view;
+        const float* v2w = sc.view.view_to_world;
+        const float uvx = ((float)px + 0.5f) / sc.view.render_dim[0];
+        float uvy = ((float)py + 0.5f) / sc.view.render_dim[1];
         uvy = 1.0f - uvy;
         const V3 cv = xform(c2v, V3{2.0f * uvx - 1.0f, 2.0f * uvy - 1.0f, 1.0f}, 1.0f);
         const V3 dv = normalize3(cv);
         const V3 dw = xform(v2w, dv, 0.0f);
         const V3 ow = xform(v2w, V3{0.0f, 0.0f, 0.0f}, 1.0f);
         const V3 o = V3{ow.x + dw.x * kRayMinT, ow.y + dw.y * kRayMinT, ow.z + dw.z * kRayMinT};
-        store_ray(A.rays[0], p, o, dw, kRayMinT, FLT_MAX);
-        A.path_of[0][p] = p;
+        store_ray(A.rays[0], i, o, dw, kRayMinT, FLT_MAX);
+        A.path_of[0][i] = p;
+        A.root_of[0][i] = sc.tlas_root;
+        A.path_scene[p] = s;
     }
 }
 
 // shade.hlsl:42-174 with the build's RNG, Russian roulette and next-event
-// estimation (see the host restatement for the exact operation order).
+// estimation (the host restatement has the same operation order).
 __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
 {
     const uint32_t n = A.counts[d];
     const uint32_t cur = d & 1u, nxt = cur ^ 1u;
     const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t seed = A.dyn->frame_seed;
     // iterate whole waves so that the ballot-based appends see converged lanes
     const uint32_t n_round = (n + 63u) & ~63u;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
@@ -213,17 +242,16 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
         V3 new_o{0, 0, 0}, new_d{0, 0, 0}, sh_o{0, 0, 0}, sh_d{0, 0, 0};
         float sh_tmax = 0.0f;
         float4 contrib = make_float4(0, 0, 0, 0);
-        uint32_t p = 0;
+        uint32_t p = 0, root = 0;
         if (active) {
             p = A.path_of[cur][i];
-            const float4 r1 = reinterpret_cast<const float4*>(A.rays[cur] + i)[1];
+            root = A.root_of[cur][i];
             const wf_hit hit = A.hits[i];
             const float4 t4 = A.thr[p];
             float4 e4 = A.energy[p];
             V3 thr{t4.x, t4.y, t4.z};
             float u[6];
-            path_rng(p, d, A.frame_seed, u);
-            (void)r1;
+            path_rng(p, d, seed, u);
             bool done = false;
             if (hit.instance_idx == 0xFFFFFFFFu) {
                 e4.x = e4.x + thr.x * A.env[0];
@@ -260,41 +288,45 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
                     }
                     done = true;
                 }
-                if (!done && A.nee && A.n_emitters > 0 && d < A.max_bounces) {
-                    uint32_t lo = 0, hi = A.n_emitters - 1u;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (__uint_as_float(A.emitters[mid].z) > u[3]) hi = mid; else lo = mid + 1u;
-                    }
-                    const uint4 er = A.emitters[lo];
-                    const GpuShadeInstance* li = A.shade_inst + er.x;
-                    const GpuShadeTri* lt = li->tris + er.y;
-                    const V3 W0 = xform(li->local_to_world, V3{lt->p[0][0], lt->p[0][1], lt->p[0][2]}, 1.0f);
-                    const V3 W1 = xform(li->local_to_world, V3{lt->p[1][0], lt->p[1][1], lt->p[1][2]}, 1.0f);
-                    const V3 W2 = xform(li->local_to_world, V3{lt->p[2][0], lt->p[2][1], lt->p[2][2]}, 1.0f);
-                    const float su = sqrtf(u[4]);
-                    const float bv = su * (1.0f - u[5]), bw = su * u[5];
-                    const V3 PL{interp(W0.x, W1.x, W2.x, bv, bw), interp(W0.y, W1.y, W2.y, bv, bw), interp(W0.z, W1.z, W2.z, bv, bw)};
-                    const V3 dl{PL.x - P.x, PL.y - P.y, PL.z - P.z};
-                    const V3 e1{W1.x - W0.x, W1.y - W0.y, W1.z - W0.z};
-                    const V3 e2{W2.x - W0.x, W2.y - W0.y, W2.z - W0.z};
-                    const float dist2 = dot3(dl, dl);
-                    const float dist = sqrtf(dist2);
-                    const float inv_d = 1.0f / dist;
-                    const V3 wi{dl.x * inv_d, dl.y * inv_d, dl.z * inv_d};
-                    const V3 nl = normalize3(cross3(e1, e2));
-                    const float cos_s = dot3(N, wi);
-                    const float cos_l = -dot3(nl, wi);
-                    if (cos_s > 0.0f && cos_l > 0.0f && dist2 > 0.0f) {
-                        const float G = (cos_s * cos_l) / dist2;
-                        const float wgt = G * A.emitter_area;
-                        contrib.x = (((thr.x * albedo.x) * kInvPi) * li->emissive[0]) * wgt;
-                        contrib.y = (((thr.y * albedo.y) * kInvPi) * li->emissive[1]) * wgt;
-                        contrib.z = (((thr.z * albedo.z) * kInvPi) * li->emissive[2]) * wgt;
-                        sh_o = V3{P.x + wi.x * kRayMinT, P.y + wi.y * kRayMinT, P.z + wi.z * kRayMinT};
-                        sh_d = wi;
-                        sh_tmax = dist * 0.9999f;
-                        shadow = true;
+                if (!done && A.nee && d < A.max_bounces) {
+                    const GpuScene& sc = A.scenes[A.path_scene[p]];
+                    if (sc.em_count > 0) {
+                        uint32_t lo = 0, hi = sc.em_count - 1u;
+                        const uint4* em_tab = A.emitters + sc.em_off;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (__uint_as_float(em_tab[mid].z) > u[3]) hi = mid; else lo = mid + 1u;
+                        }
+                        const uint4 er = em_tab[lo];
+                        const GpuShadeInstance* li = A.shade_inst + er.x;
+                        const GpuShadeTri* lt = li->tris + er.y;
+                        const V3 W0 = xform(li->local_to_world, V3{lt->p[0][0], lt->p[0][1], lt->p[0][2]}, 1.0f);
+                        const V3 W1 = xform(li->local_to_world, V3{lt->p[1][0], lt->p[1][1], lt->p[1][2]}, 1.0f);
+                        const V3 W2 = xform(li->local_to_world, V3{lt->p[2][0], lt->p[2][1], lt->p[2][2]}, 1.0f);
+                        const float su = sqrtf(u[4]);
+                        const float bv = su * (1.0f - u[5]), bw = su * u[5];
+                        const V3 PL{interp(W0.x, W1.x, W2.x, bv, bw), interp(W0.y, W1.y, W2.y, bv, bw), interp(W0.z, W1.z, W2.z, bv, bw)};
+                        const V3 dl{PL.x - P.x, PL.y - P.y, PL.z - P.z};
+                        const V3 e1{W1.x - W0.x, W1.y - W0.y, W1.z - W0.z};
+                        const V3 e2{W2.x - W0.x, W2.y - W0.y, W2.z - W0.z};
+                        const float dist2 = dot3(dl, dl);
+                        const float dist = sqrtf(dist2);
+                        const float inv_d = 1.0f / dist;
+                        const V3 wi{dl.x * inv_d, dl.y * inv_d, dl.z * inv_d};
+                        const V3 nl = normalize3(cross3(e1, e2));
+                        const float cos_s = dot3(N, wi);
+                        const float cos_l = -dot3(nl, wi);
+                        if (cos_s > 0.0f && cos_l > 0.0f && dist2 > 0.0f) {
+                            const float G = (cos_s * cos_l) / dist2;
+                            const float wgt = G * sc.em_area;
+                            contrib.x = (((thr.x * albedo.x) * kInvPi) * li->emissive[0]) * wgt;
+                            contrib.y = (((thr.y * albedo.y) * kInvPi) * li->emissive[1]) * wgt;
+                            contrib.z = (((thr.z * albedo.z) * kInvPi) * li->emissive[2]) * wgt;
+                            sh_o = V3{P.x + wi.x * kRayMinT, P.y + wi.y * kRayMinT, P.z + wi.z * kRayMinT};
+                            sh_d = wi;
+                            sh_tmax = dist * 0.9999f;
+                            shadow = true;
+                        }
                     }
                 }
                 if (!done) {
@@ -328,11 +360,13 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
         if (spawn) {
             store_ray(A.rays[nxt], slot, new_o, new_d, kRayMinT, FLT_MAX);
             A.path_of[nxt][slot] = p;
+            A.root_of[nxt][slot] = root;
         }
         const uint32_t sslot = wave_append(&A.counts[kShadowCountBase + d], shadow);
         if (shadow) {
             store_ray(A.shadow_rays, sslot, sh_o, sh_d, kRayMinT, sh_tmax);
             A.shadow_path[sslot] = p;
+            A.shadow_root[sslot] = root;
             A.shadow_contrib[sslot] = contrib;
         }
     }
@@ -342,20 +376,22 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
 __global__ void k_resolve_post(FrameArgs A)
 {
     const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t npix = A.width * A.height;
-    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += stride) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n_pix; i += stride) {
+        const uint32_t q = A.pix_base + i;
+        const GpuScene& sc = A.scenes[scene_of_pixel(A, q)];
+        const uint32_t local = q - sc.pix_off;
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         for (uint32_t s = 0; s < A.spp; ++s) {
-            const float4 e = A.energy[(size_t)s * npix + px];
+            const float4 e = A.energy[sc.path_off + s * sc.n_pix + local];
             ax = ax + e.x; ay = ay + e.y; az = az + e.z;
         }
         const float inv = (float)A.spp;
         float4 e = make_float4(ax / inv, ay / inv, az / inv, 1.0f);
-        A.image[px] = e;
+        A.image[q] = e;
         // badness detector: NaN / Inf -> magenta
         if (!(isfinite(e.x) && isfinite(e.y) && isfinite(e.z))) e = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
-        float4 acc = A.accum[px];
-        const float w = 1.0f / (float)A.sample_count;
+        float4 acc = A.accum[q];
+        const float w = 1.0f / (float)sc.sample_count;
         if (A.accumulate) {
             acc.x = acc.x * (1.0f - w) + e.x * w;
             acc.y = acc.y * (1.0f - w) + e.y * w;
@@ -364,9 +400,9 @@ __global__ void k_resolve_post(FrameArgs A)
         } else {
             acc = e;
         }
-        A.accum[px] = acc;
+        A.accum[q] = acc;
         // Reinhard, then linear -> sRGB (common.hlsl:251-259)
-        float c[3] = {acc.x / (1.0f + acc.x), acc.y / (1.0f + acc.y), acc.z / (1.0f + acc.z)};
+        const float c[3] = {acc.x / (1.0f + acc.x), acc.y / (1.0f + acc.y), acc.z / (1.0f + acc.z)};
         uint32_t rgba = 0;
         for (int k = 0; k < 3; ++k) {
             const float x = fminf(fmaxf(c[k], 0.0f), 1.0f);
@@ -375,7 +411,7 @@ __global__ void k_resolve_post(FrameArgs A)
             rgba |= b << (8 * k);
         }
         rgba |= 255u << 24;
-        A.final_rgba8[px] = rgba;
+        A.final_rgba8[q] = rgba;
     }
 }
 

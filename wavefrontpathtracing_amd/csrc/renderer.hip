@@ -87,6 +87,7 @@ static wf_render_settings default_settings()
     s.rr_survival = 0.75f;
     s.next_event_estimation = 0;
     s.frame_seed = 0;
+    s.use_hip_graph = 0;
     return s;
 }
 
@@ -147,7 +148,7 @@ extern "C" WF_API wf_status wf_renderer_exit(wf_renderer* r)
     for (auto& t : r->textures) t.texels.release();
     r->release_frame_buffers();
     r->d_tnodes.release(); r->d_inst.release(); r->d_shade_inst.release(); r->d_textures.release();
-    r->d_emitters.release();
+    r->d_emitters.release(); r->d_scenes.release(); r->d_dyn.release();
     r->d_work.release(); r->d_spill.release(); r->d_tmp_rays.release(); r->d_tmp_hits.release(); r->d_tmp_aux.release();
     for (int i = 0; i < kNumStageEvents; ++i) if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
     if (r->stream) (void)hipStreamDestroy(r->stream);
@@ -186,12 +187,13 @@ extern "C" WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_re
     if (!(s->rr_survival > 0.0f && s->rr_survival <= 1.0f)) return r->fail(WF_ERROR_INVALID_ARGUMENT, "rr_survival must be in (0, 1]");
     if (s->use_software_rt == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only software ray tracing is available");
     if (s->render_view_mode != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "debug view modes are not supported");
-    if (s->cosine_weighted_diffuse == 0 && s->next_event_estimation)
-        return r->fail(WF_ERROR_INVALID_ARGUMENT, "next-event estimation requires cosine-weighted sampling");
+    if (s->cosine_weighted_diffuse == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only cosine-weighted diffuse sampling is built");
+    if (s->use_wavefront_pathtracing == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only the wavefront path tracer is built");
     const bool reset = s->max_bounces != r->settings.max_bounces || s->spp != r->settings.spp ||
                        s->next_event_estimation != r->settings.next_event_estimation;
     r->settings = *s;
-    if (reset) r->accum_count = 0;
+    if (reset)
+        for (auto& sl : r->slots) sl.accum_count = 0;
     return WF_OK;
 }
 
@@ -207,6 +209,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     r->ext_cfg.block = c->workgroup_size;
     r->ext_cfg.lds_stack = c->lds_stack_entries;
     r->ext_cfg.skip_popped = c->skip_popped_interior ? 1u : 0u;
+    r->drop_graph();
     const wf_status s = r->ensure_trace_scratch();
     if (s != WF_OK) { r->ext_cfg = old; return s; }
     return WF_OK;
@@ -345,7 +348,11 @@ extern "C" WF_API wf_status wf_renderer_submit_render_mesh(wf_renderer* r, wf_me
     if (!r || !transform || !mat) return WF_ERROR_INVALID_ARGUMENT;
     Mesh* m = r->find_mesh(mh);
     if (!m) return r->fail(WF_ERROR_NOT_FOUND, "mesh handle {index %u, version %u} is not valid", handle_index(mh), handle_version(mh));
-    if (r->instances.size() >= kMaxInstances) return r->fail(WF_ERROR_CAPACITY, "exceeded capacity of instances (%u)", kMaxInstances);
+    if (!r->scene_open || r->cur_slot >= r->slots.size())
+        return r->fail(WF_ERROR_INVALID_STATE, "submit_render_mesh outside begin_scene / end_scene");
+    size_t total = 0;
+    for (const auto& sl : r->slots) total += sl.instances.size();
+    if (total >= kMaxInstances) return r->fail(WF_ERROR_CAPACITY, "exceeded capacity of instances (%u)", kMaxInstances);
     for (int i = 0; i < 16; ++i) if (!std::isfinite(transform[i])) return r->fail(WF_ERROR_INVALID_ARGUMENT, "non-finite transform");
     InstanceRec rec;
     rec.mesh = handle_index(mh);
@@ -353,58 +360,82 @@ extern "C" WF_API wf_status wf_renderer_submit_render_mesh(wf_renderer* r, wf_me
     rec.material = *mat;
     // software instance record (renderer.cpp:1318-1336)
     make_bvh_instance(transform, m->blas_min, m->blas_max, rec.mesh, rec.sw);
-    r->instances.push_back(rec);
+    r->slots[r->cur_slot].instances.push_back(rec);
     r->tlas_valid = false;
     return WF_OK;
 }
 
 // ---------------------------------------------------------------------------
-// TLAS build + upload (renderer.cpp:637-670)
+// TLAS build + upload (renderer.cpp:637-670), one TLAS per scene slot
 // ---------------------------------------------------------------------------
 
+// Every slot's TLAS is built by the reference algorithm, then the slots are
+// concatenated: child indices and instance indices are offset by the slot's
+// node and instance bases (both halves of left_right stay 16-bit), so slot 0
+// is unchanged and slot s is entered at its root node tlas_root.
 wf_status wf_renderer::upload_scene()
 {
     if (tlas_valid) return WF_OK;
-    if (instances.empty()) return fail(WF_ERROR_INVALID_STATE, "no instances submitted");
-    if (instances.size() > kMaxTlasInstances) return fail(WF_ERROR_CAPACITY, "%zu instances exceed the TLAS limit %u", instances.size(), kMaxTlasInstances);
-    std::vector<wf_bvh_instance> sw(instances.size());
-    for (size_t i = 0; i < instances.size(); ++i) sw[i] = instances[i].sw;
-    TlasData t;
-    wf_status s = build_tlas(sw.data(), (uint32_t)sw.size(), t);
-    if (s != WF_OK) return fail(s, "TLAS build failed");
-    tlas_host = t;
-
-    std::vector<GpuInstance> gi(instances.size());
-    std::vector<GpuShadeInstance> si(instances.size());
-    for (size_t i = 0; i < instances.size(); ++i) {
-        const InstanceRec& rec = instances[i];
-        const Mesh& m = meshes[rec.mesh];
-        for (int c = 0; c < 4; ++c)
-            for (int rr = 0; rr < 3; ++rr) gi[i].m[c * 3 + rr] = rec.sw.world_to_local[c * 4 + rr];
-        gi[i].nodes = (const wf_bvh_node*)m.nodes.p;
-        gi[i].tris = (const GpuTri*)m.tris.p;
-        GpuShadeInstance& sh = si[i];
-        std::memset(&sh, 0, sizeof(sh));
-        std::memcpy(sh.local_to_world, rec.transform, 64);
-        for (int k = 0; k < 3; ++k) {
-            sh.base_color[k] = rec.material.base_color_factor[k];
-            sh.emissive[k] = rec.material.emissive_factor[k] * rec.material.emissive_strength;
+    if (slots.empty()) return fail(WF_ERROR_INVALID_STATE, "no scene (call begin_scene)");
+    std::vector<wf_tlas_node> all_nodes;
+    std::vector<GpuInstance> gi;
+    std::vector<GpuShadeInstance> si;
+    for (auto& sl : slots) {
+        if (sl.instances.empty()) return fail(WF_ERROR_INVALID_STATE, "a scene has no instances submitted");
+        if (sl.instances.size() > kMaxTlasInstances)
+            return fail(WF_ERROR_CAPACITY, "%zu instances exceed the TLAS limit %u", sl.instances.size(), kMaxTlasInstances);
+        std::vector<wf_bvh_instance> sw(sl.instances.size());
+        for (size_t i = 0; i < sl.instances.size(); ++i) sw[i] = sl.instances[i].sw;
+        TlasData t;
+        wf_status s = build_tlas(sw.data(), (uint32_t)sw.size(), t);
+        if (s != WF_OK) return fail(s, "TLAS build failed");
+        const uint32_t node_base = (uint32_t)all_nodes.size();
+        const uint32_t inst_base = (uint32_t)gi.size();
+        if (node_base + t.node_count > 0xFFFFu)
+            return fail(WF_ERROR_CAPACITY, "the scenes' TLAS nodes exceed 65535 (16-bit child indices)");
+        sl.tlas_root = node_base;
+        sl.inst_base = inst_base;
+        std::memcpy(sl.bounds_min, t.nodes[0].aabb_min, 12);
+        std::memcpy(sl.bounds_max, t.nodes[0].aabb_max, 12);
+        for (uint32_t k = 0; k < t.node_count; ++k) {
+            wf_tlas_node nd = t.nodes[k];
+            if (nd.left_right == 0) nd.instance_idx += inst_base;
+            else nd.left_right = ((nd.left_right & 0xFFFFu) + node_base) + (((nd.left_right >> 16) + node_base) << 16);
+            all_nodes.push_back(nd);
         }
-        const Texture* bt = find_texture(rec.material.base_color_texture);
-        const Texture* et = find_texture(rec.material.emissive_texture);
-        sh.base_color_tex = bt ? (int32_t)handle_index(rec.material.base_color_texture) : -1;
-        sh.emissive_tex = et ? (int32_t)handle_index(rec.material.emissive_texture) : -1;
-        sh.tris = (const GpuShadeTri*)m.shade.p;
+        for (const InstanceRec& rec : sl.instances) {
+            const Mesh& m = meshes[rec.mesh];
+            GpuInstance g;
+            for (int c = 0; c < 4; ++c)
+                for (int rr = 0; rr < 3; ++rr) g.m[c * 3 + rr] = rec.sw.world_to_local[c * 4 + rr];
+            g.nodes = (const wf_bvh_node*)m.nodes.p;
+            g.tris = (const GpuTri*)m.tris.p;
+            gi.push_back(g);
+            GpuShadeInstance sh;
+            std::memset(&sh, 0, sizeof(sh));
+            std::memcpy(sh.local_to_world, rec.transform, 64);
+            for (int k = 0; k < 3; ++k) {
+                sh.base_color[k] = rec.material.base_color_factor[k];
+                sh.emissive[k] = rec.material.emissive_factor[k] * rec.material.emissive_strength;
+            }
+            const Texture* bt = find_texture(rec.material.base_color_texture);
+            const Texture* et = find_texture(rec.material.emissive_texture);
+            sh.base_color_tex = bt ? (int32_t)handle_index(rec.material.base_color_texture) : -1;
+            sh.emissive_tex = et ? (int32_t)handle_index(rec.material.emissive_texture) : -1;
+            sh.tris = (const GpuShadeTri*)m.shade.p;
+            si.push_back(sh);
+        }
     }
-    hipError_t e = d_tnodes.ensure(sizeof(wf_tlas_node) * t.node_count);
+    drop_graph();
+    hipError_t e = d_tnodes.ensure(sizeof(wf_tlas_node) * all_nodes.size());
     if (e == hipSuccess) e = d_inst.ensure(sizeof(GpuInstance) * gi.size());
     if (e == hipSuccess) e = d_shade_inst.ensure(sizeof(GpuShadeInstance) * si.size());
-    if (e == hipSuccess) e = hipMemcpyAsync(d_tnodes.p, t.nodes.data(), sizeof(wf_tlas_node) * t.node_count, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tnodes.p, all_nodes.data(), sizeof(wf_tlas_node) * all_nodes.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_inst.p, gi.data(), sizeof(GpuInstance) * gi.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_shade_inst.p, si.data(), sizeof(GpuShadeInstance) * si.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "TLAS upload");
-    s = upload_materials_and_emitters();
+    wf_status s = upload_materials_and_emitters();
     if (s != WF_OK) return s;
     tlas_valid = true;
     return WF_OK;
@@ -420,9 +451,9 @@ extern "C" WF_API wf_status wf_renderer_build_tlas(wf_renderer* r)
 extern "C" WF_API wf_status wf_renderer_get_scene_bounds(const wf_renderer* r, float out_min[3], float out_max[3])
 {
     if (!r || !out_min || !out_max) return WF_ERROR_INVALID_ARGUMENT;
-    if (!r->tlas_valid || r->tlas_host.nodes.empty()) return WF_ERROR_INVALID_STATE;
-    std::memcpy(out_min, r->tlas_host.nodes[0].aabb_min, 12);
-    std::memcpy(out_max, r->tlas_host.nodes[0].aabb_max, 12);
+    if (!r->tlas_valid || r->slots.empty()) return WF_ERROR_INVALID_STATE;
+    std::memcpy(out_min, r->slots[0].bounds_min, 12);
+    std::memcpy(out_max, r->slots[0].bounds_max, 12);
     return WF_OK;
 }
 
@@ -431,7 +462,7 @@ extern "C" WF_API wf_status wf_renderer_get_scene_bounds(const wf_renderer* r, f
 // ---------------------------------------------------------------------------
 
 wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
-                                  uint32_t* d_visible, uint32_t* d_counters, bool any)
+                                  uint32_t* d_visible, uint32_t* d_counters, bool any, const uint32_t* d_roots)
 {
     hipError_t e = launch_fill_u32((uint32_t*)d_work.p, 0u, 1, stream);   // work cursor; word 1 counts overflows
     if (e != hipSuccess) return hip_fail(e, "work counter reset");
@@ -439,6 +470,7 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     L.scene.tnodes = (const wf_tlas_node*)d_tnodes.p;
     L.scene.inst = (const GpuInstance*)d_inst.p;
     L.rays = d_rays;
+    L.ray_root = d_roots;
     L.d_count = d_count;
     L.n = n;
     L.hits = d_hits;

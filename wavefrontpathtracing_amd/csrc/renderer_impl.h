@@ -2,6 +2,7 @@
 // re-expressed over HIP device memory).
 #pragma once
 
+#include "stages.h"
 #include "wf_internal.h"
 
 #include <cstdarg>
@@ -44,6 +45,22 @@ struct InstanceRec {
     wf_bvh_instance sw;
 };
 
+// One scene of a frame: camera, resolution and the instances submitted to
+// it (renderer::begin_scene .. end_scene makes one; a batch holds many).
+struct SceneSlot {
+    wf_camera camera{};
+    wf_view view{};
+    uint32_t width = 0, height = 0;
+    std::vector<InstanceRec> instances;
+    uint32_t accum_count = 0;
+    // assigned by upload_scene
+    uint32_t tlas_root = 0, inst_base = 0, em_off = 0, em_count = 0;
+    float em_area = 0.0f;
+    float bounds_min[3] = {0, 0, 0}, bounds_max[3] = {0, 0, 0};
+    // assigned per frame
+    uint32_t path_off = 0, pix_off = 0;
+};
+
 } // namespace wf
 
 struct wf_renderer {
@@ -59,25 +76,25 @@ struct wf_renderer {
     std::vector<wf::Mesh> meshes;
     std::vector<wf::Texture> textures;
     bool textures_dirty = true;
-    std::vector<wf::InstanceRec> instances;
 
     // scene / frame state
     bool frame_open = false, scene_open = false;
-    wf_camera camera{};
-    wf_view view{};
-    bool view_valid = false;
+    std::vector<wf::SceneSlot> slots;   // slot 0 is the begin_scene scene
+    uint32_t cur_slot = 0;
     uint64_t env_texture = WF_INVALID_HANDLE;
     uint32_t xorshift_seed = 0xC977E154u;   // core/random.h:7
     uint64_t frame_index = 0;
-    uint32_t accum_count = 0;
     wf_frame_stats stats{};
 
-    // TLAS + instance records on the device
-    wf::TlasData tlas_host;
-    wf::DevBuf d_tnodes, d_inst, d_shade_inst, d_textures, d_emitters;
-    uint32_t n_emitters = 0;
-    float emitter_area = 0.0f;
+    // TLAS + instance records of every slot, concatenated on the device
+    wf::DevBuf d_tnodes, d_inst, d_shade_inst, d_textures, d_emitters, d_scenes, d_dyn;
     bool tlas_valid = false;
+    std::vector<uint8_t> h_scenes;   // staging of the per-frame scene table
+    uint32_t h_dyn[4] = {0, 0, 0, 0};
+
+    // captured frame graph
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
 
     // traversal scratch
     wf::ExtendConfig ext_cfg;
@@ -86,10 +103,11 @@ struct wf_renderer {
     wf::DevBuf d_work, d_spill, d_tmp_rays, d_tmp_hits, d_tmp_aux;
 
     // wavefront buffers (allocated on first render)
-    wf::DevBuf d_rays[2], d_paths[2], d_hits, d_thr, d_path_energy, d_counts;
-    wf::DevBuf d_shadow_rays, d_shadow_path, d_shadow_contrib, d_shadow_visible;
+    wf::DevBuf d_rays[2], d_paths[2], d_roots[2], d_hits, d_thr, d_path_energy, d_path_scene, d_counts;
+    wf::DevBuf d_shadow_rays, d_shadow_path, d_shadow_root, d_shadow_contrib, d_shadow_visible;
     wf::DevBuf d_image, d_accum, d_final;
     size_t frame_capacity = 0;
+    size_t image_pixels = 0;
     bool image_valid = false;
 
     wf_status fail(wf_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
@@ -100,10 +118,12 @@ struct wf_renderer {
     wf_status upload_scene();
     wf_status upload_materials_and_emitters();
     wf_status run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
-                         uint32_t* d_visible, uint32_t* d_counters, bool any);
+                         uint32_t* d_visible, uint32_t* d_counters, bool any, const uint32_t* d_roots = nullptr);
     wf_status read_overflows(uint32_t* out);
     wf_status reset_overflows();
     wf_status ensure_frame_buffers(size_t n_paths, size_t n_pixels);
-    wf_status render_frame();
+    wf_status render_range(uint32_t first, uint32_t count);
+    wf_status record_frame(const wf::FrameArgs& A, uint32_t grid, bool timed, int* n_events);
     void release_frame_buffers();
+    void drop_graph();
 };

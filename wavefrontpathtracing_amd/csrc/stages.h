@@ -1,4 +1,11 @@
-// Arguments of the wavefront stage kernels (one struct per frame).
+// Arguments of the wavefront stage kernels.
+//
+// A frame renders a contiguous range of scene slots in one set of launches
+// (a single scene is the range of length one).  Paths of all scenes share
+// the queues; each queue entry carries the TLAS root of its scene, each path
+// its scene id.  Everything that changes from frame to frame (seed,
+// accumulation weights) lives in device memory, so a captured hipGraph of
+// the frame stays valid across frames.
 #pragma once
 
 #include "wf_internal.h"
@@ -10,33 +17,55 @@ namespace wf {
 constexpr uint32_t kCountSlots = 32;
 constexpr uint32_t kShadowCountBase = 16;
 
-struct FrameArgs {
-    uint32_t width, height, spp, n_paths;
+// one scene slot (device table)
+struct alignas(16) GpuScene {
     wf_view view;
-    uint32_t frame_seed, max_bounces, rr_start, nee;
+    uint32_t width, height, n_pix, pix_off;     // pixels [pix_off, pix_off + n_pix) of the batch images
+    uint32_t path_off, n_paths;                 // global path ids [path_off, path_off + n_paths)
+    uint32_t tlas_root;                         // global TLAS node index of the scene root
+    uint32_t em_off, em_count;                  // emitter records [em_off, em_off + em_count)
+    float em_area;                              // total emitter area of the scene
+    uint32_t sample_count;                      // accumulation weight 1/sample_count this frame
+    uint32_t pad[3];
+};
+
+struct FrameDyn {
+    uint32_t frame_seed;
+    uint32_t pad[3];
+};
+
+struct FrameArgs {
+    // the scene range of this frame
+    const GpuScene* scenes;   // indexed by scene slot
+    uint32_t first_scene, n_scenes;
+    uint32_t path_base, n_paths;   // global path ids [path_base, path_base + n_paths)
+    uint32_t pix_base, n_pix;      // batch pixels [pix_base, pix_base + n_pix)
+    uint32_t spp, max_bounces, rr_start, nee;
     float rr_survival;
     float env[3];
-    uint32_t n_emitters;
-    float emitter_area;
-    uint32_t accumulate, sample_count;
+    uint32_t accumulate;
+    const FrameDyn* dyn;
 
-    float4* thr;                 // per path
-    float4* energy;              // per path
+    float4* thr;                 // per global path
+    float4* energy;              // per global path
+    uint32_t* path_scene;        // per global path
     wf_ray* rays[2];             // ping-pong ray queues
-    uint32_t* path_of[2];        // queue slot -> path id
+    uint32_t* path_of[2];        // queue slot -> global path id
+    uint32_t* root_of[2];        // queue slot -> TLAS root node
     wf_hit* hits;                // extend output for the current queue
     uint32_t* counts;            // kCountSlots
     wf_ray* shadow_rays;
     uint32_t* shadow_path;
+    uint32_t* shadow_root;
     float4* shadow_contrib;
     uint32_t* shadow_visible;
 
     const GpuShadeInstance* shade_inst;
     const GpuTexture* textures;
-    const uint4* emitters;       // (instance, primitive, cdf bits, 0)
+    const uint4* emitters;       // (instance, primitive, cdf bits, 0), per-scene ranges
 
-    float4* image;               // resolved energy per pixel
-    float4* accum;               // progressive accumulator
+    float4* image;               // resolved energy per batch pixel
+    float4* accum;               // progressive accumulator per batch pixel
     uint32_t* final_rgba8;
 };
 

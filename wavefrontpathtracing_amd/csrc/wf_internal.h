@@ -67,6 +67,7 @@ struct ExtendConfig {
 struct ExtendLaunch {
     TraceScene scene;
     const wf_ray* rays;
+    const uint32_t* ray_root;  // per-ray TLAS root node (batched scenes), or null for root 0
     const uint32_t* d_count;
     uint32_t n;
     wf_hit* hits;              // AoS output (may be null when hit_* SoA used)

@@ -124,6 +124,28 @@ class Renderer:
     def end_scene(self):
         self._check(lib().wf_renderer_end_scene(self.handle), "end_scene")
 
+    # ---- batches (many scenes per frame) ----
+    def begin_batch(self):
+        self._check(lib().wf_renderer_begin_batch(self.handle), "begin_batch")
+
+    def batch_add_scene(self, camera: L.Camera, width: int, height: int) -> int:
+        out = C.c_uint32()
+        self._check(lib().wf_renderer_batch_add_scene(self.handle, C.byref(camera), width, height, C.byref(out)), "batch_add_scene")
+        return out.value
+
+    def scene_count(self) -> int:
+        out = C.c_uint32()
+        self._check(lib().wf_renderer_get_scene_count(self.handle, C.byref(out)), "get_scene_count")
+        return out.value
+
+    def render_scenes(self, first: int, count: int):
+        self._check(lib().wf_renderer_render_scenes(self.handle, first, count), "render_scenes")
+
+    def read_scene_energy(self, scene: int, width: int, height: int) -> np.ndarray:
+        out = np.empty((height, width, 4), dtype=np.float32)
+        self._check(lib().wf_renderer_read_scene_energy(self.handle, scene, ptr(out)), "read_scene_energy")
+        return out
+
     # ---- resources ----
     def create_render_texture(self, width: int, height: int, data: np.ndarray, fmt: int = TEXTURE_FORMAT_RGB32F) -> int:
         data = np.ascontiguousarray(data)

@@ -177,6 +177,57 @@ def small_tlas_scene(seed: int = 12, width: int = 160, height: int = 120) -> Sce
                       soup_tris=3000, emitters=4)
 
 
+def batch_scenes(seed: int = 4, n_scenes: int = 64, tri_min: int = 10_000, tri_max: int = 2_000_000,
+                 resolutions=(256, 512, 1024)):
+    """Uneven scenes for the batched renderer: each a random-triangle soup
+    (log-uniform triangle count in [tri_min, tri_max]) inside a 12-quad box
+    room of half-size 1.5 lit by one emissive ceiling quad, a square
+    resolution drawn from `resolutions`, and its own camera inside the room."""
+    rng = Rng(seed, 0x5eed0020)
+    room_v, room_i = box_room()
+    quad_v, quad_i = quad()
+    out = []
+    lo, hi = np.log(tri_min), np.log(tri_max)
+    for s in range(n_scenes):
+        n = int(np.exp(lo + (hi - lo) * rng.float()))
+        res = resolutions[rng.next() % len(resolutions)]
+        ang = 2.0 * np.pi * rng.float()
+        eye = (1.35 * np.sin(ang), 0.2 + 0.4 * rng.float(), 1.35 * np.cos(ang))
+        soup = random_soup(seed * 1000 + s, n)
+        meshes = [soup, (room_v, room_i), (quad_v, quad_i)]
+        inst = [Instance(0, identity(), base_color=(rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8))),
+                Instance(1, translate_scale((0.0, 0.0, 0.0), (1.5, 1.5, 1.5)), base_color=(0.7, 0.7, 0.7)),
+                Instance(2, translate_scale((0.0, 1.45, 0.0), (1.2, 1.0, 1.2)), base_color=(0.0, 0.0, 0.0),
+                         emissive=(1.0, 1.0, 1.0), emissive_strength=6.0)]
+        out.append(Scene(meshes=meshes, instances=inst, eye=eye, target=(0.0, 0.0, 0.0), vfov_deg=70.0,
+                         width=res, height=res, name=f"batch{s}"))
+    return out
+
+
+def load_batch(renderer, scene_list):
+    """begin_batch, then per scene: create its meshes (the room and light
+    meshes are shared when they are the same arrays), open a slot and submit."""
+    from .renderer import Material, camera_create
+    cache = {}
+    handles = []
+    for sc in scene_list:
+        hs = []
+        for v, i in sc.meshes:
+            key = (v.ctypes.data, len(v), i.ctypes.data, len(i))
+            if key not in cache:
+                cache[key] = renderer.create_render_mesh(v, i)
+            hs.append(cache[key])
+        handles.append(hs)
+    renderer.begin_batch()
+    for sc, hs in zip(scene_list, handles):
+        renderer.batch_add_scene(camera_create(sc.eye, sc.target, sc.vfov_deg), sc.width, sc.height)
+        for ins in sc.instances:
+            mat = Material(base_color_factor=(*ins.base_color, 1.0), emissive_factor=ins.emissive,
+                           emissive_strength=ins.emissive_strength)
+            renderer.submit_render_mesh(hs[ins.mesh], ins.transform, mat)
+    renderer.end_scene()
+
+
 def load(renderer, scene: Scene, begin: bool = True):
     """Create meshes/textures and submit instances (the caller side of
     renderer::create_render_mesh / submit_render_mesh, core/scene.cpp:92-105)."""
