@@ -11,7 +11,19 @@
 /* glm::min / glm::max on floats and ints (extern/glm/detail/func_common.inl:17-35). */
 static inline float gmin(float x, float y) { return (y < x) ? y : x; }
 static inline float gmax(float x, float y) { return (x < y) ? y : x; }
-static inline int32_t gmini(int32_t x, int32_t y) { return (y < x) ? y : x; }
+/* Bin of a centroid: min(B-1, (int32_t)((c - lo) * scale)) as in
+ * bvh_builder.cpp:194-195,130-131.  When the centroid extent is so small
+ * that scale overflows, the product is inf or NaN and the reference's
+ * float->int conversion is undefined; here it saturates (inf -> B-1,
+ * NaN -> 0).  For finite products the value is the reference's. */
+static inline int32_t bin_of(float c, float lo, float scale, uint32_t bins)
+{
+    float f = (c - lo) * scale;
+    if (!(f >= 0.0f)) return 0;
+    if (f >= (float)bins) return (int32_t)bins - 1;
+    int32_t b = (int32_t)f;
+    return b < (int32_t)bins - 1 ? b : (int32_t)bins - 1;
+}
 
 /* as_util::get_aabb_volume (as_util.h:7-11): half surface area. */
 static inline float half_area(const float mn[3], const float mx[3])
@@ -81,7 +93,7 @@ static float find_best_split_plane(blas_state* b, const orc_node* node, const fl
         for (uint32_t i = node->left_first; i < node->left_first + node->prim_count; ++i) {
             const orc_tri* t = &b->tris[b->idx[i]];
             const float* c = &b->cen[3 * (size_t)b->idx[i]];
-            int32_t bi = gmini((int32_t)B - 1, (int32_t)((c[axis] - lo) * scale));
+            int32_t bi = bin_of(c[axis], lo, scale, B);
             b->bin_cnt[bi]++;
             const float* ps[3] = { t->p0, t->p1, t->p2 };
             for (int v = 0; v < 3; ++v)
@@ -156,7 +168,7 @@ static int subdivide_all(blas_state* b, uint32_t root)
         float scale = (float)b->bins / (cmax[axis] - cmin[axis]);
         while (i <= j) {
             const float* c = &b->cen[3 * (size_t)b->idx[i]];
-            int32_t bi = gmini((int32_t)b->bins - 1, (int32_t)((c[axis] - cmin[axis]) * scale));
+            int32_t bi = bin_of(c[axis], cmin[axis], scale, b->bins);
             if ((uint32_t)bi < split) {
                 i++;
             } else {

@@ -86,6 +86,23 @@ def test_blas_single_prim_mode():
     assert nodes["prim_count"].max() == 1
 
 
+def test_blas_denormal_centroid_extent():
+    """A chain of geometrically shrinking triangles reaches denormal centroid
+    extents, where B / (max - min) overflows; the bin index saturates instead
+    of the reference's undefined float->int conversion (regression: this
+    input crashed both builders)."""
+    n = 240
+    pos = np.zeros((n, 9), np.float32)
+    for k in range(n):
+        x = np.float32(2.0) ** np.float32(-k * 0.5)
+        s = x * np.float32(0.01)
+        pos[k] = [x, -s, 0, x, s, s, x, s, -s]
+    for single in (False, True):
+        a = R.bvh_build(pos, 8, subdivide_single_prim=single)
+        assert a == O.blas_blob(pos, 8, single_prim=single)
+        _check_blas_invariants(a, pos)
+
+
 def test_blas_two_triangles_closed_form():
     """Two triangles far apart on x: the root splits on x, left child (node 2)
     holds the triangle with the smaller centroid, right child (node 3) the other."""

@@ -21,6 +21,19 @@ namespace wf {
 
 namespace {
 
+// Bin of a centroid, min(B-1, int((c - lo) * scale)) (bvh_builder.cpp:194-195).
+// A centroid extent so small that `scale` overflows makes the product inf or
+// NaN, where the reference's float->int conversion is undefined; the bin then
+// saturates (inf -> B-1, NaN -> 0).  Finite products give the reference's bin.
+inline int32_t bin_of(float c, float lo, float scale, uint32_t bins)
+{
+    const float f = (c - lo) * scale;
+    if (!(f >= 0.0f)) return 0;
+    if (f >= (float)bins) return (int32_t)bins - 1;
+    const int32_t b = (int32_t)f;
+    return b < (int32_t)bins - 1 ? b : (int32_t)bins - 1;
+}
+
 struct TmpNode {
     float mn[3], mx[3];
     uint32_t first, count;   // triangle range (leaf) / original range (interior)
@@ -70,8 +83,7 @@ struct BlasBuilder {
             const float* c = cen[axis].data();
             for (uint32_t i = nd.first, e = nd.first + nd.count; i < e; ++i) {
                 const uint32_t t = idx[i];
-                int32_t b = (int32_t)((c[t] - lo) * scale);
-                if ((int32_t)B - 1 < b) b = (int32_t)B - 1;
+                const int32_t b = bin_of(c[t], lo, scale, B);
                 bc[b]++;
                 for (int k = 0; k < 3; ++k) {
                     bmn[3 * b + k] = gmin(bmn[3 * b + k], tmin[k][t]);
@@ -128,8 +140,7 @@ struct BlasBuilder {
             const float lo = cmn[axis];
             const float* c = cen[axis].data();
             while (i <= j) {
-                int32_t b = (int32_t)((c[idx[i]] - lo) * scale);
-                if ((int32_t)bins - 1 < b) b = (int32_t)bins - 1;
+                const int32_t b = bin_of(c[idx[i]], lo, scale, bins);
                 if ((uint32_t)b < split) ++i;
                 else { std::swap(idx[i], idx[j]); --j; }
             }

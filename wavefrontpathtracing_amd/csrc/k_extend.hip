@@ -105,8 +105,10 @@ struct LdsStack {
     float* dist;
     uint2* spill;
     uint32_t spill_n;
+    uint32_t cap;
     __device__ __forceinline__ bool push(uint32_t sp, uint32_t e, float d)
     {
+        if (sp >= cap) return false;
         if (sp < (uint32_t)LDS) { ent[sp * BLOCK] = e; dist[sp * BLOCK] = d; return true; }
         const uint32_t k = sp - LDS;
         if (k >= spill_n) return false;
@@ -125,9 +127,10 @@ template <int N>
 struct PrivStack {
     uint32_t ent[N];
     float dist[N];
+    uint32_t cap;
     __device__ __forceinline__ bool push(uint32_t sp, uint32_t e, float d)
     {
-        if (sp >= (uint32_t)N) return false;
+        if (sp >= (uint32_t)N || sp >= cap) return false;
         ent[sp] = e; dist[sp] = d;
         return true;
     }
@@ -310,10 +313,12 @@ __global__ __launch_bounds__(BLOCK) void k_extend(ExtendLaunch L)
         uint32_t cnt[4] = {0, 0, 0, 0};
         int rc;
         if constexpr (LDS > 0) {
-            LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + (size_t)gtid * L.spill_per_thread, L.spill_per_thread};
+            LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + (size_t)gtid * L.spill_per_thread, L.spill_per_thread,
+                                    L.stack_cap};
             rc = trace_ray<SKIP, COUNT, ANY>(L.scene, ray, root, st, h, cnt);
         } else {
             PrivStack<128> st;
+            st.cap = L.stack_cap;
             rc = trace_ray<SKIP, COUNT, ANY>(L.scene, ray, root, st, h, cnt);
         }
         if (rc == kTraceOverflow) {

@@ -209,6 +209,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     r->ext_cfg.block = c->workgroup_size;
     r->ext_cfg.lds_stack = c->lds_stack_entries;
     r->ext_cfg.skip_popped = c->skip_popped_interior ? 1u : 0u;
+    r->ext_cfg.stack_limit = c->stack_limit;
     r->drop_graph();
     const wf_status s = r->ensure_trace_scratch();
     if (s != WF_OK) { r->ext_cfg = old; return s; }
@@ -222,6 +223,7 @@ extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* 
     out->lds_stack_entries = r->ext_cfg.lds_stack;
     out->skip_popped_interior = r->ext_cfg.skip_popped;
     out->bvh_layout = 0;
+    out->stack_limit = r->ext_cfg.stack_limit;
     return WF_OK;
 }
 
@@ -479,6 +481,7 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     L.work = (uint32_t*)d_work.p;
     L.spill = (uint2*)d_spill.p;
     L.spill_per_thread = spill_per_thread;
+    L.stack_cap = ext_cfg.stack_limit ? std::min(ext_cfg.stack_limit, kStackTotal) : kStackTotal;
     L.grid = ext_grid;
     L.any_hit = any ? 1 : 0;
     e = launch_extend(L, ext_cfg, stream);

@@ -61,6 +61,7 @@ struct ExtendConfig {
     uint32_t block = 128;       // workgroup size: 64, 128, 256
     uint32_t lds_stack = 16;    // LDS stack entries per lane (0 = private/scratch stack)
     uint32_t skip_popped = 1;   // skip popped interior nodes whose entry distance >= t
+    uint32_t stack_limit = 0;   // cap on stack entries per lane (0 = capacity)
 };
 
 // Launch the closest-hit kernel.  d_count (device) overrides n when non-null.
@@ -76,6 +77,7 @@ struct ExtendLaunch {
     uint32_t* work;            // [0] next ray, [1] stack overflows
     uint2* spill;              // global overflow stack
     uint32_t spill_per_thread;
+    uint32_t stack_cap;        // entries a lane may hold (LDS + spill, or private)
     uint32_t grid;             // persistent blocks
     int any_hit;
 };

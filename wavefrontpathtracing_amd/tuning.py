@@ -7,7 +7,6 @@ from __future__ import annotations
 
 import ctypes as C
 import itertools
-import time
 
 from . import _lib as L
 
@@ -15,15 +14,16 @@ DEFAULT = dict(block=128, lds_stack=16, skip_popped=1)
 
 
 def set_extend_config(renderer, block: int = DEFAULT["block"], lds_stack: int = DEFAULT["lds_stack"],
-                      skip_popped: int = DEFAULT["skip_popped"], bvh_layout: int = 0) -> None:
-    c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout)
+                      skip_popped: int = DEFAULT["skip_popped"], bvh_layout: int = 0, stack_limit: int = 0) -> None:
+    c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout, stack_limit)
     L.check(L.lib().wf_renderer_set_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "set_traversal_config")
 
 
 def get_extend_config(renderer) -> dict:
     c = L.TraversalConfig()
     L.check(L.lib().wf_renderer_get_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "get_traversal_config")
-    return dict(block=c.workgroup_size, lds_stack=c.lds_stack_entries, skip_popped=c.skip_popped_interior, bvh_layout=c.bvh_layout)
+    return dict(block=c.workgroup_size, lds_stack=c.lds_stack_entries, skip_popped=c.skip_popped_interior,
+                bvh_layout=c.bvh_layout, stack_limit=c.stack_limit)
 
 
 def candidates():
