@@ -143,6 +143,48 @@ int orc_render(const orc_scene* s, const orc_shading* sh, const orc_view* v, uin
  * (call with NULL outputs for the count) and the total world area. */
 uint32_t orc_emitters(const orc_shading* sh, uint32_t* ids, float* cdf, float* total_area);
 
+/* ---- differentiable rendering (FP64 reference) -------------------------------
+ * The FP32 path walker of orc_render records, per path vertex, everything
+ * that does not depend on albedo or emission (paths are sampled
+ * independently of both: cosine sampling, constant Russian roulette).  The
+ * FP64 evaluator re-weights the recorded paths for any texel values and
+ * emission strengths; the FP64 adjoint is path-replay backpropagation over
+ * the same records. */
+enum { ORC_V_END = 0, ORC_V_MISS = 1, ORC_V_EMIT = 2, ORC_V_DIFFUSE = 3 };
+typedef struct {
+    uint32_t kind;          /* ORC_V_* */
+    uint32_t inst;          /* hit instance */
+    uint32_t emit_counted;  /* ORC_V_EMIT: emission added (no NEE, or camera ray) */
+    uint32_t rr;            /* throughput divided by rr_survival after this vertex */
+    float    noL, inv_pdf;  /* bounce weight (NoL * (a / pi)) * inv_pdf */
+    int32_t  tex;           /* base-color texture of the instance, -1 none */
+    uint32_t texel[4];      /* bilinear corners (x0,y0) (x1,y0) (x0,y1) (x1,y1), texel index in the texture */
+    float    fx, fy;        /* bilinear fractions */
+    uint32_t nee;           /* 1: a visible shadow-ray connection to light_inst */
+    uint32_t light_inst;
+    float    nee_wgt;       /* G * total emitter area */
+} orc_vertex;
+
+/* Walk every path once (FP32 geometry, as orc_render); rec holds
+ * (max_bounces + 1) vertices per path, terminated by ORC_V_END. */
+int orc_record_paths(const orc_scene* s, const orc_shading* sh, const orc_view* v, uint32_t width, uint32_t height,
+                     const orc_render_params* p, orc_vertex* rec, int nthreads);
+
+/* FP64 re-weighting of recorded paths.  texels: RGBA doubles of texture
+ * `tex` (other textures use their float texels); strength: per instance,
+ * the emissive_strength (emission = emissive_factor * strength).
+ * image: width*height*3 doubles (mean over spp). */
+int orc_eval_f64(const orc_shading* sh, const orc_render_params* p, uint32_t width, uint32_t height,
+                 const orc_vertex* rec, int32_t tex, const double* texels, const double* factor3,
+                 const double* strength, double* image);
+
+/* FP64 path-replay adjoint: given dLoss/dimage (width*height*3), the
+ * gradients with respect to the texels of `tex` (RGBA doubles) and the
+ * per-instance emission strengths. */
+int orc_adjoint_f64(const orc_shading* sh, const orc_render_params* p, uint32_t width, uint32_t height,
+                    const orc_vertex* rec, int32_t tex, const double* texels, const double* factor3,
+                    const double* strength, const double* dl_dimage, double* d_texels, double* d_strength);
+
 /* RNG and trigonometry used by shade (exposed for known-answer tests). */
 uint32_t orc_pcg32_first(uint64_t initstate, uint64_t initseq);
 void     orc_path_rng(uint32_t path, uint32_t bounce, uint32_t frame_seed, float out[6]);

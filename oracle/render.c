@@ -155,8 +155,9 @@ void orc_cosine_sample(const float n[3], float r0, float r1, float out[3])
     for (int k = 0; k < 3; ++k) out[k] = (vn[0] * t[k] + vn[1] * n[k]) + vn[2] * b[k];
 }
 
-/* bilinear, wrap addressing (the sampler of renderer.cpp:341-355, exact weights) */
-static void tex_sample(const orc_shading* sh, int32_t tex, float u, float v, float out[3])
+/* bilinear, wrap addressing (the sampler of renderer.cpp:341-355, exact
+ * weights); optionally reports the corners and fractions */
+static void tex_sample(const orc_shading* sh, int32_t tex, float u, float v, float out[3], orc_vertex* rec)
 {
     const float* t = sh->tex_rgba + sh->tex_off[tex];
     uint32_t w = sh->tex_w[tex], h = sh->tex_h[tex];
@@ -170,6 +171,11 @@ static void tex_sample(const orc_shading* sh, int32_t tex, float u, float v, flo
     int64_t ix = (int64_t)fx0, iy = (int64_t)fy0;
     uint32_t x0 = (uint32_t)(((ix % (int64_t)w) + w) % w), y0 = (uint32_t)(((iy % (int64_t)h) + h) % h);
     uint32_t x1 = (x0 + 1u == w) ? 0u : x0 + 1u, y1 = (y0 + 1u == h) ? 0u : y0 + 1u;
+    if (rec) {
+        rec->texel[0] = y0 * w + x0; rec->texel[1] = y0 * w + x1;
+        rec->texel[2] = y1 * w + x0; rec->texel[3] = y1 * w + x1;
+        rec->fx = fx; rec->fy = fy;
+    }
     for (int k = 0; k < 3; ++k) {
         float a = t[((size_t)y0 * w + x0) * 4 + k], b = t[((size_t)y0 * w + x1) * 4 + k];
         float c = t[((size_t)y1 * w + x0) * 4 + k], d = t[((size_t)y1 * w + x1) * 4 + k];
@@ -229,6 +235,7 @@ typedef struct {
     uint32_t w, h; float* path_energy;
     const uint32_t* em_ids; const float* em_cdf; uint32_t n_em; float em_area;
     uint32_t first, count, stride; int rc;
+    orc_vertex* rec;   /* optional: (max_bounces + 1) records per path */
 } render_job;
 
 static void primary_ray(const orc_view* v, uint32_t px, uint32_t py, orc_ray* r)
@@ -248,7 +255,9 @@ static void primary_ray(const orc_view* v, uint32_t px, uint32_t py, orc_ray* r)
     r->tmax = FLT_MAX;
 }
 
-static int trace_path(render_job* J, uint32_t path, float energy[3])
+/* One path; rec (max_bounces + 1 entries, may be NULL) receives the vertex
+ * records for the FP64 evaluator. */
+static int trace_path(render_job* J, uint32_t path, float energy[3], orc_vertex* rec)
 {
     const orc_render_params* P = J->p;
     const orc_shading* sh = J->sh;
@@ -259,7 +268,9 @@ static int trace_path(render_job* J, uint32_t path, float energy[3])
     primary_ray(J->v, pixel % J->w, pixel / J->w, &ray);
     float thr[3] = { 1.0f, 1.0f, 1.0f };
     energy[0] = energy[1] = energy[2] = 0.0f;
+    if (rec) memset(rec, 0, sizeof(orc_vertex) * (P->max_bounces + 1));
     for (uint32_t d = 0; d <= P->max_bounces; ++d) {
+        orc_vertex* vr = rec ? &rec[d] : NULL;
         orc_hit hit;
         int rc = orc_trace_one(J->s, &ray, &hit, 0);
         if (rc) return rc;
@@ -267,8 +278,10 @@ static int trace_path(render_job* J, uint32_t path, float energy[3])
         orc_path_rng(path, d, P->frame_seed, u);
         if (hit.instance_idx == ~0u) {
             for (int k = 0; k < 3; ++k) energy[k] = energy[k] + thr[k] * sh->env[k];
+            if (vr) vr->kind = ORC_V_MISS;
             break;
         }
+        if (vr) { vr->inst = hit.instance_idx; vr->tex = -1; }
         const orc_material_instance* m = &sh->mi[hit.instance_idx];
         const float* t = sh->shade_tris + 24 * ((size_t)sh->shade_off[m->mesh] + hit.primitive_idx);
         float bx = hit.bary[0], by = hit.bary[1];
@@ -285,14 +298,17 @@ static int trace_path(render_job* J, uint32_t path, float energy[3])
         float albedo[3] = { m->base_color[0], m->base_color[1], m->base_color[2] };
         if (m->base_color_tex >= 0) {
             float tc[3];
-            tex_sample(sh, m->base_color_tex, uv[0], uv[1], tc);
+            tex_sample(sh, m->base_color_tex, uv[0], uv[1], tc, vr);
             for (int k = 0; k < 3; ++k) albedo[k] = albedo[k] * tc[k];
+            if (vr) vr->tex = m->base_color_tex;
         }
         if (is_emissive(m)) {
             if (!P->nee || d == 0)
                 for (int k = 0; k < 3; ++k) energy[k] = energy[k] + thr[k] * m->emissive[k];
+            if (vr) { vr->kind = ORC_V_EMIT; vr->emit_counted = (!P->nee || d == 0) ? 1u : 0u; }
             break;
         }
+        if (vr) vr->kind = ORC_V_DIFFUSE;
         if (P->nee && J->n_em > 0 && d < P->max_bounces) {
             /* emitter by area: first k with cdf[k] > u[3] */
             uint32_t lo = 0, hi = J->n_em - 1;
@@ -333,8 +349,10 @@ static int trace_path(render_job* J, uint32_t path, float energy[3])
                 orc_hit sh_hit;
                 rc = orc_trace_one(J->s, &sr, &sh_hit, 1);
                 if (rc) return rc;
-                if (sh_hit.instance_idx == ~0u)
+                if (sh_hit.instance_idx == ~0u) {
                     for (int k = 0; k < 3; ++k) energy[k] = energy[k] + c[k];
+                    if (vr) { vr->nee = 1; vr->light_inst = li; vr->nee_wgt = wgt; }
+                }
             }
         }
         float L[3];
@@ -345,9 +363,11 @@ static int trace_path(render_job* J, uint32_t path, float energy[3])
         float pdf = NoL * INV_PI;
         float inv_pdf = 1.0f / pdf;
         for (int k = 0; k < 3; ++k) thr[k] = thr[k] * ((NoL * (albedo[k] * INV_PI)) * inv_pdf);
+        if (vr) { vr->noL = NoL; vr->inv_pdf = inv_pdf; }
         if (d >= P->rr_start) {
             if (u[2] >= P->rr_survival) break;
             for (int k = 0; k < 3; ++k) thr[k] = thr[k] / P->rr_survival;
+            if (vr) vr->rr = 1;
         }
         if (d >= P->max_bounces) break;
         for (int k = 0; k < 3; ++k) { ray.origin[k] = P3[k] + L[k] * RAY_MIN_T; ray.dir[k] = L[k]; }
@@ -362,16 +382,78 @@ static void* render_worker(void* arg)
     render_job* J = (render_job*)arg;
     J->rc = ORC_OK;
     for (uint32_t p = J->first; p < J->count; p += J->stride) {
-        int rc = trace_path(J, p, &J->path_energy[3 * (size_t)p]);
+        orc_vertex* rec = J->rec ? J->rec + (size_t)p * (J->p->max_bounces + 1) : NULL;
+        int rc = trace_path(J, p, &J->path_energy[3 * (size_t)p], rec);
         if (rc) J->rc = rc;
     }
     return NULL;
+}
+
+static int run_paths(const orc_scene* s, const orc_shading* sh, const orc_view* v, uint32_t width, uint32_t height,
+                     const orc_render_params* p, float* pe, orc_vertex* rec, int nthreads)
+{
+    uint32_t npath = width * height * p->spp;
+    uint32_t n_em = orc_emitters(sh, NULL, NULL, NULL);
+    uint32_t* ids = (uint32_t*)malloc(sizeof(uint32_t) * 2 * (n_em ? n_em : 1));
+    float* cdf = (float*)malloc(sizeof(float) * (n_em ? n_em : 1));
+    if (!ids || !cdf) { free(ids); free(cdf); return ORC_ECAP; }
+    float area = 0.0f;
+    orc_emitters(sh, ids, cdf, &area);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    render_job jobs[64];
+    pthread_t th[64];
+    for (int t = 0; t < nthreads; ++t) {
+        render_job j = { s, sh, v, p, width, height, pe, ids, cdf, n_em, area, (uint32_t)t, npath, (uint32_t)nthreads, 0, rec };
+        jobs[t] = j;
+    }
+    int rc = ORC_OK;
+    if (nthreads == 1) {
+        render_worker(&jobs[0]);
+        rc = jobs[0].rc;
+    } else {
+        for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, render_worker, &jobs[t]);
+        for (int t = 0; t < nthreads; ++t) { pthread_join(th[t], NULL); if (jobs[t].rc) rc = jobs[t].rc; }
+    }
+    free(ids); free(cdf);
+    return rc;
+}
+
+int orc_record_paths(const orc_scene* s, const orc_shading* sh, const orc_view* v, uint32_t width, uint32_t height,
+                     const orc_render_params* p, orc_vertex* rec, int nthreads)
+{
+    if (!s || !sh || !v || !p || !rec || width == 0 || height == 0 || p->spp == 0) return ORC_EINVAL;
+    float* pe = (float*)malloc(sizeof(float) * 3 * (size_t)width * height * p->spp);
+    if (!pe) return ORC_ECAP;
+    int rc = run_paths(s, sh, v, width, height, p, pe, rec, nthreads);
+    free(pe);
+    return rc;
 }
 
 int orc_render(const orc_scene* s, const orc_shading* sh, const orc_view* v, uint32_t width, uint32_t height,
                const orc_render_params* p, float* energy_rgba, int nthreads)
 {
     if (!s || !sh || !v || !p || !energy_rgba || width == 0 || height == 0 || p->spp == 0) return ORC_EINVAL;
+    uint32_t npix = width * height;
+    uint32_t npath = npix * p->spp;
+    float* pe = (float*)malloc(sizeof(float) * 3 * (size_t)npath);
+    if (!pe) return ORC_ECAP;
+    int rc = run_paths(s, sh, v, width, height, p, pe, NULL, nthreads);
+    /* resolve: mean over the spp paths of each pixel, samples in order */
+    for (uint32_t px = 0; px < npix; ++px) {
+        float acc[3] = { 0.0f, 0.0f, 0.0f };
+        for (uint32_t sidx = 0; sidx < p->spp; ++sidx)
+            for (int k = 0; k < 3; ++k) acc[k] = acc[k] + pe[3 * ((size_t)sidx * npix + px) + k];
+        for (int k = 0; k < 3; ++k) energy_rgba[4 * (size_t)px + k] = acc[k] / (float)p->spp;
+        energy_rgba[4 * (size_t)px + 3] = 1.0f;
+    }
+    free(pe);
+    return rc;
+}
+
+static int orc_render_unused(const orc_scene* s, const orc_shading* sh, const orc_view* v, uint32_t width, uint32_t height,
+               const orc_render_params* p, float* energy_rgba, int nthreads)
+{
     uint32_t npix = width * height;
     uint32_t npath = npix * p->spp;
     float* pe = (float*)malloc(sizeof(float) * 3 * (size_t)npath);
