@@ -160,6 +160,32 @@ WF_API wf_status wf_renderer_render_scenes(wf_renderer* r, uint32_t first, uint3
 /* linear energy of one scene slot: width*height RGBA float */
 WF_API wf_status wf_renderer_read_scene_energy(wf_renderer* r, uint32_t scene, float* out_rgba);
 
+/* ---- differentiable rendering (no reference counterpart) -----------------
+ * Path-replay adjoint (Vicini, Speierer, Jakob, SIGGRAPH 2021) of the last
+ * render(): every path of that frame is replayed with the same random
+ * numbers and the gradients of a loss are accumulated, given dLoss/dimage
+ * (RGBA floats per pixel of the frame's image, all scenes of a batch
+ * concatenated; alpha ignored):
+ *   grad_texels   dLoss/d(texel) of base-color texture `texture` (RGBA floats,
+ *                 width*height*4; pass WF_INVALID_HANDLE to skip), summed
+ *                 over every instance and scene that samples it;
+ *   grad_emission dLoss/d(emissive_strength) per instance (all scenes'
+ *                 instances in submission order).
+ * Russian roulette and sampling do not depend on albedo or emission, so the
+ * replay follows the forward paths exactly.  Requires that render() ran last
+ * with the same scene, settings, texture data and emission strengths. */
+WF_API wf_status wf_renderer_render_adjoint(wf_renderer* r, const float* dl_dimage, wf_texture_handle texture,
+                                           float* out_grad_texels, float* out_grad_emission);
+WF_API wf_status wf_renderer_render_adjoint_device(wf_renderer* r, const float* d_dl_dimage, wf_texture_handle texture,
+                                                  float* d_grad_texels, float* d_grad_emission);
+/* in-place texture update (RGBA floats, same size) and emission strength of
+ * one instance (index over all scenes' instances), for optimisation loops */
+WF_API wf_status wf_renderer_update_texture(wf_renderer* r, wf_texture_handle texture, const float* rgba);
+WF_API wf_status wf_renderer_update_texture_device(wf_renderer* r, wf_texture_handle texture, const float* d_rgba);
+WF_API wf_status wf_renderer_set_emission_strength(wf_renderer* r, uint32_t instance, float strength);
+/* device-to-device copy of the whole frame image (all scenes' pixels) */
+WF_API wf_status wf_renderer_read_energy_device(wf_renderer* r, float* d_out_rgba);
+
 /* ---- resources: create_render_texture / create_render_mesh / submit_render_mesh
  *      (renderer.cpp:1150-1350) ---------------------------------------- */
 WF_API wf_status wf_renderer_create_render_texture(wf_renderer* r, const wf_render_texture_params* params, wf_texture_handle* out_handle);

@@ -451,42 +451,3 @@ int orc_render(const orc_scene* s, const orc_shading* sh, const orc_view* v, uin
     return rc;
 }
 
-static int orc_render_unused(const orc_scene* s, const orc_shading* sh, const orc_view* v, uint32_t width, uint32_t height,
-               const orc_render_params* p, float* energy_rgba, int nthreads)
-{
-    uint32_t npix = width * height;
-    uint32_t npath = npix * p->spp;
-    float* pe = (float*)malloc(sizeof(float) * 3 * (size_t)npath);
-    if (!pe) return ORC_ECAP;
-    uint32_t n_em = orc_emitters(sh, NULL, NULL, NULL);
-    uint32_t* ids = (uint32_t*)malloc(sizeof(uint32_t) * 2 * (n_em ? n_em : 1));
-    float* cdf = (float*)malloc(sizeof(float) * (n_em ? n_em : 1));
-    float area = 0.0f;
-    orc_emitters(sh, ids, cdf, &area);
-    if (nthreads < 1) nthreads = 1;
-    if (nthreads > 64) nthreads = 64;
-    render_job jobs[64];
-    pthread_t th[64];
-    for (int t = 0; t < nthreads; ++t) {
-        render_job j = { s, sh, v, p, width, height, pe, ids, cdf, n_em, area, (uint32_t)t, npath, (uint32_t)nthreads, 0 };
-        jobs[t] = j;
-    }
-    int rc = ORC_OK;
-    if (nthreads == 1) {
-        render_worker(&jobs[0]);
-        rc = jobs[0].rc;
-    } else {
-        for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, render_worker, &jobs[t]);
-        for (int t = 0; t < nthreads; ++t) { pthread_join(th[t], NULL); if (jobs[t].rc) rc = jobs[t].rc; }
-    }
-    /* resolve: mean over the spp paths of each pixel, samples in order */
-    for (uint32_t px = 0; px < npix; ++px) {
-        float acc[3] = { 0.0f, 0.0f, 0.0f };
-        for (uint32_t sidx = 0; sidx < p->spp; ++sidx)
-            for (int k = 0; k < 3; ++k) acc[k] = acc[k] + pe[3 * ((size_t)sidx * npix + px) + k];
-        for (int k = 0; k < 3; ++k) energy_rgba[4 * (size_t)px + k] = acc[k] / (float)p->spp;
-        energy_rgba[4 * (size_t)px + 3] = 1.0f;
-    }
-    free(pe); free(ids); free(cdf);
-    return rc;
-}

@@ -17,6 +17,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
 COUNTERS = np.dtype([("tlas_nodes", "<u4"), ("blas_nodes", "<u4"), ("instances", "<u4"), ("tri_tests", "<u4")])
+VERTEX_REC = np.dtype([("kind", "<u4"), ("inst", "<u4"), ("emit_counted", "<u4"), ("rr", "<u4"), ("noL", "<f4"),
+                       ("inv_pdf", "<f4"), ("tex", "<i4"), ("texel", "<u4", 4), ("fx", "<f4"), ("fy", "<f4"),
+                       ("nee", "<u4"), ("light_inst", "<u4"), ("nee_wgt", "<f4")])
+assert VERTEX_REC.itemsize == 64
 
 
 class OrcScene(C.Structure):
@@ -77,6 +81,12 @@ def olib():
             "orc_sin_turns": (f32, [f32]),
             "orc_cos_turns": (f32, [f32]),
             "orc_cosine_sample": (None, [vp, f32, f32, vp]),
+            "orc_record_paths": (C.c_int, [C.POINTER(OrcScene), C.POINTER(OrcShading), C.POINTER(OrcView), u32, u32,
+                                           C.POINTER(OrcRenderParams), vp, C.c_int]),
+            "orc_eval_f64": (C.c_int, [C.POINTER(OrcShading), C.POINTER(OrcRenderParams), u32, u32, vp, C.c_int32,
+                                       vp, vp, vp, vp]),
+            "orc_adjoint_f64": (C.c_int, [C.POINTER(OrcShading), C.POINTER(OrcRenderParams), u32, u32, vp, C.c_int32,
+                                          vp, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(o, name)
@@ -257,11 +267,64 @@ class OracleScene:
             raise RuntimeError(f"orc_render rc={rc}")
         return out
 
+    # ---- FP64 differentiable reference ----
+    def record(self, max_bounces=3, spp=1, rr_start=3, rr_survival=0.75, nee=0, frame_seed=1, env=1.0, nthreads=8,
+               path_base=0):
+        """Walk every path once (FP32 geometry) and keep the vertex records;
+        returns a Differentiable over them."""
+        sc = self.scene
+        sh = self.shading(env)
+        view = camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
+        prm = OrcRenderParams(max_bounces, spp, rr_start, nee, frame_seed, rr_survival, path_base)
+        rec = np.zeros(sc.width * sc.height * spp * (max_bounces + 1), dtype=VERTEX_REC)
+        rc = olib().orc_record_paths(C.byref(self.s), C.byref(sh), C.byref(view), sc.width, sc.height, C.byref(prm),
+                                     p(rec), nthreads)
+        if rc != 0:
+            raise RuntimeError(f"orc_record_paths rc={rc}")
+        return Differentiable(self, sh, prm, rec)
+
     def brute_closest(self, rays: np.ndarray) -> np.ndarray:
         rays = np.ascontiguousarray(rays, dtype=L.RAY)
         hits = np.zeros(len(rays), dtype=L.HIT)
         olib().orc_brute_closest(C.byref(self.s), p(rays), len(rays), p(hits))
         return hits
+
+
+class Differentiable:
+    """FP64 image and gradients of recorded paths for given texels of one
+    texture (RGBA float64) and per-instance emission strengths."""
+
+    def __init__(self, orc, sh, prm, rec):
+        self.orc, self.sh, self.prm, self.rec = orc, sh, prm, rec
+        sc = orc.scene
+        self.w, self.h = sc.width, sc.height
+        self.factor3 = np.ascontiguousarray([list(i.emissive) for i in sc.instances], dtype=np.float64).reshape(-1)
+        self.strength0 = np.array([i.emissive_strength for i in sc.instances], dtype=np.float64)
+
+    def texels0(self, tex: int = 0) -> np.ndarray:
+        w, h, rgb = self.orc.scene.textures[tex]
+        return np.ascontiguousarray(np.concatenate([rgb.reshape(-1, 3), np.ones((w * h, 1), np.float32)], axis=1),
+                                    dtype=np.float64)
+
+    def image(self, texels=None, strength=None, tex: int = 0) -> np.ndarray:
+        texels = self.texels0(tex) if texels is None else np.ascontiguousarray(texels, dtype=np.float64)
+        strength = self.strength0 if strength is None else np.ascontiguousarray(strength, dtype=np.float64)
+        out = np.zeros((self.h, self.w, 3), dtype=np.float64)
+        rc = olib().orc_eval_f64(C.byref(self.sh), C.byref(self.prm), self.w, self.h, p(self.rec), tex, p(texels),
+                                 p(self.factor3), p(strength), p(out))
+        assert rc == 0
+        return out
+
+    def adjoint(self, dl_dimage, texels=None, strength=None, tex: int = 0):
+        texels = self.texels0(tex) if texels is None else np.ascontiguousarray(texels, dtype=np.float64)
+        strength = self.strength0 if strength is None else np.ascontiguousarray(strength, dtype=np.float64)
+        dl = np.ascontiguousarray(dl_dimage, dtype=np.float64).reshape(self.h, self.w, 3)
+        d_tex = np.zeros_like(texels)
+        d_str = np.zeros_like(strength)
+        rc = olib().orc_adjoint_f64(C.byref(self.sh), C.byref(self.prm), self.w, self.h, p(self.rec), tex, p(texels),
+                                    p(self.factor3), p(strength), p(dl), p(d_tex), p(d_str))
+        assert rc == 0
+        return d_tex, d_str
 
 
 def hits_equal(a: np.ndarray, b: np.ndarray):

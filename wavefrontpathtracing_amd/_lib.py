@@ -119,6 +119,12 @@ SIGNATURES = {
     "wf_renderer_get_scene_count": (_st, [_vp, C.POINTER(_u32)]),
     "wf_renderer_render_scenes": (_st, [_vp, _u32, _u32]),
     "wf_renderer_read_scene_energy": (_st, [_vp, _u32, _vp]),
+    "wf_renderer_render_adjoint": (_st, [_vp, _vp, _u64, _vp, _vp]),
+    "wf_renderer_render_adjoint_device": (_st, [_vp, _vp, _u64, _vp, _vp]),
+    "wf_renderer_update_texture": (_st, [_vp, _u64, _vp]),
+    "wf_renderer_update_texture_device": (_st, [_vp, _u64, _vp]),
+    "wf_renderer_set_emission_strength": (_st, [_vp, _u32, _f32]),
+    "wf_renderer_read_energy_device": (_st, [_vp, _vp]),
     "wf_renderer_create_render_texture": (_st, [_vp, C.POINTER(RenderTextureParams), C.POINTER(_u64)]),
     "wf_renderer_create_render_mesh": (_st, [_vp, C.POINTER(RenderMeshParams), C.POINTER(_u64)]),
     "wf_renderer_submit_render_mesh": (_st, [_vp, _u64, _vp, C.POINTER(MaterialAsset)]),

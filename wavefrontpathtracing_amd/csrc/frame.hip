@@ -120,6 +120,7 @@ void wf_renderer::drop_graph()
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     graph_exec = nullptr;
     graph_key.clear();
+    forward_valid = false;
 }
 
 void wf_renderer::release_frame_buffers()
@@ -435,6 +436,8 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
     st.stack_overflows = ov;
     stats = st;
     image_valid = true;
+    last_args = A;
+    forward_valid = true;
     for (uint32_t k = first; k < first + count; ++k) slots[k].accum_count++;
     if (ov) return fail(WF_ERROR_STACK_OVERFLOW, "%u rays overflowed the traversal stack this frame", ov);
     return WF_OK;

@@ -226,8 +226,35 @@ __global__ void k_generate(FrameArgs A)
     }
 }
 
+// bilinear corners and fractions of tex_sample, for scattering gradients
+__device__ __forceinline__ void tex_corners(const GpuTexture& tx, float u, float v, uint32_t c[4], float& ax, float& ay)
+{
+    float fx = u * (float)tx.width - 0.5f, fy = v * (float)tx.height - 0.5f;
+    fx = fminf(fmaxf(fx, -1e9f), 1e9f);
+    fy = fminf(fmaxf(fy, -1e9f), 1e9f);
+    const float fx0 = floorf(fx), fy0 = floorf(fy);
+    ax = fx - fx0;
+    ay = fy - fy0;
+    const int64_t ix = (int64_t)fx0, iy = (int64_t)fy0;
+    const int64_t w = tx.width, h = tx.height;
+    const uint32_t x0 = (uint32_t)(((ix % w) + w) % w), y0 = (uint32_t)(((iy % h) + h) % h);
+    const uint32_t x1 = (x0 + 1u == (uint32_t)w) ? 0u : x0 + 1u, y1 = (y0 + 1u == (uint32_t)h) ? 0u : y0 + 1u;
+    c[0] = y0 * (uint32_t)w + x0; c[1] = y0 * (uint32_t)w + x1;
+    c[2] = y1 * (uint32_t)w + x0; c[3] = y1 * (uint32_t)w + x1;
+}
+
 // shade.hlsl:42-174 with the build's RNG, Russian roulette and next-event
 // estimation (the host restatement has the same operation order).
+//
+// ADJ = true is the path-replay adjoint (Vicini, Speierer, Jakob 2021): the
+// same path is replayed with the same random numbers; instead of adding
+// contributions to the path's energy it subtracts them from lrem (the
+// path's forward estimate minus what was collected before), and since every
+// contribution is linear in the albedo of each vertex it passes,
+// dEstimate/dalbedo_d = lrem / albedo_d at vertex d, scattered to the
+// texture's bilinear corners; emission gradients are the collected
+// contributions per unit emission strength.
+template <bool ADJ>
 __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
 {
     const uint32_t n = A.counts[d];
@@ -241,22 +268,29 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
         bool spawn = false, shadow = false;
         V3 new_o{0, 0, 0}, new_d{0, 0, 0}, sh_o{0, 0, 0}, sh_d{0, 0, 0};
         float sh_tmax = 0.0f;
-        float4 contrib = make_float4(0, 0, 0, 0);
-        uint32_t p = 0, root = 0;
+        float4 contrib = make_float4(0, 0, 0, 0), unit = make_float4(0, 0, 0, 0);
+        uint32_t p = 0, root = 0, light = 0;
         if (active) {
             p = A.path_of[cur][i];
             root = A.root_of[cur][i];
             const wf_hit hit = A.hits[i];
             const float4 t4 = A.thr[p];
-            float4 e4 = A.energy[p];
+            float4 e4 = ADJ ? A.lrem[p] : A.energy[p];
+            const float4 dlt = ADJ ? A.delta[p] : make_float4(0, 0, 0, 0);
             V3 thr{t4.x, t4.y, t4.z};
             float u[6];
             path_rng(p, d, seed, u);
             bool done = false;
             if (hit.instance_idx == 0xFFFFFFFFu) {
-                e4.x = e4.x + thr.x * A.env[0];
-                e4.y = e4.y + thr.y * A.env[1];
-                e4.z = e4.z + thr.z * A.env[2];
+                if (ADJ) {
+                    e4.x = e4.x - thr.x * A.env[0];
+                    e4.y = e4.y - thr.y * A.env[1];
+                    e4.z = e4.z - thr.z * A.env[2];
+                } else {
+                    e4.x = e4.x + thr.x * A.env[0];
+                    e4.y = e4.y + thr.y * A.env[1];
+                    e4.z = e4.z + thr.z * A.env[2];
+                }
                 done = true;
             }
             if (!done) {
@@ -279,14 +313,42 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
                     const V3 tc = tex_sample(A.textures[si->base_color_tex], tu, tv);
                     albedo = V3{albedo.x * tc.x, albedo.y * tc.y, albedo.z * tc.z};
                 }
-                const V3 em{si->emissive[0], si->emissive[1], si->emissive[2]};
+                const float es = si->emissive_strength;
+                const V3 ef{si->emissive_factor[0], si->emissive_factor[1], si->emissive_factor[2]};
+                const V3 em{ef.x * es, ef.y * es, ef.z * es};
                 if (em.x != 0.0f || em.y != 0.0f || em.z != 0.0f) {
                     if (!A.nee || d == 0) {
-                        e4.x = e4.x + thr.x * em.x;
-                        e4.y = e4.y + thr.y * em.y;
-                        e4.z = e4.z + thr.z * em.z;
+                        if (ADJ) {
+                            e4.x = e4.x - thr.x * em.x;
+                            e4.y = e4.y - thr.y * em.y;
+                            e4.z = e4.z - thr.z * em.z;
+                            const float g = (dlt.x * (thr.x * ef.x) + dlt.y * (thr.y * ef.y)) + dlt.z * (thr.z * ef.z);
+                            if (g != 0.0f) atomicAdd(&A.grad_em[hit.instance_idx], g);
+                        } else {
+                            e4.x = e4.x + thr.x * em.x;
+                            e4.y = e4.y + thr.y * em.y;
+                            e4.z = e4.z + thr.z * em.z;
+                        }
                     }
                     done = true;
+                }
+                if (ADJ && !done && si->base_color_tex == A.grad_tex_slot && si->base_color_tex >= 0) {
+                    // d estimate / d texel: (lrem / albedo) * base_color * bilinear weight
+                    uint32_t c[4];
+                    float ax, ay;
+                    tex_corners(A.textures[si->base_color_tex], tu, tv, c, ax, ay);
+                    const float wts[4] = {(1.0f - ax) * (1.0f - ay), ax * (1.0f - ay), (1.0f - ax) * ay, ax * ay};
+                    const float g3[3] = {albedo.x > 0.0f ? dlt.x * e4.x / albedo.x * si->base_color[0] : 0.0f,
+                                         albedo.y > 0.0f ? dlt.y * e4.y / albedo.y * si->base_color[1] : 0.0f,
+                                         albedo.z > 0.0f ? dlt.z * e4.z / albedo.z * si->base_color[2] : 0.0f};
+                    if (g3[0] != 0.0f || g3[1] != 0.0f || g3[2] != 0.0f) {
+                        for (int j = 0; j < 4; ++j) {
+                            float* gt = reinterpret_cast<float*>(A.grad_tex + c[j]);
+                            atomicAdd(gt + 0, g3[0] * wts[j]);
+                            atomicAdd(gt + 1, g3[1] * wts[j]);
+                            atomicAdd(gt + 2, g3[2] * wts[j]);
+                        }
+                    }
                 }
                 if (!done && A.nee && d < A.max_bounces) {
                     const GpuScene& sc = A.scenes[A.path_scene[p]];
@@ -319,9 +381,16 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
                         if (cos_s > 0.0f && cos_l > 0.0f && dist2 > 0.0f) {
                             const float G = (cos_s * cos_l) / dist2;
                             const float wgt = G * sc.em_area;
-                            contrib.x = (((thr.x * albedo.x) * kInvPi) * li->emissive[0]) * wgt;
-                            contrib.y = (((thr.y * albedo.y) * kInvPi) * li->emissive[1]) * wgt;
-                            contrib.z = (((thr.z * albedo.z) * kInvPi) * li->emissive[2]) * wgt;
+                            const float ls = li->emissive_strength;
+                            contrib.x = (((thr.x * albedo.x) * kInvPi) * (li->emissive_factor[0] * ls)) * wgt;
+                            contrib.y = (((thr.y * albedo.y) * kInvPi) * (li->emissive_factor[1] * ls)) * wgt;
+                            contrib.z = (((thr.z * albedo.z) * kInvPi) * (li->emissive_factor[2] * ls)) * wgt;
+                            if (ADJ) {
+                                unit.x = (((thr.x * albedo.x) * kInvPi) * li->emissive_factor[0]) * wgt;
+                                unit.y = (((thr.y * albedo.y) * kInvPi) * li->emissive_factor[1]) * wgt;
+                                unit.z = (((thr.z * albedo.z) * kInvPi) * li->emissive_factor[2]) * wgt;
+                                light = er.x;
+                            }
                             sh_o = V3{P.x + wi.x * kRayMinT, P.y + wi.y * kRayMinT, P.z + wi.z * kRayMinT};
                             sh_d = wi;
                             sh_tmax = dist * 0.9999f;
@@ -354,7 +423,8 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
                 }
             }
             A.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
-            A.energy[p] = e4;
+            if (ADJ) A.lrem[p] = e4;
+            else A.energy[p] = e4;
         }
         const uint32_t slot = wave_append(&A.counts[d + 1], spawn);
         if (spawn) {
@@ -368,6 +438,10 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
             A.shadow_path[sslot] = p;
             A.shadow_root[sslot] = root;
             A.shadow_contrib[sslot] = contrib;
+            if (ADJ) {
+                A.shadow_unit[sslot] = unit;
+                A.shadow_light[sslot] = light;
+            }
         }
     }
 }
@@ -430,7 +504,61 @@ __global__ void k_connect_accumulate(FrameArgs A, uint32_t d)
     }
 }
 
+// adjoint: throughput 1, lrem = the path's forward estimate, delta = dLoss /
+// d(pixel) / spp for the path's pixel
+__global__ void k_adjoint_clear(FrameArgs A)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n_paths; i += stride) {
+        const uint32_t p = A.path_base + i;
+        const GpuScene& sc = A.scenes[scene_of_path(A, p)];
+        const uint32_t pixel = (p - sc.path_off) % sc.n_pix;
+        const float4 g = A.dl_dimage[sc.pix_off + pixel];
+        const float inv = (float)A.spp;
+        A.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        A.lrem[p] = A.energy[p];
+        A.delta[p] = make_float4(g.x / inv, g.y / inv, g.z / inv, 0.0f);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kCountSlots) A.counts[threadIdx.x] = (threadIdx.x == 0) ? A.n_paths : 0u;
+}
+
+// adjoint connect: collected shadow contributions leave lrem and feed the
+// emission-strength gradient of their emitter
+__global__ void k_adjoint_connect(FrameArgs A, uint32_t d)
+{
+    const uint32_t n = A.counts[kShadowCountBase + d];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (!A.shadow_visible[i]) continue;
+        const uint32_t p = A.shadow_path[i];
+        const float4 c = A.shadow_contrib[i];
+        const float4 un = A.shadow_unit[i];
+        const float4 dl = A.delta[p];
+        float4 e = A.lrem[p];
+        e.x = e.x - c.x; e.y = e.y - c.y; e.z = e.z - c.z;
+        A.lrem[p] = e;
+        const float g = (dl.x * un.x + dl.y * un.y) + dl.z * un.z;
+        if (g != 0.0f) atomicAdd(&A.grad_em[A.shadow_light[i]], g);
+    }
+}
+
 } // namespace
+
+hipError_t launch_adjoint_clear(const FrameArgs& A, uint32_t grid, hipStream_t s)
+{
+    k_adjoint_clear<<<grid, 256, 0, s>>>(A);
+    return hipGetLastError();
+}
+hipError_t launch_adjoint_shade(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s)
+{
+    k_shade<true><<<grid, 256, 0, s>>>(A, d);
+    return hipGetLastError();
+}
+hipError_t launch_adjoint_connect(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s)
+{
+    k_adjoint_connect<<<grid, 256, 0, s>>>(A, d);
+    return hipGetLastError();
+}
 
 hipError_t launch_clear(const FrameArgs& A, uint32_t grid, hipStream_t s)
 {
@@ -444,7 +572,7 @@ hipError_t launch_generate(const FrameArgs& A, uint32_t grid, hipStream_t s)
 }
 hipError_t launch_shade(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s)
 {
-    k_shade<<<grid, 256, 0, s>>>(A, d);
+    k_shade<false><<<grid, 256, 0, s>>>(A, d);
     return hipGetLastError();
 }
 hipError_t launch_connect_accumulate(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s)

@@ -149,6 +149,8 @@ extern "C" WF_API wf_status wf_renderer_exit(wf_renderer* r)
     r->release_frame_buffers();
     r->d_tnodes.release(); r->d_inst.release(); r->d_shade_inst.release(); r->d_textures.release();
     r->d_emitters.release(); r->d_scenes.release(); r->d_dyn.release();
+    r->d_lrem.release(); r->d_delta.release(); r->d_shadow_unit.release(); r->d_shadow_light.release();
+    r->d_adj_dl.release(); r->d_adj_tex.release(); r->d_adj_em.release();
     r->d_work.release(); r->d_spill.release(); r->d_tmp_rays.release(); r->d_tmp_hits.release(); r->d_tmp_aux.release();
     for (int i = 0; i < kNumStageEvents; ++i) if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
     if (r->stream) (void)hipStreamDestroy(r->stream);
@@ -191,6 +193,7 @@ extern "C" WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_re
     if (s->use_wavefront_pathtracing == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only the wavefront path tracer is built");
     const bool reset = s->max_bounces != r->settings.max_bounces || s->spp != r->settings.spp ||
                        s->next_event_estimation != r->settings.next_event_estimation;
+    if (std::memcmp(s, &r->settings, sizeof(*s)) != 0) r->forward_valid = false;
     r->settings = *s;
     if (reset)
         for (auto& sl : r->slots) sl.accum_count = 0;
@@ -418,8 +421,9 @@ wf_status wf_renderer::upload_scene()
             std::memcpy(sh.local_to_world, rec.transform, 64);
             for (int k = 0; k < 3; ++k) {
                 sh.base_color[k] = rec.material.base_color_factor[k];
-                sh.emissive[k] = rec.material.emissive_factor[k] * rec.material.emissive_strength;
+                sh.emissive_factor[k] = rec.material.emissive_factor[k];
             }
+            sh.emissive_strength = rec.material.emissive_strength;
             const Texture* bt = find_texture(rec.material.base_color_texture);
             const Texture* et = find_texture(rec.material.emissive_texture);
             sh.base_color_tex = bt ? (int32_t)handle_index(rec.material.base_color_texture) : -1;

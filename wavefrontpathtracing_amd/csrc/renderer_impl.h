@@ -96,6 +96,11 @@ struct wf_renderer {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uint64_t> graph_key;
 
+    // the last forward frame (what the adjoint replays)
+    wf::FrameArgs last_args{};
+    bool forward_valid = false;
+    wf::DevBuf d_lrem, d_delta, d_shadow_unit, d_shadow_light, d_adj_dl, d_adj_tex, d_adj_em;
+
     // traversal scratch
     wf::ExtendConfig ext_cfg;
     uint32_t ext_grid = 0;

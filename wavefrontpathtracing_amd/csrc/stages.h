@@ -67,6 +67,16 @@ struct FrameArgs {
     float4* image;               // resolved energy per batch pixel
     float4* accum;               // progressive accumulator per batch pixel
     uint32_t* final_rgba8;
+
+    // path-replay adjoint
+    float4* lrem;                // per path: estimate still to be collected
+    float4* delta;               // per path: dLoss/dpixel / spp
+    const float4* dl_dimage;     // per batch pixel
+    float4* shadow_unit;         // shadow contribution per unit emission strength
+    uint32_t* shadow_light;      // shadow ray's emitter instance
+    float4* grad_tex;            // per texel of the differentiated texture
+    float* grad_em;              // per instance: dLoss/d emissive_strength
+    int32_t grad_tex_slot;       // texture slot differentiated (-1: none)
 };
 
 hipError_t launch_clear(const FrameArgs& A, uint32_t grid, hipStream_t s);
@@ -74,5 +84,8 @@ hipError_t launch_generate(const FrameArgs& A, uint32_t grid, hipStream_t s);
 hipError_t launch_shade(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s);
 hipError_t launch_connect_accumulate(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s);
 hipError_t launch_resolve_post(const FrameArgs& A, uint32_t grid, hipStream_t s);
+hipError_t launch_adjoint_clear(const FrameArgs& A, uint32_t grid, hipStream_t s);
+hipError_t launch_adjoint_shade(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s);
+hipError_t launch_adjoint_connect(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s);
 
 } // namespace wf

@@ -39,10 +39,10 @@ static_assert(sizeof(GpuShadeTri) == 96, "GpuShadeTri is 96 B");
 // Per-instance shading record (instance_data_t, shared.hlsl.h:87-95).
 struct alignas(16) GpuShadeInstance {
     float local_to_world[16];
-    float base_color[3]; int32_t base_color_tex;   // texture slot, -1 = none
-    float emissive[3];   int32_t emissive_tex;     // emissive_factor * strength
+    float base_color[3]; int32_t base_color_tex;        // texture slot, -1 = none
+    float emissive_factor[3]; int32_t emissive_tex;     // emission = emissive_factor * emissive_strength
+    float emissive_strength; uint32_t pad;
     const GpuShadeTri* tris;
-    uint64_t pad;
 };
 static_assert(sizeof(GpuShadeInstance) == 112, "GpuShadeInstance is 112 B");
 

@@ -74,6 +74,7 @@ class Renderer:
         self._h = h
         self.render_width = render_width
         self.render_height = render_height
+        self._batch_dims = None
 
     # ---- lifecycle ----
     @property
@@ -117,6 +118,7 @@ class Renderer:
 
     def begin_scene(self, camera: L.Camera, env_render_texture_handle: int = L.INVALID_HANDLE):
         self._check(lib().wf_renderer_begin_scene(self.handle, C.byref(camera), env_render_texture_handle), "begin_scene")
+        self._batch_dims = None
 
     def render(self):
         self._check(lib().wf_renderer_render(self.handle), "render")
@@ -127,11 +129,20 @@ class Renderer:
     # ---- batches (many scenes per frame) ----
     def begin_batch(self):
         self._check(lib().wf_renderer_begin_batch(self.handle), "begin_batch")
+        self._batch_dims = []
 
     def batch_add_scene(self, camera: L.Camera, width: int, height: int) -> int:
         out = C.c_uint32()
         self._check(lib().wf_renderer_batch_add_scene(self.handle, C.byref(camera), width, height, C.byref(out)), "batch_add_scene")
+        self._batch_dims.append((width, height))
         return out.value
+
+    def image_pixels(self) -> int:
+        """Pixels of one frame's image: every batch scene's, concatenated, or
+        the render resolution outside a batch."""
+        if self._batch_dims is not None:
+            return sum(w * h for w, h in self._batch_dims)
+        return self.render_width * self.render_height
 
     def scene_count(self) -> int:
         out = C.c_uint32()
@@ -145,6 +156,38 @@ class Renderer:
         out = np.empty((height, width, 4), dtype=np.float32)
         self._check(lib().wf_renderer_read_scene_energy(self.handle, scene, ptr(out)), "read_scene_energy")
         return out
+
+    # ---- differentiable rendering ----
+    def render_adjoint(self, dl_dimage: np.ndarray, texture: int, texture_shape, n_instances: int):
+        """Path-replay adjoint of the last render(): returns (dLoss/dtexels
+        (h, w, 4), dLoss/d emission strength per instance)."""
+        dl = np.ascontiguousarray(dl_dimage, dtype=np.float32)
+        if texture != L.INVALID_HANDLE:
+            th, tw = texture_shape
+            g_tex = np.zeros((th, tw, 4), dtype=np.float32)
+            gp = ptr(g_tex)
+        else:
+            g_tex, gp = None, None
+        g_em = np.zeros(n_instances, dtype=np.float32)
+        self._check(lib().wf_renderer_render_adjoint(self.handle, ptr(dl), texture, gp, ptr(g_em)), "render_adjoint")
+        return g_tex, g_em
+
+    def render_adjoint_device(self, d_dl: int, texture: int, d_grad_tex: int, d_grad_em: int) -> None:
+        self._check(lib().wf_renderer_render_adjoint_device(self.handle, C.c_void_p(d_dl), texture, C.c_void_p(d_grad_tex),
+                                                            C.c_void_p(d_grad_em)), "render_adjoint_device")
+
+    def update_texture(self, texture: int, rgba: np.ndarray) -> None:
+        a = np.ascontiguousarray(rgba, dtype=np.float32)
+        self._check(lib().wf_renderer_update_texture(self.handle, texture, ptr(a)), "update_texture")
+
+    def update_texture_device(self, texture: int, d_rgba: int) -> None:
+        self._check(lib().wf_renderer_update_texture_device(self.handle, texture, C.c_void_p(d_rgba)), "update_texture_device")
+
+    def set_emission_strength(self, instance: int, strength: float) -> None:
+        self._check(lib().wf_renderer_set_emission_strength(self.handle, instance, float(strength)), "set_emission_strength")
+
+    def read_energy_device(self, d_out: int) -> None:
+        self._check(lib().wf_renderer_read_energy_device(self.handle, C.c_void_p(d_out)), "read_energy_device")
 
     # ---- resources ----
     def create_render_texture(self, width: int, height: int, data: np.ndarray, fmt: int = TEXTURE_FORMAT_RGB32F) -> int:

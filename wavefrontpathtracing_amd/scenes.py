@@ -177,6 +177,23 @@ def small_tlas_scene(seed: int = 12, width: int = 160, height: int = 120) -> Sce
                       soup_tris=3000, emitters=4)
 
 
+def differentiable_scene(seed: int = 5, width: int = 512, height: int = 512, ico_subdiv: int = 6, tex_size: int = 512,
+                         tex_seed: int = None) -> Scene:
+    """Box room (12 quads, half-size 2) with one emissive ceiling quad and two
+    icospheres; every diffuse surface takes its albedo from one RGB texture of
+    periodic value noise in [0.1, 0.9] (texture 0).  The emission scalar is
+    the ceiling quad's emissive strength (instance 3)."""
+    tex = value_noise_texture(seed if tex_seed is None else tex_seed, tex_size, tex_size, cells=16, lo=0.1, hi=0.9)
+    meshes = [box_room(), icosphere(ico_subdiv), quad()]
+    inst = [Instance(0, translate_scale((0.0, 0.0, 0.0), (2.0, 2.0, 2.0)), base_color=(1.0, 1.0, 1.0), texture=0),
+            Instance(1, translate_scale((-0.8, -1.35, 0.3), (0.6, 0.6, 0.6)), base_color=(1.0, 1.0, 1.0), texture=0),
+            Instance(1, translate_scale((0.9, -1.45, -0.4), (0.55, 0.55, 0.55)), base_color=(1.0, 1.0, 1.0), texture=0),
+            Instance(2, translate_scale((0.0, 1.95, 0.0), (1.2, 1.0, 1.2)), base_color=(0.0, 0.0, 0.0),
+                     emissive=(1.0, 1.0, 1.0), emissive_strength=8.0)]
+    return Scene(meshes=meshes, instances=inst, eye=(0.0, 0.2, -1.9), target=(0.0, -0.6, 0.5), vfov_deg=70.0,
+                 width=width, height=height, textures=[(tex_size, tex_size, tex)], name="diff")
+
+
 def batch_scenes(seed: int = 4, n_scenes: int = 64, tri_min: int = 10_000, tri_max: int = 2_000_000,
                  resolutions=(256, 512, 1024)):
     """Uneven scenes for the batched renderer: each a random-triangle soup
