@@ -62,18 +62,21 @@ def test_textured_frame_matches_oracle():
 
 
 def test_furnace_on_gpu():
+    """Constant environment E around a Lambertian icosphere: a camera path that
+    scatters once and escapes returns albedo * E.  The facets are flat and the
+    shading normals smooth, so a cosine-sampled direction near a silhouette
+    edge can graze a neighbouring facet and scatter twice (seed 4 has one such
+    pixel); the oracle, which has the same geometry, fixes those pixels."""
     sc = furnace_scene()
     gpu, _, _, _ = render_gpu(sc, max_bounces=4, spp=2, nee=1, env=1.5, seed=4)
     orc = O.OracleScene(sc)
     view = O.camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
     hit = (orc.trace_closest(O.primary_rays(view, sc.width, sc.height))["instance_idx"] != L.MISS).reshape(sc.height, sc.width)
-    expect = np.float32(1.5) * np.array([0.6, 0.5, 0.4], np.float32)
-    got = gpu[hit][:, :3]
-    rel = np.abs(got.astype(np.float64) - expect) / expect
     ref = orc.render(max_bounces=4, spp=2, nee=1, env=1.5, frame_seed=4)
-    bad = np.argwhere(~(rel <= 2e-6).all(axis=1))
-    assert len(bad) == 0, (f"{len(bad)} of {len(got)} hit pixels off: first {got[bad[:4, 0]]} "
-                           f"max rel {np.nanmax(rel):.3e}; oracle there {ref[hit][bad[:4, 0]][:, :3]}")
+    assert np.array_equal(gpu[..., :3], ref[..., :3]), np.argwhere(np.any(gpu[..., :3] != ref[..., :3], axis=-1))[:4]
+    expect = np.float32(1.5) * np.array([0.6, 0.5, 0.4], np.float32)
+    rel = np.abs(gpu[hit][:, :3].astype(np.float64) - expect) / expect
+    assert np.mean((rel <= 2e-6).all(axis=1)) >= 0.99
     assert np.all(gpu[~hit][:, :3] == np.float32(1.5))
 
 

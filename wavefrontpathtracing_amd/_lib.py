@@ -173,6 +173,14 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built: run `python __graft_entry__.py` or "
                                "`python -m wavefrontpathtracing_amd.build` first")
+        # PyTorch-ROCm ships its own HIP runtime.  Loading torch first makes
+        # libwf.so bind to that same runtime (one runtime, one device context
+        # per process), so torch tensors, streams and events interoperate with
+        # the library; a second runtime loaded behind ours would find no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(l, name)
