@@ -35,6 +35,8 @@ def bytes_per_ray(counters: np.ndarray) -> float:
 
 
 def cpu_baseline(scene, rays, seconds_target=12.0):
+    """The oracle's closest-hit traversal on the host cores: whole passes
+    over the same incoherent rays until about `seconds_target` of work."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as O
     orc = O.OracleScene(scene)
@@ -48,15 +50,126 @@ def cpu_baseline(scene, rays, seconds_target=12.0):
         if dt > 1.0 or n >= len(rays):
             break
         n = min(len(rays), n * 4)
-    # scale to ~seconds_target of work
-    n_final = int(min(len(rays), max(n, n * seconds_target / max(dt, 1e-6))))
+    passes = max(1, int(round(seconds_target / max(dt, 1e-6))))
+    n_final = n
+    if n < len(rays):
+        n_final = int(min(len(rays), n * seconds_target / max(dt, 1e-6)))
+        passes = 1
     sample = rays[:n_final]
     t0 = time.perf_counter()
-    orc.trace_closest(sample, nthreads=threads)
+    for _ in range(passes):
+        orc.trace_closest(sample, nthreads=threads)
     dt = time.perf_counter() - t0
-    return {"value": round(n_final / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"first {n_final} of the {len(rays)} incoherent rays, oracle traversal (C, -O3, no FP contraction), "
-                      f"{threads} threads, {dt:.1f} s"}
+    return {"value": round(passes * n_final / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{passes} pass(es) over the first {n_final} of the {len(rays)} incoherent rays, oracle traversal "
+                      f"(C, -O3, no FP contraction), {threads} threads, {dt:.1f} s"}
+
+
+def bench_batched(frames: int, tune: bool):
+    """64 uneven scenes (soup of 10K..2M triangles in a lit box room, 256^2 /
+    512^2 / 1024^2 pixels), 4 spp, depth 4, next-event estimation: the whole
+    batch as one frame of one launch per stage (captured hipGraph) against
+    the plain loop of one frame per scene.  Both produce the same images
+    (the batch path numbering equals the per-scene one)."""
+    from wavefrontpathtracing_amd import Renderer, scenes, tuning
+    t0 = time.perf_counter()
+    sl = scenes.batch_scenes()
+    r = Renderer(256, 256)
+    scenes.load_batch(r, sl)
+    setup_s = time.perf_counter() - t0
+    r.update_settings(max_bounces=4, spp=4, next_event_estimation=1, hdr_env_strength=0.0, frame_seed=9,
+                      rr_start_bounce=3, rr_survival=0.75, accumulate=0, use_hip_graph=1)
+    n = len(sl)
+
+    def batched_frame():
+        t = time.perf_counter()
+        r.render_scenes(0, n)
+        dt = time.perf_counter() - t
+        st = r.frame_stats()
+        return dt, st.rays_extended + st.shadow_rays
+
+    tuned = None
+    if tune:
+        cfgs = [dict(block=b, lds_stack=l, skip_popped=1) for b in (64, 128, 256) for l in (0, 16)]
+        tuned, table = tuning.autotune(r, lambda: batched_frame()[0], configs=cfgs, repeats=1)
+        tuned = dict(tuned, sweep={f"{c['block']}/{c['lds_stack']}": round(t * 1e3, 1) for c, t in table})
+    batched_frame()
+    bt, brays = 0.0, 0
+    for _ in range(frames):
+        dt, nr = batched_frame()
+        bt += dt
+        brays += nr
+    r.update_settings(use_hip_graph=0)
+    for k in range(n):
+        r.render_scenes(k, 1)
+    pt, prays = 0.0, 0
+    for _ in range(frames):
+        for k in range(n):
+            t = time.perf_counter()
+            r.render_scenes(k, 1)
+            pt += time.perf_counter() - t
+            st = r.frame_stats()
+            prays += st.rays_extended + st.shadow_rays
+    r.exit()
+    b_mr, p_mr = brays / bt / 1e6, prays / pt / 1e6
+    return {"workload": f"{n} scenes, {sum(len(i) // 3 for s in sl for _, i in s.meshes[:1]):,} soup tris, "
+                        f"{sum(s.width * s.height for s in sl):,} pixels, 4 spp, depth 4, NEE",
+            "batched_mrays": round(b_mr, 2), "plain_mrays": round(p_mr, 2), "ratio": round(b_mr / p_mr, 3),
+            "rays_per_frame": brays // frames, "batched_ms_per_frame": round(bt / frames * 1e3, 2),
+            "plain_ms_per_frame": round(pt / frames * 1e3, 2), "setup_s": round(setup_s, 1), "frames": frames,
+            "autotuned": tuned}
+
+
+def bench_adjoint(frames: int):
+    """Differentiable room (512^2, 16 spp, depth 4, NEE, 512^2 value-noise
+    albedo texture, two subdivision-6 icospheres): forward frame and the
+    path-replay adjoint of an L2 loss against a target rendered with another
+    albedo texture.  adjoint spp/s = spp / adjoint seconds per frame."""
+    import torch
+    from wavefrontpathtracing_amd import Renderer, scenes
+    sc = scenes.differentiable_scene()
+    tgt = scenes.differentiable_scene(tex_seed=77).textures[0][2]
+    r = Renderer(sc.width, sc.height)
+    _, tex_h = scenes.load(r, sc)
+    spp = 16
+    r.update_settings(max_bounces=4, spp=spp, next_event_estimation=1, hdr_env_strength=0.0, frame_seed=3,
+                      accumulate=0, use_hip_graph=1)
+    dev = torch.device("cuda:0")
+    n_pix = sc.width * sc.height
+    tw, th = sc.textures[0][0], sc.textures[0][1]
+    t_src = np.ones((th * tw, 4), np.float32)
+    t_src[:, :3] = tgt.reshape(-1, 3)
+    r.update_texture(tex_h[0], t_src)
+    r.render()
+    target = torch.empty((n_pix, 4), dtype=torch.float32, device=dev)
+    r.read_energy_device(target.data_ptr())
+    t0 = np.ones((th * tw, 4), np.float32)
+    t0[:, :3] = sc.textures[0][2].reshape(-1, 3)
+    r.update_texture(tex_h[0], t0)
+    img = torch.empty_like(target)
+    g_tex = torch.empty((th * tw, 4), dtype=torch.float32, device=dev)
+    g_em = torch.empty(len(sc.instances), dtype=torch.float32, device=dev)
+    fwd, adj = [], []
+    for k in range(frames + 1):
+        t = time.perf_counter()
+        r.render()
+        tf = time.perf_counter() - t
+        r.read_energy_device(img.data_ptr())
+        dl = (2.0 * (img - target)).contiguous()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r.render_adjoint_device(dl.data_ptr(), tex_h[0], g_tex.data_ptr(), g_em.data_ptr())
+        ta = time.perf_counter() - t
+        if k:
+            fwd.append(tf)
+            adj.append(ta)
+    r.exit()
+    tf, ta = float(np.mean(fwd)), float(np.mean(adj))
+    return {"workload": f"differentiable room {sc.width}x{sc.height}, {spp} spp, depth 4, NEE, "
+                        f"{tw}x{th} albedo texture + emission",
+            "adjoint_spp_per_s": round(spp / ta, 2), "forward_spp_per_s": round(spp / tf, 2),
+            "adjoint_ms": round(ta * 1e3, 2), "forward_ms": round(tf * 1e3, 2), "cost_ratio": round(ta / tf, 3),
+            "grad_norm": float(torch.linalg.vector_norm(g_tex[:, :3]).item()), "frames": frames}
 
 
 def main():
@@ -66,7 +179,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rays", type=int, default=1 << 21)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--tune", action="store_true", help="autotune the traversal kernel on the batched frame")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": src.numel() * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the path tracer is a single-GPU program (asked for --gpus {args.gpus})",
               file=sys.stderr)
@@ -154,6 +281,10 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, rays)
     r.exit()
+    if not args.no_batch:
+        out["batched"] = bench_batched(frames=2, tune=args.tune)
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(frames=3)
     print(json.dumps(out), flush=True)
 
 

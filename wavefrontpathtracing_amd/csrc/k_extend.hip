@@ -93,12 +93,11 @@ __device__ __forceinline__ bool tri_test(const float4& a, const float4& b, const
 }
 
 // ---- stacks -------------------------------------------------------------------
-// Entry encodings
-//   TLAS interior: left_right (high half = node index >= 1, <= 0xFFFE)
-//   TLAS leaf:     0xFFFF0000 | instance_idx
-//   BLAS interior: left_first of the node (bit 31 clear)
-//   BLAS leaf:     0x80000000 | prim_count << 27 | left_first   (prim_count 1..15, left_first < 2^27)
-//                  0x80000000 | node_index                      (other leaves; the node is re-read)
+// An entry is a child reference plus its slab entry distance.  Encodings:
+//   interior child:  ref                      (bit 31 clear; the child's pair record index)
+//   leaf child:      0x80000000 | kind << 27 | ref   (kind 1..15, ref < 2^27; TLAS leaves are kind 1)
+//                    0x80000000 | node index          (BLAS leaves with more triangles or a far
+//                                                      first triangle; the node is re-read)
 template <int BLOCK, int LDS>
 struct LdsStack {
     uint32_t* ent;
@@ -137,154 +136,211 @@ struct PrivStack {
     __device__ __forceinline__ void get(uint32_t sp, uint32_t& e, float& d) const { e = ent[sp]; d = dist[sp]; }
 };
 
-struct HitState { uint32_t inst, prim; float t, u, v; };
-
-enum TraceResult { kTraceDone = 0, kTraceOverflow = 1 };
-
 __device__ __forceinline__ float4 ld4(const void* p) { return *reinterpret_cast<const float4*>(p); }
 
-template <bool SKIP, bool COUNT, bool ANY, class Stack>
-__device__ __forceinline__ int trace_ray(const TraceScene& sc, const wf_ray& ray, uint32_t root, Stack& st, HitState& h, uint32_t* cnt)
+// Loads through the global address space (the pointers below all point to
+// device memory; generic pointers would compile to flat loads).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef float __attribute__((ext_vector_type(4))) native_float4;
+typedef __attribute__((address_space(1))) const native_float4 gfloat4;
+typedef __attribute__((address_space(1))) const uint32_t guint;
+__device__ __forceinline__ float4 gld4(const void* p)
 {
-    h.inst = ~0u; h.prim = ~0u; h.t = kRayMaxT; h.u = 0.0f; h.v = 0.0f;
-    float t = ray.tmax;
-    Ray3 wr;
-    wr.ox = ray.origin[0]; wr.oy = ray.origin[1]; wr.oz = ray.origin[2];
-    wr.dx = ray.direction[0]; wr.dy = ray.direction[1]; wr.dz = ray.direction[2];
-    wr.ix = 1.0f / wr.dx; wr.iy = 1.0f / wr.dy; wr.iz = 1.0f / wr.dz;
+    const native_float4 v = *(gfloat4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t gldu(const void* p) { return *(guint*)p; }
+#else
+__device__ __forceinline__ float4 gld4(const void* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ uint32_t gldu(const void* p) { return *reinterpret_cast<const uint32_t*>(p); }
+#endif
 
-    const wf_tlas_node* tn = sc.tnodes;
-    const float4 ra = ld4(&tn[root]);
-    const float4 rb = ld4(reinterpret_cast<const char*>(&tn[root]) + 16);
-    if (COUNT) cnt[0]++;
-    if (slab(ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, wr, t) == kRayMaxT) return kTraceDone;
-    uint32_t cur_lr = __float_as_uint(ra.w);
-    uint32_t cur_ii = __float_as_uint(rb.w);
+enum LaneStatus { kRun = 0, kDone = 1, kOverflow = 2 };
 
+// Persistent traversal with per-lane refill.  Every lane owns one ray at a
+// time; one pass of the loop performs, for each lane with a ray, an interior
+// visit (TLAS or BLAS: both children tested, near child next, far child
+// pushed), then an instance entry or a triangle leaf if that is what the lane
+// now points at, then pops until its next node is known.  `ref`/`kind` name
+// the lane's node: kind 0 = interior node whose child pair sits at
+// nodes + ref * 32 bytes (`nodes` = the TLAS pair array, or the BLAS node
+// array, whose children are adjacent), kind != 0 = leaf (a TLAS leaf's ref
+// is the instance, a BLAS leaf's ref/kind its first triangle/count).
+// A lane whose ray finishes writes its result and goes idle; once at least
+// `refill_min` lanes of the wave are idle, the wave claims that many rays
+// from the global cursor with one atomic.  The loop ends when every lane is
+// idle and the queue is drained.
+template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, class Stack>
+__device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, uint32_t lane, uint32_t n)
+{
+    const TraceScene sc = L.scene;
+    const uint32_t refill_min = L.refill_min;
+    Ray3 wr, cr;
+    float t = 0.0f;
+    uint32_t ref = 0, kind = 0, sp = 0, base_sp = 0;
     bool in_blas = false;
-    uint32_t sp = 0, base_sp = 0;
-    Ray3 lr;                              // object-space ray of the current instance
-    const wf_bvh_node* bn = nullptr;
-    const GpuTri* bt = nullptr;
-    uint32_t lf = 0, pc = 0;              // current BLAS node (left_first, prim_count)
-    uint32_t inst_id = 0;
+    const char* nodes = nullptr;
+    const GpuTri* tris = nullptr;
+    uint32_t inst_id = 0, h_inst = ~0u, h_prim = ~0u;
+    float h_u = 0.0f, h_v = 0.0f;
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    uint32_t ray_i = ~0u;
+    bool drained = false;
 
     for (;;) {
-        bool pop = false;
-        if (!in_blas) {
-            if (cur_lr == 0) {
-                // TLAS leaf: enter the instance (trace_ray_bvh_instance)
-                if (COUNT) cnt[2]++;
-                const GpuInstance* in = sc.inst + cur_ii;
-                const float4 m0 = ld4(&in->m[0]), m1 = ld4(&in->m[4]), m2 = ld4(&in->m[8]);
-                bn = in->nodes;
-                bt = in->tris;
-                // m[c*3+r]: m0 = (c0r0 c0r1 c0r2 c1r0), m1 = (c1r1 c1r2 c2r0 c2r1), m2 = (c2r2 c3r0 c3r1 c3r2)
-                lr.ox = (m0.x * wr.ox + m0.w * wr.oy) + (m1.z * wr.oz + m2.y * 1.0f);
-                lr.oy = (m0.y * wr.ox + m1.x * wr.oy) + (m1.w * wr.oz + m2.z * 1.0f);
-                lr.oz = (m0.z * wr.ox + m1.y * wr.oy) + (m2.x * wr.oz + m2.w * 1.0f);
-                lr.dx = (m0.x * wr.dx + m0.w * wr.dy) + (m1.z * wr.dz + m2.y * 0.0f);
-                lr.dy = (m0.y * wr.dx + m1.x * wr.dy) + (m1.w * wr.dz + m2.z * 0.0f);
-                lr.dz = (m0.z * wr.dx + m1.y * wr.dy) + (m2.x * wr.dz + m2.w * 0.0f);
-                lr.ix = 1.0f / lr.dx; lr.iy = 1.0f / lr.dy; lr.iz = 1.0f / lr.dz;
-                inst_id = cur_ii;
-                lf = bn[0].left_first;
-                pc = bn[0].prim_count;
-                if (COUNT) cnt[1]++;
-                base_sp = sp;
-                in_blas = true;
-                continue;
-            }
-            const uint32_t li = cur_lr >> 16, ri = cur_lr & 0xFFFFu;
-            const float4 la = ld4(&tn[li]), lb = ld4(reinterpret_cast<const char*>(&tn[li]) + 16);
-            const float4 ra = ld4(&tn[ri]), rb = ld4(reinterpret_cast<const char*>(&tn[ri]) + 16);
-            if (COUNT) cnt[0] += 2;
-            float dl = slab(la.x, la.y, la.z, lb.x, lb.y, lb.z, wr, t);
-            float dr = slab(ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, wr, t);
-            uint32_t l_lr = __float_as_uint(la.w), l_ii = __float_as_uint(lb.w);
-            uint32_t r_lr = __float_as_uint(ra.w), r_ii = __float_as_uint(rb.w);
-            if (dl > dr) {
-                float tf = dl; dl = dr; dr = tf;
-                uint32_t tu = l_lr; l_lr = r_lr; r_lr = tu;
-                tu = l_ii; l_ii = r_ii; r_ii = tu;
-            }
-            if (dl == kRayMaxT) {
-                pop = true;
+        const uint64_t idle = __ballot(ray_i == ~0u);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (drained && n_idle == 64u) break;
+        int status = kRun;
+        if (!drained && n_idle > 0u && (n_idle >= refill_min || n_idle == 64u)) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&L.work[0], n_idle);
+            base = __shfl(base, (int)leader, 64);
+            if ((uint64_t)base + n_idle >= (uint64_t)n) drained = true;
+            if (ray_i != ~0u) continue;
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            const uint32_t i = base + rank;
+            if (i >= n) continue;
+            // start the ray: TLAS root box tested once (trace_ray_tlas)
+            ray_i = i;
+            const wf_ray ray = L.rays[i];
+            const uint32_t root = L.ray_root ? L.ray_root[i] : 0u;
+            if (COUNT) { cnt[0] = 1; cnt[1] = 0; cnt[2] = 0; cnt[3] = 0; }
+            h_inst = ~0u; h_prim = ~0u; h_u = 0.0f; h_v = 0.0f;
+            t = ray.tmax;
+            wr.ox = ray.origin[0]; wr.oy = ray.origin[1]; wr.oz = ray.origin[2];
+            wr.dx = ray.direction[0]; wr.dy = ray.direction[1]; wr.dz = ray.direction[2];
+            wr.ix = 1.0f / wr.dx; wr.iy = 1.0f / wr.dy; wr.iz = 1.0f / wr.dz;
+            cr = wr;
+            sp = 0; base_sp = 0; in_blas = false;
+            nodes = reinterpret_cast<const char*>(sc.tpairs);
+            const float4 ra = gld4(&sc.tnodes[root]);
+            const float4 rb = gld4(reinterpret_cast<const char*>(&sc.tnodes[root]) + 16);
+            if (slab(ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, wr, t) == kRayMaxT) {
+                status = kDone;
             } else {
-                cur_lr = l_lr; cur_ii = l_ii;
-                if (dr != kRayMaxT) {
-                    const uint32_t e = (r_lr == 0) ? (0xFFFF0000u | r_ii) : r_lr;
-                    if (!st.push(sp, e, dr)) return kTraceOverflow;
-                    ++sp;
-                }
+                const uint32_t lr = __float_as_uint(ra.w);
+                if (lr == 0) { ref = __float_as_uint(rb.w); kind = 1; }
+                else { ref = 2u * root; kind = 0; }
             }
-        } else {
-            if (pc > 0) {
-                for (uint32_t k = lf; k < lf + pc; ++k) {
-                    const GpuTri* tr = bt + k;
-                    const float4 a = ld4(&tr->v0[0]), b = ld4(&tr->e1[0]), c = ld4(&tr->e2[0]);
-                    if (COUNT) cnt[3]++;
-                    if (tri_test(a, b, c, lr, t, h.u, h.v)) {
-                        h.prim = __float_as_uint(a.w);
-                        h.inst = inst_id;
-                        h.t = t;
-                        if (ANY) return kTraceDone;
-                    }
-                }
-                pop = true;
-            } else {
-                const float4* cp = reinterpret_cast<const float4*>(bn + lf);
-                const float4 a = cp[0], b = cp[1], c = cp[2], d = cp[3];
-                if (COUNT) cnt[1] += 2;
-                float dl = slab(a.x, a.y, a.z, b.x, b.y, b.z, lr, t);
-                float dr = slab(c.x, c.y, c.z, d.x, d.y, d.z, lr, t);
-                uint32_t l_lf = __float_as_uint(a.w), l_pc = __float_as_uint(b.w), l_id = lf;
-                uint32_t r_lf = __float_as_uint(c.w), r_pc = __float_as_uint(d.w), r_id = lf + 1;
-                if (dl > dr) {
-                    float tf = dl; dl = dr; dr = tf;
-                    uint32_t tu = l_lf; l_lf = r_lf; r_lf = tu;
-                    tu = l_pc; l_pc = r_pc; r_pc = tu;
-                    tu = l_id; l_id = r_id; r_id = tu;
-                }
-                if (dl == kRayMaxT) {
+        } else if (ray_i != ~0u) {
+            bool pop = false;
+            if (kind == 0) {
+                // interior node: near child next, far child pushed (swap
+                // only when dist_left > dist_right)
+                const char* cp = nodes + (size_t)ref * 32u;
+                const float4 a = gld4(cp), b = gld4(cp + 16), c = gld4(cp + 32), d = gld4(cp + 48);
+                if (COUNT) cnt[in_blas ? 1 : 0] += 2;
+                const float dl = slab(a.x, a.y, a.z, b.x, b.y, b.z, cr, t);
+                const float dr = slab(c.x, c.y, c.z, d.x, d.y, d.z, cr, t);
+                const bool sw = dl > dr;
+                const float dn = sw ? dr : dl, df = sw ? dl : dr;
+                const uint32_t n_ref = __float_as_uint(sw ? c.w : a.w), n_kind = __float_as_uint(sw ? d.w : b.w);
+                const uint32_t f_ref = __float_as_uint(sw ? a.w : c.w), f_kind = __float_as_uint(sw ? b.w : d.w);
+                const uint32_t f_id = ref + (sw ? 0u : 1u);
+                if (dn == kRayMaxT) {
                     pop = true;
                 } else {
-                    lf = l_lf; pc = l_pc;
-                    if (dr != kRayMaxT) {
+                    ref = n_ref; kind = n_kind;
+                    if (df != kRayMaxT) {
                         uint32_t e;
-                        if (r_pc == 0) e = r_lf;
-                        else if (r_pc < 16u && r_lf < (1u << 27)) e = 0x80000000u | (r_pc << 27) | r_lf;
-                        else e = 0x80000000u | r_id;
-                        if (!st.push(sp, e, dr)) return kTraceOverflow;
-                        ++sp;
+                        if (f_kind == 0) e = f_ref;
+                        else if (f_kind < 16u && f_ref < (1u << 27)) e = 0x80000000u | (f_kind << 27) | f_ref;
+                        else e = 0x80000000u | f_id;
+                        if (st.push(sp, e, df)) ++sp;
+                        else status = kOverflow;
                     }
+                }
+            }
+            if (status == kRun && !pop && kind != 0 && !in_blas) {
+                // TLAS leaf: enter the instance (trace_ray_bvh_instance); the
+                // BLAS root is entered without a box test
+                if (COUNT) { cnt[2]++; cnt[1]++; }
+                const GpuInstance* in = sc.inst + ref;
+                const float4 m0 = gld4(&in->m[0]), m1 = gld4(&in->m[4]), m2 = gld4(&in->m[8]);
+                const float4 pp = gld4(&in->nodes);
+                const wf_bvh_node* bn;
+                const GpuTri* bt;
+                {
+                    uint64_t u0 = (uint64_t)__float_as_uint(pp.x) | ((uint64_t)__float_as_uint(pp.y) << 32);
+                    uint64_t u1 = (uint64_t)__float_as_uint(pp.z) | ((uint64_t)__float_as_uint(pp.w) << 32);
+                    bn = reinterpret_cast<const wf_bvh_node*>(u0);
+                    bt = reinterpret_cast<const GpuTri*>(u1);
+                }
+                nodes = reinterpret_cast<const char*>(bn);
+                tris = bt;
+                // m[c*3+r]: m0 = (c0r0 c0r1 c0r2 c1r0), m1 = (c1r1 c1r2 c2r0 c2r1), m2 = (c2r2 c3r0 c3r1 c3r2)
+                cr.ox = (m0.x * wr.ox + m0.w * wr.oy) + (m1.z * wr.oz + m2.y * 1.0f);
+                cr.oy = (m0.y * wr.ox + m1.x * wr.oy) + (m1.w * wr.oz + m2.z * 1.0f);
+                cr.oz = (m0.z * wr.ox + m1.y * wr.oy) + (m2.x * wr.oz + m2.w * 1.0f);
+                cr.dx = (m0.x * wr.dx + m0.w * wr.dy) + (m1.z * wr.dz + m2.y * 0.0f);
+                cr.dy = (m0.y * wr.dx + m1.x * wr.dy) + (m1.w * wr.dz + m2.z * 0.0f);
+                cr.dz = (m0.z * wr.dx + m1.y * wr.dy) + (m2.x * wr.dz + m2.w * 0.0f);
+                cr.ix = 1.0f / cr.dx; cr.iy = 1.0f / cr.dy; cr.iz = 1.0f / cr.dz;
+                inst_id = ref;
+                ref = gldu(reinterpret_cast<const char*>(bn) + 12);    // root left_first
+                kind = gldu(reinterpret_cast<const char*>(bn) + 28);   // root prim_count
+                base_sp = sp;
+                in_blas = true;
+            }
+            if (status == kRun && !pop && kind != 0 && in_blas) {
+                // BLAS leaf: triangles in index order; a hit replaces the
+                // closest only if t < ray.t
+                for (uint32_t k = ref; k < ref + kind; ++k) {
+                    const GpuTri* tr = tris + k;
+                    const float4 a = gld4(&tr->v0[0]), b = gld4(&tr->e1[0]), c = gld4(&tr->e2[0]);
+                    if (COUNT) cnt[3]++;
+                    if (tri_test(a, b, c, cr, t, h_u, h_v)) {
+                        h_prim = __float_as_uint(a.w);
+                        h_inst = inst_id;
+                        if (ANY) { status = kDone; break; }
+                    }
+                }
+                pop = true;
+            }
+            if (status == kRun && pop) {
+                for (;;) {
+                    if (in_blas && sp == base_sp) {
+                        in_blas = false;
+                        cr = wr;
+                        nodes = reinterpret_cast<const char*>(sc.tpairs);
+                    }
+                    if (sp == 0) { status = kDone; break; }
+                    uint32_t e; float dd;
+                    --sp;
+                    st.get(sp, e, dd);
+                    if (e & 0x80000000u) {
+                        const uint32_t k4 = (e >> 27) & 15u;
+                        if (k4) { kind = k4; ref = e & 0x07FFFFFFu; }
+                        else {
+                            const char* nd = nodes + (size_t)(e & 0x07FFFFFFu) * 32u;
+                            ref = gldu(nd + 12); kind = gldu(nd + 28);
+                        }
+                        break;
+                    }
+                    if (SKIP && dd >= t) { if (COUNT) cnt[in_blas ? 1 : 0] += 2; continue; }
+                    ref = e; kind = 0;
+                    break;
                 }
             }
         }
-        if (!pop) continue;
-        // unified pop: finish the BLAS when its part of the stack is empty
-        for (;;) {
-            if (in_blas && sp == base_sp) in_blas = false;
-            if (sp == 0) return kTraceDone;
-            uint32_t e; float dd;
-            --sp;
-            st.get(sp, e, dd);
-            if (in_blas) {
-                if (e & 0x80000000u) {
-                    const uint32_t cnt4 = (e >> 27) & 15u;
-                    if (cnt4) { pc = cnt4; lf = e & 0x07FFFFFFu; }
-                    else { const uint32_t ni = e & 0x07FFFFFFu; lf = bn[ni].left_first; pc = bn[ni].prim_count; }
-                    break;
-                }
-                if (SKIP && dd >= t) { if (COUNT) cnt[1] += 2; continue; }
-                lf = e; pc = 0;
-                break;
-            } else {
-                if ((e >> 16) == 0xFFFFu) { cur_lr = 0; cur_ii = e & 0xFFFFu; break; }
-                if (SKIP && dd >= t) { if (COUNT) cnt[0] += 2; continue; }
-                cur_lr = e;
-                break;
+        if (status != kRun) {
+            if (status == kOverflow) {
+                atomicAdd(&L.work[1], 1u);
+                h_inst = ~0u; h_prim = ~0u; h_u = 0.0f; h_v = 0.0f;
             }
+            if (ANY) {
+                L.visible[ray_i] = (h_inst == ~0u && status != kOverflow) ? 1u : 0u;
+            } else {
+                wf_hit o;
+                o.instance_idx = h_inst; o.primitive_idx = h_prim;
+                o.t = h_inst == ~0u ? kRayMaxT : t;
+                o.bary[0] = h_u; o.bary[1] = h_v;
+                L.hits[ray_i] = o;
+            }
+            if (COUNT) reinterpret_cast<uint4*>(L.counters)[ray_i] = make_uint4(cnt[0], cnt[1], cnt[2], cnt[3]);
+            ray_i = ~0u;
         }
     }
 }
@@ -296,46 +352,16 @@ __global__ __launch_bounds__(BLOCK) void k_extend(ExtendLaunch L)
     __shared__ uint32_t s_ent[kLdsN];
     __shared__ float s_dist[kLdsN];
     const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63u;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t n = L.d_count ? *L.d_count : L.n;
-
-    for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&L.work[0], 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= n) break;
-        const uint32_t i = base + lane;
-        if (i >= n) continue;
-        const wf_ray ray = L.rays[i];
-        const uint32_t root = L.ray_root ? L.ray_root[i] : 0u;
-        HitState h;
-        uint32_t cnt[4] = {0, 0, 0, 0};
-        int rc;
-        if constexpr (LDS > 0) {
-            LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + (size_t)gtid * L.spill_per_thread, L.spill_per_thread,
-                                    L.stack_cap};
-            rc = trace_ray<SKIP, COUNT, ANY>(L.scene, ray, root, st, h, cnt);
-        } else {
-            PrivStack<128> st;
-            st.cap = L.stack_cap;
-            rc = trace_ray<SKIP, COUNT, ANY>(L.scene, ray, root, st, h, cnt);
-        }
-        if (rc == kTraceOverflow) {
-            atomicAdd(&L.work[1], 1u);
-            h.inst = ~0u; h.prim = ~0u; h.t = kRayMaxT; h.u = 0.0f; h.v = 0.0f;
-        }
-        if (ANY) {
-            L.visible[i] = (h.inst == ~0u && rc == kTraceDone) ? 1u : 0u;
-        } else {
-            wf_hit o;
-            o.instance_idx = h.inst; o.primitive_idx = h.prim; o.t = h.t; o.bary[0] = h.u; o.bary[1] = h.v;
-            L.hits[i] = o;
-        }
-        if (COUNT) {
-            uint4 c4 = make_uint4(cnt[0], cnt[1], cnt[2], cnt[3]);
-            reinterpret_cast<uint4*>(L.counters)[i] = c4;
-        }
+    if constexpr (LDS > 0) {
+        LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + (size_t)gtid * L.spill_per_thread, L.spill_per_thread,
+                                L.stack_cap};
+        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY>(L, st, tid & 63u, n);
+    } else {
+        PrivStack<128> st;
+        st.cap = L.stack_cap;
+        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY>(L, st, tid & 63u, n);
     }
 }
 

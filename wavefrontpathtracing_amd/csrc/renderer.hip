@@ -147,7 +147,7 @@ extern "C" WF_API wf_status wf_renderer_exit(wf_renderer* r)
     for (auto& m : r->meshes) { m.nodes.release(); m.tris.release(); m.shade.release(); }
     for (auto& t : r->textures) t.texels.release();
     r->release_frame_buffers();
-    r->d_tnodes.release(); r->d_inst.release(); r->d_shade_inst.release(); r->d_textures.release();
+    r->d_tnodes.release(); r->d_tpairs.release(); r->d_inst.release(); r->d_shade_inst.release(); r->d_textures.release();
     r->d_emitters.release(); r->d_scenes.release(); r->d_dyn.release();
     r->d_lrem.release(); r->d_delta.release(); r->d_shadow_unit.release(); r->d_shadow_light.release();
     r->d_adj_dl.release(); r->d_adj_tex.release(); r->d_adj_em.release();
@@ -208,11 +208,13 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     if (c->lds_stack_entries != 0 && c->lds_stack_entries != 8 && c->lds_stack_entries != 16 && c->lds_stack_entries != 32)
         return r->fail(WF_ERROR_INVALID_ARGUMENT, "lds_stack_entries must be 0, 8, 16 or 32");
     if (c->bvh_layout != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "bvh_layout must be 0 (BVH2)");
+    if (c->refill_min > 64) return r->fail(WF_ERROR_INVALID_ARGUMENT, "refill_min must be in [0, 64]");
     const ExtendConfig old = r->ext_cfg;
     r->ext_cfg.block = c->workgroup_size;
     r->ext_cfg.lds_stack = c->lds_stack_entries;
     r->ext_cfg.skip_popped = c->skip_popped_interior ? 1u : 0u;
     r->ext_cfg.stack_limit = c->stack_limit;
+    r->ext_cfg.refill_min = c->refill_min ? c->refill_min : ExtendConfig{}.refill_min;
     r->drop_graph();
     const wf_status s = r->ensure_trace_scratch();
     if (s != WF_OK) { r->ext_cfg = old; return s; }
@@ -227,6 +229,7 @@ extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* 
     out->skip_popped_interior = r->ext_cfg.skip_popped;
     out->bvh_layout = 0;
     out->stack_limit = r->ext_cfg.stack_limit;
+    out->refill_min = r->ext_cfg.refill_min;
     return WF_OK;
 }
 
@@ -432,8 +435,28 @@ wf_status wf_renderer::upload_scene()
             si.push_back(sh);
         }
     }
+    // child-pair records of the interior TLAS nodes (see TraceScene)
+    std::vector<float> pairs(16 * all_nodes.size(), 0.0f);
+    for (size_t k = 0; k < all_nodes.size(); ++k) {
+        const uint32_t lr = all_nodes[k].left_right;
+        if (lr == 0) continue;
+        const uint32_t child[2] = {lr >> 16, lr & 0xFFFFu};
+        for (int c = 0; c < 2; ++c) {
+            const wf_tlas_node& cn = all_nodes[child[c]];
+            float* slot = &pairs[16 * k + 8 * c];
+            const bool leaf = cn.left_right == 0;
+            const uint32_t ref = leaf ? cn.instance_idx : 2u * child[c];
+            const uint32_t kind = leaf ? 1u : 0u;
+            std::memcpy(slot, cn.aabb_min, 12);
+            std::memcpy(slot + 3, &ref, 4);
+            std::memcpy(slot + 4, cn.aabb_max, 12);
+            std::memcpy(slot + 7, &kind, 4);
+        }
+    }
     drop_graph();
     hipError_t e = d_tnodes.ensure(sizeof(wf_tlas_node) * all_nodes.size());
+    if (e == hipSuccess) e = d_tpairs.ensure(sizeof(float) * pairs.size());
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tpairs.p, pairs.data(), sizeof(float) * pairs.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = d_inst.ensure(sizeof(GpuInstance) * gi.size());
     if (e == hipSuccess) e = d_shade_inst.ensure(sizeof(GpuShadeInstance) * si.size());
     if (e == hipSuccess) e = hipMemcpyAsync(d_tnodes.p, all_nodes.data(), sizeof(wf_tlas_node) * all_nodes.size(), hipMemcpyHostToDevice, stream);
@@ -474,6 +497,7 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     if (e != hipSuccess) return hip_fail(e, "work counter reset");
     ExtendLaunch L{};
     L.scene.tnodes = (const wf_tlas_node*)d_tnodes.p;
+    L.scene.tpairs = (const float4*)d_tpairs.p;
     L.scene.inst = (const GpuInstance*)d_inst.p;
     L.rays = d_rays;
     L.ray_root = d_roots;
@@ -488,6 +512,7 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     L.stack_cap = ext_cfg.stack_limit ? std::min(ext_cfg.stack_limit, kStackTotal) : kStackTotal;
     L.grid = ext_grid;
     L.any_hit = any ? 1 : 0;
+    L.refill_min = ext_cfg.refill_min;
     e = launch_extend(L, ext_cfg, stream);
     if (e != hipSuccess) return hip_fail(e, "extend launch");
     return WF_OK;

@@ -87,7 +87,7 @@ struct wf_renderer {
     wf_frame_stats stats{};
 
     // TLAS + instance records of every slot, concatenated on the device
-    wf::DevBuf d_tnodes, d_inst, d_shade_inst, d_textures, d_emitters, d_scenes, d_dyn;
+    wf::DevBuf d_tnodes, d_tpairs, d_inst, d_shade_inst, d_textures, d_emitters, d_scenes, d_dyn;
     bool tlas_valid = false;
     std::vector<uint8_t> h_scenes;   // staging of the per-frame scene table
     uint32_t h_dyn[4] = {0, 0, 0, 0};

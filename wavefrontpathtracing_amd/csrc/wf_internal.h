@@ -51,8 +51,15 @@ struct GpuTexture {
     uint32_t width, height;
 };
 
+// TLAS child-pair records: record k (64 B) holds the two children of
+// interior TLAS node k as {aabb_min, ref, aabb_max, kind}, with kind 0 and
+// ref = 2 * child for an interior child (its record sits at ref * 32 bytes)
+// and kind 1, ref = instance for a leaf; the same layout as two adjacent
+// BLAS nodes {aabb_min, left_first, aabb_max, prim_count}, so one visit
+// routine serves both levels.
 struct TraceScene {
-    const wf_tlas_node* tnodes;
+    const wf_tlas_node* tnodes;   // reference-layout nodes (root box test)
+    const float4* tpairs;
     const GpuInstance* inst;
 };
 
@@ -62,6 +69,7 @@ struct ExtendConfig {
     uint32_t lds_stack = 16;    // LDS stack entries per lane (0 = private/scratch stack)
     uint32_t skip_popped = 1;   // skip popped interior nodes whose entry distance >= t
     uint32_t stack_limit = 0;   // cap on stack entries per lane (0 = capacity)
+    uint32_t refill_min = 16;   // idle lanes of a wave that trigger a refill
 };
 
 // Launch the closest-hit kernel.  d_count (device) overrides n when non-null.
@@ -80,6 +88,7 @@ struct ExtendLaunch {
     uint32_t stack_cap;        // entries a lane may hold (LDS + spill, or private)
     uint32_t grid;             // persistent blocks
     int any_hit;
+    uint32_t refill_min;       // idle lanes of a wave that trigger a refill (1..64)
 };
 hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStream_t stream);
 hipError_t extend_occupancy(const ExtendConfig& cfg, int* blocks_per_cu);

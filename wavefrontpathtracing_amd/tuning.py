@@ -1,7 +1,8 @@
 """Traversal kernel configuration and the autotuner over it.
 
 Axes: workgroup size {64, 128, 256} x stack placement {LDS 8/16/32 entries
-per lane with global spill, private (scratch) stack} x popped-interior skip.
+per lane with global spill, private (scratch) stack} x popped-interior skip
+x refill threshold (finished lanes of a wave before it claims new rays).
 """
 from __future__ import annotations
 
@@ -10,12 +11,13 @@ import itertools
 
 from . import _lib as L
 
-DEFAULT = dict(block=128, lds_stack=16, skip_popped=1)
+DEFAULT = dict(block=128, lds_stack=16, skip_popped=1, refill_min=16)
 
 
 def set_extend_config(renderer, block: int = DEFAULT["block"], lds_stack: int = DEFAULT["lds_stack"],
-                      skip_popped: int = DEFAULT["skip_popped"], bvh_layout: int = 0, stack_limit: int = 0) -> None:
-    c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout, stack_limit)
+                      skip_popped: int = DEFAULT["skip_popped"], bvh_layout: int = 0, stack_limit: int = 0,
+                      refill_min: int = DEFAULT["refill_min"]) -> None:
+    c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout, stack_limit, refill_min)
     L.check(L.lib().wf_renderer_set_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "set_traversal_config")
 
 
@@ -23,7 +25,7 @@ def get_extend_config(renderer) -> dict:
     c = L.TraversalConfig()
     L.check(L.lib().wf_renderer_get_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "get_traversal_config")
     return dict(block=c.workgroup_size, lds_stack=c.lds_stack_entries, skip_popped=c.skip_popped_interior,
-                bvh_layout=c.bvh_layout, stack_limit=c.stack_limit)
+                bvh_layout=c.bvh_layout, stack_limit=c.stack_limit, refill_min=c.refill_min)
 
 
 def candidates():
