@@ -99,6 +99,7 @@ def bench_batched(frames: int, tune: bool):
         dt, nr = batched_frame()
         bt += dt
         brays += nr
+    # the same images, one scene at a time (no graph)
     r.update_settings(use_hip_graph=0)
     for k in range(n):
         r.render_scenes(k, 1)
@@ -110,11 +111,25 @@ def bench_batched(frames: int, tune: bool):
             pt += time.perf_counter() - t
             st = r.frame_stats()
             prays += st.rays_extended + st.shadow_rays
+    # the reference's frame structure: one scene, one sample per pixel per
+    # frame, frames accumulated until the scene has 4 samples
+    r.update_settings(spp=1, accumulate=1, frame_seed=0)
+    rt, rrays = 0.0, 0
+    for _ in range(frames):
+        for k in range(n):
+            for _s in range(4):
+                t = time.perf_counter()
+                r.render_scenes(k, 1)
+                rt += time.perf_counter() - t
+                st = r.frame_stats()
+                rrays += st.rays_extended + st.shadow_rays
     r.exit()
-    b_mr, p_mr = brays / bt / 1e6, prays / pt / 1e6
+    b_mr, p_mr, r_mr = brays / bt / 1e6, prays / pt / 1e6, rrays / rt / 1e6
     return {"workload": f"{n} scenes, {sum(len(i) // 3 for s in sl for _, i in s.meshes[:1]):,} soup tris, "
                         f"{sum(s.width * s.height for s in sl):,} pixels, 4 spp, depth 4, NEE",
             "batched_mrays": round(b_mr, 2), "plain_mrays": round(p_mr, 2), "ratio": round(b_mr / p_mr, 3),
+            "reference_loop_mrays": round(r_mr, 2), "ratio_vs_reference_loop": round(b_mr / r_mr, 3),
+            "reference_loop_ms_per_batch": round(rt / frames * 1e3, 2),
             "rays_per_frame": brays // frames, "batched_ms_per_frame": round(bt / frames * 1e3, 2),
             "plain_ms_per_frame": round(pt / frames * 1e3, 2), "setup_s": round(setup_s, 1), "frames": frames,
             "autotuned": tuned}

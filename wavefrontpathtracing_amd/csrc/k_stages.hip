@@ -155,6 +155,27 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
     return slot;
 }
 
+// atomicAdd(&arr[key], v) for every lane with `pred`, combined per distinct
+// key: the wave sums the values of the lanes that share the first pending
+// lane's key and issues one atomic for them (the emission gradients of a
+// frame all land on a handful of emitter instances).  Called by all lanes.
+__device__ __forceinline__ void wave_add_keyed(float* arr, uint32_t key, float v, bool pred)
+{
+    const uint32_t lane = __lane_id();
+    uint64_t pending = __ballot(pred);
+    while (pending) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const uint32_t k = __shfl(key, (int)leader, 64);
+        const bool mine = pred && key == k;
+        const uint64_t sel = __ballot(mine);
+        float sum = mine ? v : 0.0f;
+        for (int off = 32; off > 0; off >>= 1) sum = sum + __shfl_xor(sum, off, 64);
+        if (lane == leader) atomicAdd(arr + k, sum);
+        pending &= ~sel;
+        pred = pred && !mine;
+    }
+}
+
 __device__ __forceinline__ void store_ray(wf_ray* q, uint32_t slot, V3 o, V3 d, float tmin, float tmax)
 {
     float4* p = reinterpret_cast<float4*>(q + slot);
@@ -270,6 +291,9 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
         float sh_tmax = 0.0f;
         float4 contrib = make_float4(0, 0, 0, 0), unit = make_float4(0, 0, 0, 0);
         uint32_t p = 0, root = 0, light = 0;
+        uint32_t em_key = 0;
+        float em_g = 0.0f;
+        bool em_has = false;
         if (active) {
             p = A.path_of[cur][i];
             root = A.root_of[cur][i];
@@ -322,8 +346,9 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
                             e4.x = e4.x - thr.x * em.x;
                             e4.y = e4.y - thr.y * em.y;
                             e4.z = e4.z - thr.z * em.z;
-                            const float g = (dlt.x * (thr.x * ef.x) + dlt.y * (thr.y * ef.y)) + dlt.z * (thr.z * ef.z);
-                            if (g != 0.0f) atomicAdd(&A.grad_em[hit.instance_idx], g);
+                            em_g = (dlt.x * (thr.x * ef.x) + dlt.y * (thr.y * ef.y)) + dlt.z * (thr.z * ef.z);
+                            em_key = hit.instance_idx;
+                            em_has = em_g != 0.0f;
                         } else {
                             e4.x = e4.x + thr.x * em.x;
                             e4.y = e4.y + thr.y * em.y;
@@ -426,6 +451,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
             if (ADJ) A.lrem[p] = e4;
             else A.energy[p] = e4;
         }
+        if (ADJ) wave_add_keyed(A.grad_em, em_key, em_g, em_has);
         const uint32_t slot = wave_append(&A.counts[d + 1], spawn);
         if (spawn) {
             store_ray(A.rays[nxt], slot, new_o, new_d, kRayMinT, FLT_MAX);
@@ -528,17 +554,24 @@ __global__ void k_adjoint_connect(FrameArgs A, uint32_t d)
 {
     const uint32_t n = A.counts[kShadowCountBase + d];
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        if (!A.shadow_visible[i]) continue;
-        const uint32_t p = A.shadow_path[i];
-        const float4 c = A.shadow_contrib[i];
-        const float4 un = A.shadow_unit[i];
-        const float4 dl = A.delta[p];
-        float4 e = A.lrem[p];
-        e.x = e.x - c.x; e.y = e.y - c.y; e.z = e.z - c.z;
-        A.lrem[p] = e;
-        const float g = (dl.x * un.x + dl.y * un.y) + dl.z * un.z;
-        if (g != 0.0f) atomicAdd(&A.grad_em[A.shadow_light[i]], g);
+    const uint32_t n_round = (n + 63u) & ~63u;   // whole waves for the keyed reduction
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
+        float g = 0.0f;
+        uint32_t light = 0;
+        bool has = false;
+        if (i < n && A.shadow_visible[i]) {
+            const uint32_t p = A.shadow_path[i];
+            const float4 c = A.shadow_contrib[i];
+            const float4 un = A.shadow_unit[i];
+            const float4 dl = A.delta[p];
+            float4 e = A.lrem[p];
+            e.x = e.x - c.x; e.y = e.y - c.y; e.z = e.z - c.z;
+            A.lrem[p] = e;
+            g = (dl.x * un.x + dl.y * un.y) + dl.z * un.z;
+            light = A.shadow_light[i];
+            has = g != 0.0f;
+        }
+        wave_add_keyed(A.grad_em, light, g, has);
     }
 }
 
