@@ -65,6 +65,64 @@ def cpu_baseline(scene, rays, seconds_target=12.0):
                       f"(C, -O3, no FP contraction), {threads} threads, {dt:.1f} s"}
 
 
+def time_cpu(orc, rays, threads, seconds_target):
+    """Oracle closest-hit passes over `rays` until ~seconds_target of work."""
+    t0 = time.perf_counter()
+    orc.trace_closest(rays, nthreads=threads)
+    dt = time.perf_counter() - t0
+    passes = max(1, int(seconds_target / max(dt, 1e-6)))
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        orc.trace_closest(rays, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return round(passes * len(rays) / dt / 1e6, 4), passes, round(dt, 1)
+
+
+def bench_soup(n_tris: int, steps: int, cpu: bool):
+    """Random-triangle soup (centroids in [-1,1]^3, offsets +-U(0.005,0.02)),
+    16-bin SAH, camera (0,0,-3) -> origin, 60 deg, 512x512: closest hit of
+    the 262,144 camera rays and of 2^21 incoherent rays (uniform origin in the
+    soup bounds, uniform sphere directions)."""
+    import torch
+    from wavefrontpathtracing_amd import Renderer, scenes
+    sc = scenes.soup_scene(n_tris)
+    r = Renderer(sc.width, sc.height)
+    r.set_build_options(interval_count=16)
+    scenes.load(r, sc)
+    r.build_tlas()
+    bmin, bmax = r.scene_bounds()
+    sets = {"primary": scenes.primary_rays(sc), "incoherent": scenes.sphere_rays(3, bmin, bmax, 1 << 21)}
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.ExternalStream(r.stream)
+    out = {"workload": f"{n_tris:,}-triangle soup, 16-bin SAH, 512x512 camera rays + 2^21 incoherent rays"}
+    for name, rays in sets.items():
+        d_rays = torch.from_numpy(rays.view(np.uint8)).to(dev)
+        d_hits = torch.empty(len(rays) * 20, dtype=torch.uint8, device=dev)
+        torch.cuda.synchronize()
+        for _ in range(2):
+            r.trace_closest_device(d_rays.data_ptr(), len(rays), d_hits.data_ptr())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            r.trace_closest_device(d_rays.data_ptr(), len(rays), d_hits.data_ptr())
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out[f"{name}_mrays"] = round(len(rays) * steps / (e0.elapsed_time(e1) * 1e-3) / 1e6, 2)
+    r.exit()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_api as O
+        orc = O.OracleScene(sc, bins=16)
+        threads = min(os.cpu_count() or 1, 16)
+        prim = sets["primary"]
+        v1, p1, d1 = time_cpu(orc, prim[: len(prim) // 8], 1, 4.0)
+        vn, pn, dn = time_cpu(orc, prim, threads, 4.0)
+        out["cpu_primary"] = {"1_thread_mrays": v1, f"{threads}_threads_mrays": vn, "kind": "port",
+                              "sample": f"oracle traversal: 1 thread {p1} pass(es) over 1/8 of the camera rays "
+                                        f"({d1} s); {threads} threads {pn} pass(es) over all ({dn} s)"}
+    return out
+
+
 def bench_batched(frames: int, tune: bool):
     """64 uneven scenes (soup of 10K..2M triangles in a lit box room, 256^2 /
     512^2 / 1024^2 pixels), 4 spp, depth 4, next-event estimation: the whole
@@ -196,7 +254,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-soup", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune the traversal kernel on the batched frame")
+    ap.add_argument("--sweep", action="store_true",
+                    help="autotune the traversal kernel on the incoherent workload first (table on stderr)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -235,6 +296,24 @@ def main():
 
     def step():
         r.trace_closest_device(d_rays.data_ptr(), len(rays), d_hits.data_ptr())
+
+    if args.sweep:
+        # autotuner table on this workload (stderr), then keep the fastest
+        ev_a, ev_b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+        def timed():
+            ev_a.record(stream)
+            step()
+            ev_b.record(stream)
+            ev_b.synchronize()
+            return ev_a.elapsed_time(ev_b) * 1e-3
+
+        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, refill_min=f, tlas_in_global=g)
+                for b in (64, 128, 256) for l in (0, 8, 16) for f in (8, 16, 24) for g in (0, 1)]
+        best, table = tuning.autotune(r, timed, configs=cfgs, repeats=3)
+        for c, t in table:
+            print(f"sweep block={c['block']} lds={c['lds_stack']} refill={c['refill_min']} "
+                  f"tlas_global={c['tlas_in_global']}: {len(rays) / t / 1e6:.1f} Mrays/s", file=sys.stderr)
 
     for _ in range(args.warmup):
         step()
@@ -296,6 +375,9 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, rays)
     r.exit()
+    if not args.no_soup:
+        out["soup_65k"] = bench_soup(65536, steps=10, cpu=not args.no_cpu)
+        out["soup_1m"] = bench_soup(1 << 20, steps=10, cpu=False)
     if not args.no_batch:
         out["batched"] = bench_batched(frames=2, tune=args.tune)
     if not args.no_adjoint:

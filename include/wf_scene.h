@@ -55,6 +55,9 @@ WF_API void wfs_random_transform(wfs_pcg32* rng, float scale_min, float scale_ma
 /* n rays: origin uniform in [bmin,bmax], direction uniform on the sphere,
  * tmin 1e-8, tmax FLT_MAX. */
 WF_API wf_status wfs_sphere_rays(uint64_t seed, const float bmin[3], const float bmax[3], uint32_t n, wf_ray* out);
+/* the camera rays of generate (make_primary_ray, common.hlsl:145-169) for a
+ * w x h image of view v, row-major (ray y*w + x), tmin 1e-8, tmax FLT_MAX */
+WF_API wf_status wfs_primary_rays(const wf_view* v, uint32_t width, uint32_t height, wf_ray* out);
 /* w x h RGB float texture of periodic value noise (lattice period `cells`)
  * mapped into [lo, hi]. */
 WF_API wf_status wfs_value_noise_texture(uint64_t seed, uint32_t w, uint32_t h, uint32_t cells, float lo, float hi, float* out_rgb);

@@ -182,6 +182,16 @@ def test_camera_matches_library_view():
         assert bytes(ov) == bytes(lv)
 
 
+def test_library_primary_rays_match_oracle():
+    """The library's host restatement of the generate stage's rays equals the
+    oracle's bit for bit."""
+    for eye, tgt, w, h in [((0, 0, -3), (0, 0, 0), 512, 512), ((0, 5, -30), (0, 0, 0), 320, 180)]:
+        sc = scenes.Scene(meshes=[], instances=[], eye=eye, target=tgt, vfov_deg=60.0, width=w, height=h)
+        lib_rays = scenes.primary_rays(sc)
+        orc_rays = O.primary_rays(O.camera_view(eye, tgt, 60.0, w, h), w, h)
+        assert lib_rays.tobytes() == orc_rays.tobytes()
+
+
 # ---- traversal (accelstruct.hlsl:58-218) vs brute force -----------------------
 @pytest.mark.parametrize("which", ["soup", "tlas"])
 def test_traversal_equals_brute_force(which):

@@ -90,7 +90,8 @@ class FrameStats(C.Structure):
 
 class TraversalConfig(C.Structure):
     _fields_ = [("workgroup_size", C.c_uint32), ("lds_stack_entries", C.c_uint32), ("skip_popped_interior", C.c_uint32),
-                ("bvh_layout", C.c_uint32), ("stack_limit", C.c_uint32), ("refill_min", C.c_uint32)]
+                ("bvh_layout", C.c_uint32), ("stack_limit", C.c_uint32), ("refill_min", C.c_uint32),
+                ("tlas_in_global", C.c_uint32)]
 
 
 class Pcg32(C.Structure):
@@ -160,6 +161,7 @@ SIGNATURES = {
     "wfs_quad": (_st, [_vp, _vp]),
     "wfs_random_transform": (None, [C.POINTER(Pcg32), _f32, _f32, _f32, _vp]),
     "wfs_sphere_rays": (_st, [_u64, _vp, _vp, _u32, _vp]),
+    "wfs_primary_rays": (_st, [C.POINTER(View), _u32, _u32, _vp]),
     "wfs_value_noise_texture": (_st, [_u64, _u32, _u32, _u32, _f32, _f32, _vp]),
 }
 

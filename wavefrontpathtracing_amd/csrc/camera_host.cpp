@@ -2,7 +2,9 @@
 // (core/camera/camera.cpp:8-16) and the view_t that renderer::begin_scene
 // derives (renderer.cpp:599-630), with glm's float arithmetic.
 #include "wf_host.h"
+#include "../../include/wf_scene.h"
 
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 
@@ -62,5 +64,37 @@ extern "C" WF_API wf_status wf_make_view(const wf_camera* camera, uint32_t width
     if (!camera || !out || width == 0 || height == 0 || !(camera->vfov_deg > 0.0f && camera->vfov_deg < 180.0f))
         return WF_ERROR_INVALID_ARGUMENT;
     wf::make_view(*camera, width, height, *out);
+    return WF_OK;
+}
+
+// The camera rays the generate stage makes (make_primary_ray + make_ray,
+// common.hlsl:145-169; generate.hlsl:12-20), on the host: pixel (x, y) of a
+// w x h image is ray y * w + x.
+extern "C" WF_API wf_status wfs_primary_rays(const wf_view* v, uint32_t width, uint32_t height, wf_ray* out)
+{
+    if (!v || !out || width == 0 || height == 0) return WF_ERROR_INVALID_ARGUMENT;
+    const float zero4[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+    float ow[4];
+    wf::mat4_mul_vec4(v->view_to_world, zero4, ow);
+    for (uint32_t py = 0; py < height; ++py) {
+        for (uint32_t px = 0; px < width; ++px) {
+            const float uvx = ((float)px + 0.5f) / v->render_dim[0];
+            float uvy = ((float)py + 0.5f) / v->render_dim[1];
+            uvy = 1.0f - uvy;
+            const float c4[4] = {2.0f * uvx - 1.0f, 2.0f * uvy - 1.0f, 1.0f, 1.0f};
+            float cv[4], dv[3], d4[4], dw[4];
+            wf::mat4_mul_vec4(v->clip_to_view, c4, cv);
+            wf::normalize3(cv, dv);
+            d4[0] = dv[0]; d4[1] = dv[1]; d4[2] = dv[2]; d4[3] = 0.0f;
+            wf::mat4_mul_vec4(v->view_to_world, d4, dw);
+            wf_ray& r = out[(size_t)py * width + px];
+            for (int k = 0; k < 3; ++k) {
+                r.origin[k] = ow[k] + dw[k] * 1e-8f;
+                r.direction[k] = dw[k];
+            }
+            r.tmin = 1e-8f;
+            r.tmax = FLT_MAX;
+        }
+    }
     return WF_OK;
 }

@@ -170,8 +170,8 @@ enum LaneStatus { kRun = 0, kDone = 1, kOverflow = 2 };
 // `refill_min` lanes of the wave are idle, the wave claims that many rays
 // from the global cursor with one atomic.  The loop ends when every lane is
 // idle and the queue is drained.
-template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, class Stack>
-__device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, uint32_t lane, uint32_t n)
+template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, class Stack>
+__device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, const float4* s_tl, uint32_t lane, uint32_t n)
 {
     const TraceScene sc = L.scene;
     const uint32_t refill_min = L.refill_min;
@@ -221,16 +221,22 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, ui
                 status = kDone;
             } else {
                 const uint32_t lr = __float_as_uint(ra.w);
-                if (lr == 0) { ref = __float_as_uint(rb.w); kind = 1; }
-                else { ref = 2u * root; kind = 0; }
+                ref = lr == 0 ? __float_as_uint(rb.w) : 2u * __float_as_uint(rb.w);   // instance, or 2 * record
+                kind = lr == 0 ? 1u : 0u;
             }
         } else if (ray_i != ~0u) {
             bool pop = false;
             if (kind == 0) {
                 // interior node: near child next, far child pushed (swap
                 // only when dist_left > dist_right)
-                const char* cp = nodes + (size_t)ref * 32u;
-                const float4 a = gld4(cp), b = gld4(cp + 16), c = gld4(cp + 32), d = gld4(cp + 48);
+                float4 a, b, c, d;
+                if (TLDS && !in_blas) {
+                    const float4* q = s_tl + 2u * ref;
+                    a = q[0]; b = q[1]; c = q[2]; d = q[3];
+                } else {
+                    const char* cp = nodes + (size_t)ref * 32u;
+                    a = gld4(cp); b = gld4(cp + 16); c = gld4(cp + 32); d = gld4(cp + 48);
+                }
                 if (COUNT) cnt[in_blas ? 1 : 0] += 2;
                 const float dl = slab(a.x, a.y, a.z, b.x, b.y, b.z, cr, t);
                 const float dr = slab(c.x, c.y, c.z, d.x, d.y, d.z, cr, t);
@@ -345,23 +351,31 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, ui
     }
 }
 
-template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY>
-__global__ __launch_bounds__(BLOCK) void k_extend(ExtendLaunch L)
+// TLDS: the workgroup first copies the TLAS records (at most
+// kTlasLdsPairs) to LDS and reads TLAS nodes from there.
+template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS>
+__global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
 {
     constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
     __shared__ uint32_t s_ent[kLdsN];
     __shared__ float s_dist[kLdsN];
+    __shared__ float4 s_tl[TLDS ? kTlasLdsPairs * 4 : 1];
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t n = L.d_count ? *L.d_count : L.n;
+    if constexpr (TLDS) {
+        const uint32_t n4 = 4u * L.scene.n_tpairs;
+        for (uint32_t k = tid; k < n4; k += BLOCK) s_tl[k] = gld4(L.scene.tpairs + k);
+        __syncthreads();
+    }
     if constexpr (LDS > 0) {
         LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + (size_t)gtid * L.spill_per_thread, L.spill_per_thread,
                                 L.stack_cap};
-        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY>(L, st, tid & 63u, n);
+        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS>(L, st, s_tl, tid & 63u, n);
     } else {
         PrivStack<128> st;
         st.cap = L.stack_cap;
-        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY>(L, st, tid & 63u, n);
+        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS>(L, st, s_tl, tid & 63u, n);
     }
 }
 
@@ -371,14 +385,15 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n)
     if (i < n) p[i] = v;
 }
 
-template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY>
-void* kernel_ptr() { return reinterpret_cast<void*>(&k_extend<BLOCK, LDS, SKIP, COUNT, ANY>); }
+template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS>
+void* kernel_ptr() { return reinterpret_cast<void*>(&k_extend<BLOCK, LDS, SKIP, COUNT, ANY, TLDS>); }
 
-template <bool COUNT, bool ANY>
+template <bool COUNT, bool ANY, bool TLDS>
 void* pick_lds(uint32_t block, uint32_t lds, bool skip)
 {
 #define WF_CASE(B, S) \
-    if (block == B && lds == S) return skip ? kernel_ptr<B, S, true, COUNT, ANY>() : kernel_ptr<B, S, false, COUNT, ANY>();
+    if (block == B && lds == S) \
+        return skip ? kernel_ptr<B, S, true, COUNT, ANY, TLDS>() : kernel_ptr<B, S, false, COUNT, ANY, TLDS>();
     WF_CASE(64, 0) WF_CASE(64, 8) WF_CASE(64, 16) WF_CASE(64, 32)
     WF_CASE(128, 0) WF_CASE(128, 8) WF_CASE(128, 16) WF_CASE(128, 32)
     WF_CASE(256, 0) WF_CASE(256, 8) WF_CASE(256, 16) WF_CASE(256, 32)
@@ -386,29 +401,36 @@ void* pick_lds(uint32_t block, uint32_t lds, bool skip)
     return nullptr;
 }
 
-void* pick_kernel(const ExtendConfig& c, bool count, bool any)
+template <bool TLDS>
+void* pick_tlds(const ExtendConfig& c, bool count, bool any)
 {
-    if (count) return any ? nullptr : pick_lds<true, false>(c.block, c.lds_stack, c.skip_popped != 0);
-    return any ? pick_lds<false, true>(c.block, c.lds_stack, c.skip_popped != 0)
-               : pick_lds<false, false>(c.block, c.lds_stack, c.skip_popped != 0);
+    if (count) return any ? nullptr : pick_lds<true, false, TLDS>(c.block, c.lds_stack, c.skip_popped != 0);
+    return any ? pick_lds<false, true, TLDS>(c.block, c.lds_stack, c.skip_popped != 0)
+               : pick_lds<false, false, TLDS>(c.block, c.lds_stack, c.skip_popped != 0);
+}
+
+void* pick_kernel(const ExtendConfig& c, bool count, bool any, bool tlds)
+{
+    return tlds ? pick_tlds<true>(c, count, any) : pick_tlds<false>(c, count, any);
 }
 
 } // namespace
 
-hipError_t extend_occupancy(const ExtendConfig& cfg, int* blocks_per_cu)
+hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, int* blocks_per_cu)
 {
-    void* k = pick_kernel(cfg, false, false);
+    void* k = pick_kernel(cfg, false, false, tlas_lds);
     if (!k) return hipErrorInvalidValue;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, (int)cfg.block, 0);
 }
 
 hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStream_t stream)
 {
-    void* k = pick_kernel(cfg, L.counters != nullptr, L.any_hit != 0);
+    const bool tlds = cfg.tlas_lds != 0 && L.scene.n_tpairs <= kTlasLdsPairs;
+    void* k = pick_kernel(cfg, L.counters != nullptr, L.any_hit != 0, tlds);
     if (!k) return hipErrorInvalidValue;
     ExtendLaunch arg = L;
     void* args[] = {&arg};
-    return hipLaunchKernel(k, dim3(L.grid), dim3(cfg.block), args, 0, stream);
+    return hipLaunchKernel(k, dim3(tlds ? L.grid_tlds : L.grid), dim3(cfg.block), args, 0, stream);
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t stream)

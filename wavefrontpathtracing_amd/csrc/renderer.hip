@@ -94,11 +94,14 @@ static wf_render_settings default_settings()
 wf_status wf_renderer::ensure_trace_scratch()
 {
     // persistent grid: as many workgroups as stay resident on every CU
-    int per_cu = 0;
-    hipError_t e = extend_occupancy(ext_cfg, &per_cu);
+    int per_cu = 0, per_cu_t = 0;
+    hipError_t e = extend_occupancy(ext_cfg, false, &per_cu);
+    if (e == hipSuccess) e = extend_occupancy(ext_cfg, true, &per_cu_t);
     if (e != hipSuccess) return hip_fail(e, "occupancy query");
     if (per_cu < 1) per_cu = 1;
+    if (per_cu_t < 1) per_cu_t = 1;
     ext_grid = (uint32_t)per_cu * (uint32_t)cu_count;
+    ext_grid_tlds = std::min((uint32_t)per_cu_t * (uint32_t)cu_count, ext_grid);
     spill_per_thread = ext_cfg.lds_stack > 0 ? (kStackTotal > ext_cfg.lds_stack ? kStackTotal - ext_cfg.lds_stack : 0) : 0;
     const size_t threads = (size_t)ext_grid * ext_cfg.block;
     if ((e = d_spill.ensure(std::max<size_t>(threads * spill_per_thread * sizeof(uint2), 16))) != hipSuccess)
@@ -215,6 +218,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     r->ext_cfg.skip_popped = c->skip_popped_interior ? 1u : 0u;
     r->ext_cfg.stack_limit = c->stack_limit;
     r->ext_cfg.refill_min = c->refill_min ? c->refill_min : ExtendConfig{}.refill_min;
+    r->ext_cfg.tlas_lds = c->tlas_in_global ? 0u : 1u;
     r->drop_graph();
     const wf_status s = r->ensure_trace_scratch();
     if (s != WF_OK) { r->ext_cfg = old; return s; }
@@ -230,6 +234,7 @@ extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* 
     out->bvh_layout = 0;
     out->stack_limit = r->ext_cfg.stack_limit;
     out->refill_min = r->ext_cfg.refill_min;
+    out->tlas_in_global = r->ext_cfg.tlas_lds ? 0u : 1u;
     return WF_OK;
 }
 
@@ -435,17 +440,25 @@ wf_status wf_renderer::upload_scene()
             si.push_back(sh);
         }
     }
-    // child-pair records of the interior TLAS nodes (see TraceScene)
-    std::vector<float> pairs(16 * all_nodes.size(), 0.0f);
+    // child-pair records of the interior TLAS nodes, numbered densely (see
+    // TraceScene); the device node copy carries each interior node's number
+    std::vector<uint32_t> rec_of(all_nodes.size(), 0);
+    uint32_t n_rec = 0;
+    for (size_t k = 0; k < all_nodes.size(); ++k)
+        if (all_nodes[k].left_right != 0) rec_of[k] = n_rec++;
+    std::vector<wf_tlas_node> dev_nodes = all_nodes;
+    for (size_t k = 0; k < all_nodes.size(); ++k)
+        if (all_nodes[k].left_right != 0) dev_nodes[k].instance_idx = rec_of[k];
+    std::vector<float> pairs(16 * std::max<uint32_t>(n_rec, 1), 0.0f);
     for (size_t k = 0; k < all_nodes.size(); ++k) {
         const uint32_t lr = all_nodes[k].left_right;
         if (lr == 0) continue;
         const uint32_t child[2] = {lr >> 16, lr & 0xFFFFu};
         for (int c = 0; c < 2; ++c) {
             const wf_tlas_node& cn = all_nodes[child[c]];
-            float* slot = &pairs[16 * k + 8 * c];
+            float* slot = &pairs[16 * rec_of[k] + 8 * c];
             const bool leaf = cn.left_right == 0;
-            const uint32_t ref = leaf ? cn.instance_idx : 2u * child[c];
+            const uint32_t ref = leaf ? cn.instance_idx : 2u * rec_of[child[c]];
             const uint32_t kind = leaf ? 1u : 0u;
             std::memcpy(slot, cn.aabb_min, 12);
             std::memcpy(slot + 3, &ref, 4);
@@ -459,11 +472,12 @@ wf_status wf_renderer::upload_scene()
     if (e == hipSuccess) e = hipMemcpyAsync(d_tpairs.p, pairs.data(), sizeof(float) * pairs.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = d_inst.ensure(sizeof(GpuInstance) * gi.size());
     if (e == hipSuccess) e = d_shade_inst.ensure(sizeof(GpuShadeInstance) * si.size());
-    if (e == hipSuccess) e = hipMemcpyAsync(d_tnodes.p, all_nodes.data(), sizeof(wf_tlas_node) * all_nodes.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tnodes.p, dev_nodes.data(), sizeof(wf_tlas_node) * dev_nodes.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_inst.p, gi.data(), sizeof(GpuInstance) * gi.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_shade_inst.p, si.data(), sizeof(GpuShadeInstance) * si.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "TLAS upload");
+    n_tpairs = n_rec;
     wf_status s = upload_materials_and_emitters();
     if (s != WF_OK) return s;
     tlas_valid = true;
@@ -498,6 +512,7 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     ExtendLaunch L{};
     L.scene.tnodes = (const wf_tlas_node*)d_tnodes.p;
     L.scene.tpairs = (const float4*)d_tpairs.p;
+    L.scene.n_tpairs = n_tpairs;
     L.scene.inst = (const GpuInstance*)d_inst.p;
     L.rays = d_rays;
     L.ray_root = d_roots;
@@ -511,6 +526,7 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     L.spill_per_thread = spill_per_thread;
     L.stack_cap = ext_cfg.stack_limit ? std::min(ext_cfg.stack_limit, kStackTotal) : kStackTotal;
     L.grid = ext_grid;
+    L.grid_tlds = ext_grid_tlds;
     L.any_hit = any ? 1 : 0;
     L.refill_min = ext_cfg.refill_min;
     e = launch_extend(L, ext_cfg, stream);

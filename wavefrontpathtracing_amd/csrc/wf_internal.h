@@ -51,25 +51,33 @@ struct GpuTexture {
     uint32_t width, height;
 };
 
-// TLAS child-pair records: record k (64 B) holds the two children of
-// interior TLAS node k as {aabb_min, ref, aabb_max, kind}, with kind 0 and
-// ref = 2 * child for an interior child (its record sits at ref * 32 bytes)
+// TLAS child-pair records: interior TLAS nodes are numbered densely in node
+// order; record j (64 B) holds the two children of interior node j as
+// {aabb_min, ref, aabb_max, kind}, with kind 0 and ref = 2 * (the child's
+// record number) for an interior child (its record sits at ref * 32 bytes)
 // and kind 1, ref = instance for a leaf; the same layout as two adjacent
 // BLAS nodes {aabb_min, left_first, aabb_max, prim_count}, so one visit
-// routine serves both levels.
+// routine serves both levels.  The device copy of the reference-layout
+// nodes (used for the root box test) carries an interior node's record
+// number in its otherwise unused instance_idx.
 struct TraceScene {
-    const wf_tlas_node* tnodes;   // reference-layout nodes (root box test)
+    const wf_tlas_node* tnodes;
     const float4* tpairs;
+    uint32_t n_tpairs;
     const GpuInstance* inst;
 };
 
+// TLAS records that fit the LDS copy a workgroup makes (16 KB)
+constexpr uint32_t kTlasLdsPairs = 256;
+
 // Traversal configuration (the autotuner's axes).
 struct ExtendConfig {
-    uint32_t block = 128;       // workgroup size: 64, 128, 256
-    uint32_t lds_stack = 16;    // LDS stack entries per lane (0 = private/scratch stack)
+    uint32_t block = 256;       // workgroup size: 64, 128, 256
+    uint32_t lds_stack = 8;     // LDS stack entries per lane (0 = private/scratch stack)
     uint32_t skip_popped = 1;   // skip popped interior nodes whose entry distance >= t
     uint32_t stack_limit = 0;   // cap on stack entries per lane (0 = capacity)
     uint32_t refill_min = 16;   // idle lanes of a wave that trigger a refill
+    uint32_t tlas_lds = 1;      // copy the TLAS records to LDS when they fit
 };
 
 // Launch the closest-hit kernel.  d_count (device) overrides n when non-null.
@@ -86,12 +94,13 @@ struct ExtendLaunch {
     uint2* spill;              // global overflow stack
     uint32_t spill_per_thread;
     uint32_t stack_cap;        // entries a lane may hold (LDS + spill, or private)
-    uint32_t grid;             // persistent blocks
+    uint32_t grid;             // persistent blocks (TLAS in global memory)
+    uint32_t grid_tlds;        // persistent blocks of the LDS-TLAS variant
     int any_hit;
     uint32_t refill_min;       // idle lanes of a wave that trigger a refill (1..64)
 };
 hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStream_t stream);
-hipError_t extend_occupancy(const ExtendConfig& cfg, int* blocks_per_cu);
+hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, int* blocks_per_cu);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t stream);
 
 } // namespace wf

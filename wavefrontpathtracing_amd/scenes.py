@@ -109,6 +109,15 @@ def sphere_rays(seed: int, bmin, bmax, n: int) -> np.ndarray:
     return out
 
 
+def primary_rays(scene: "Scene") -> np.ndarray:
+    """The generate stage's camera rays for the scene's camera and size."""
+    from .renderer import camera_create, make_view
+    view = make_view(camera_create(scene.eye, scene.target, scene.vfov_deg), scene.width, scene.height)
+    out = np.zeros(scene.width * scene.height, dtype=L.RAY)
+    check(lib().wfs_primary_rays(C.byref(view), scene.width, scene.height, ptr(out)), what="primary_rays")
+    return out
+
+
 class Rng:
     def __init__(self, seed: int, seq: int = 0x5eed0010):
         self._s = L.Pcg32()
