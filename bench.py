@@ -308,12 +308,12 @@ def main():
             ev_b.synchronize()
             return ev_a.elapsed_time(ev_b) * 1e-3
 
-        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, refill_min=f, tlas_in_global=g)
-                for b in (64, 128, 256) for l in (0, 8, 16) for f in (8, 16, 24) for g in (0, 1)]
+        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, refill_min=f, tlas_in_global=0, interior_visits=v)
+                for b, l in ((64, 16), (128, 8), (256, 8), (64, 8), (128, 16)) for f in (12, 16, 20) for v in (4, 5, 6, 8)]
         best, table = tuning.autotune(r, timed, configs=cfgs, repeats=3)
         for c, t in table:
             print(f"sweep block={c['block']} lds={c['lds_stack']} refill={c['refill_min']} "
-                  f"tlas_global={c['tlas_in_global']}: {len(rays) / t / 1e6:.1f} Mrays/s", file=sys.stderr)
+                  f"visits={c['interior_visits']}: {len(rays) / t / 1e6:.1f} Mrays/s", file=sys.stderr)
 
     for _ in range(args.warmup):
         step()

@@ -175,6 +175,7 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
 {
     const TraceScene sc = L.scene;
     const uint32_t refill_min = L.refill_min;
+    const uint32_t visits = L.visits;
     Ray3 wr, cr;
     float t = 0.0f;
     uint32_t ref = 0, kind = 0, sp = 0, base_sp = 0;
@@ -225,8 +226,36 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
                 kind = lr == 0 ? 1u : 0u;
             }
         } else if (ray_i != ~0u) {
-            bool pop = false;
-            if (kind == 0) {
+            // pop until the next node to process is known (or the stack is empty)
+            auto pop_next = [&]() {
+                for (;;) {
+                    if (in_blas && sp == base_sp) {
+                        in_blas = false;
+                        cr = wr;
+                        nodes = reinterpret_cast<const char*>(sc.tpairs);
+                    }
+                    if (sp == 0) { status = kDone; return; }
+                    uint32_t e; float dd;
+                    --sp;
+                    st.get(sp, e, dd);
+                    if (e & 0x80000000u) {
+                        const uint32_t k4 = (e >> 27) & 15u;
+                        if (k4) { kind = k4; ref = e & 0x07FFFFFFu; }
+                        else {
+                            const char* nd = nodes + (size_t)(e & 0x07FFFFFFu) * 32u;
+                            ref = gldu(nd + 12); kind = gldu(nd + 28);
+                        }
+                        return;
+                    }
+                    if (SKIP && dd >= t) { if (COUNT) cnt[in_blas ? 1 : 0] += 2; continue; }
+                    ref = e; kind = 0;
+                    return;
+                }
+            };
+            // up to `visits` interior nodes per pass (each with its pops), so
+            // the rarer instance and leaf steps below are shared by more work
+            for (uint32_t v = 0; v < visits; ++v) {
+                if (status != kRun || kind != 0) break;
                 // interior node: near child next, far child pushed (swap
                 // only when dist_left > dist_right)
                 float4 a, b, c, d;
@@ -246,7 +275,7 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
                 const uint32_t f_ref = __float_as_uint(sw ? a.w : c.w), f_kind = __float_as_uint(sw ? b.w : d.w);
                 const uint32_t f_id = ref + (sw ? 0u : 1u);
                 if (dn == kRayMaxT) {
-                    pop = true;
+                    pop_next();
                 } else {
                     ref = n_ref; kind = n_kind;
                     if (df != kRayMaxT) {
@@ -259,7 +288,7 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
                     }
                 }
             }
-            if (status == kRun && !pop && kind != 0 && !in_blas) {
+            if (status == kRun && kind != 0 && !in_blas) {
                 // TLAS leaf: enter the instance (trace_ray_bvh_instance); the
                 // BLAS root is entered without a box test
                 if (COUNT) { cnt[2]++; cnt[1]++; }
@@ -290,7 +319,7 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
                 base_sp = sp;
                 in_blas = true;
             }
-            if (status == kRun && !pop && kind != 0 && in_blas) {
+            if (status == kRun && kind != 0 && in_blas) {
                 // BLAS leaf: triangles in index order; a hit replaces the
                 // closest only if t < ray.t
                 for (uint32_t k = ref; k < ref + kind; ++k) {
@@ -303,32 +332,7 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
                         if (ANY) { status = kDone; break; }
                     }
                 }
-                pop = true;
-            }
-            if (status == kRun && pop) {
-                for (;;) {
-                    if (in_blas && sp == base_sp) {
-                        in_blas = false;
-                        cr = wr;
-                        nodes = reinterpret_cast<const char*>(sc.tpairs);
-                    }
-                    if (sp == 0) { status = kDone; break; }
-                    uint32_t e; float dd;
-                    --sp;
-                    st.get(sp, e, dd);
-                    if (e & 0x80000000u) {
-                        const uint32_t k4 = (e >> 27) & 15u;
-                        if (k4) { kind = k4; ref = e & 0x07FFFFFFu; }
-                        else {
-                            const char* nd = nodes + (size_t)(e & 0x07FFFFFFu) * 32u;
-                            ref = gldu(nd + 12); kind = gldu(nd + 28);
-                        }
-                        break;
-                    }
-                    if (SKIP && dd >= t) { if (COUNT) cnt[in_blas ? 1 : 0] += 2; continue; }
-                    ref = e; kind = 0;
-                    break;
-                }
+                if (status == kRun) pop_next();
             }
         }
         if (status != kRun) {

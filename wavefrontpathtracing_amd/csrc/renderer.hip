@@ -212,6 +212,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
         return r->fail(WF_ERROR_INVALID_ARGUMENT, "lds_stack_entries must be 0, 8, 16 or 32");
     if (c->bvh_layout != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "bvh_layout must be 0 (BVH2)");
     if (c->refill_min > 64) return r->fail(WF_ERROR_INVALID_ARGUMENT, "refill_min must be in [0, 64]");
+    if (c->interior_visits > 8) return r->fail(WF_ERROR_INVALID_ARGUMENT, "interior_visits must be in [0, 8]");
     const ExtendConfig old = r->ext_cfg;
     r->ext_cfg.block = c->workgroup_size;
     r->ext_cfg.lds_stack = c->lds_stack_entries;
@@ -219,6 +220,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     r->ext_cfg.stack_limit = c->stack_limit;
     r->ext_cfg.refill_min = c->refill_min ? c->refill_min : ExtendConfig{}.refill_min;
     r->ext_cfg.tlas_lds = c->tlas_in_global ? 0u : 1u;
+    r->ext_cfg.visits = c->interior_visits ? c->interior_visits : ExtendConfig{}.visits;
     r->drop_graph();
     const wf_status s = r->ensure_trace_scratch();
     if (s != WF_OK) { r->ext_cfg = old; return s; }
@@ -235,6 +237,7 @@ extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* 
     out->stack_limit = r->ext_cfg.stack_limit;
     out->refill_min = r->ext_cfg.refill_min;
     out->tlas_in_global = r->ext_cfg.tlas_lds ? 0u : 1u;
+    out->interior_visits = r->ext_cfg.visits;
     return WF_OK;
 }
 
@@ -529,6 +532,7 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     L.grid_tlds = ext_grid_tlds;
     L.any_hit = any ? 1 : 0;
     L.refill_min = ext_cfg.refill_min;
+    L.visits = ext_cfg.visits;
     e = launch_extend(L, ext_cfg, stream);
     if (e != hipSuccess) return hip_fail(e, "extend launch");
     return WF_OK;

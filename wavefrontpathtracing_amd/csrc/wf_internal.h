@@ -72,12 +72,13 @@ constexpr uint32_t kTlasLdsPairs = 256;
 
 // Traversal configuration (the autotuner's axes).
 struct ExtendConfig {
-    uint32_t block = 256;       // workgroup size: 64, 128, 256
-    uint32_t lds_stack = 8;     // LDS stack entries per lane (0 = private/scratch stack)
+    uint32_t block = 64;        // workgroup size: 64, 128, 256
+    uint32_t lds_stack = 16;    // LDS stack entries per lane (0 = private/scratch stack)
     uint32_t skip_popped = 1;   // skip popped interior nodes whose entry distance >= t
     uint32_t stack_limit = 0;   // cap on stack entries per lane (0 = capacity)
     uint32_t refill_min = 16;   // idle lanes of a wave that trigger a refill
     uint32_t tlas_lds = 1;      // copy the TLAS records to LDS when they fit
+    uint32_t visits = 4;        // interior visits per pass of the traversal loop
 };
 
 // Launch the closest-hit kernel.  d_count (device) overrides n when non-null.
@@ -98,6 +99,7 @@ struct ExtendLaunch {
     uint32_t grid_tlds;        // persistent blocks of the LDS-TLAS variant
     int any_hit;
     uint32_t refill_min;       // idle lanes of a wave that trigger a refill (1..64)
+    uint32_t visits;           // interior visits per pass of the traversal loop (1..8)
 };
 hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStream_t stream);
 hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, int* blocks_per_cu);

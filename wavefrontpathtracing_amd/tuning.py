@@ -11,13 +11,15 @@ import itertools
 
 from . import _lib as L
 
-DEFAULT = dict(block=256, lds_stack=8, skip_popped=1, refill_min=16, tlas_in_global=0)
+DEFAULT = dict(block=64, lds_stack=16, skip_popped=1, refill_min=16, tlas_in_global=0, interior_visits=4)
 
 
 def set_extend_config(renderer, block: int = DEFAULT["block"], lds_stack: int = DEFAULT["lds_stack"],
                       skip_popped: int = DEFAULT["skip_popped"], bvh_layout: int = 0, stack_limit: int = 0,
-                      refill_min: int = DEFAULT["refill_min"], tlas_in_global: int = DEFAULT["tlas_in_global"]) -> None:
-    c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout, stack_limit, refill_min, tlas_in_global)
+                      refill_min: int = DEFAULT["refill_min"], tlas_in_global: int = DEFAULT["tlas_in_global"],
+                      interior_visits: int = DEFAULT["interior_visits"]) -> None:
+    c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout, stack_limit, refill_min, tlas_in_global,
+                          interior_visits)
     L.check(L.lib().wf_renderer_set_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "set_traversal_config")
 
 
@@ -26,7 +28,7 @@ def get_extend_config(renderer) -> dict:
     L.check(L.lib().wf_renderer_get_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "get_traversal_config")
     return dict(block=c.workgroup_size, lds_stack=c.lds_stack_entries, skip_popped=c.skip_popped_interior,
                 bvh_layout=c.bvh_layout, stack_limit=c.stack_limit, refill_min=c.refill_min,
-                tlas_in_global=c.tlas_in_global)
+                tlas_in_global=c.tlas_in_global, interior_visits=c.interior_visits)
 
 
 def candidates():
