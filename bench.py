@@ -123,6 +123,41 @@ def bench_soup(n_tris: int, steps: int, cpu: bool):
     return out
 
 
+def bench_frame(frames: int):
+    """One wavefront frame of the 256-instance scene at 1920x1080, 1 spp,
+    depth 5, Russian roulette from bounce 3, next-event estimation to the 4
+    emissive quads, black environment: stage times from the library's own
+    events (direct launches), then the same frame as a captured graph."""
+    from wavefrontpathtracing_amd import Renderer, scenes
+    sc = scenes.tlas_scene()
+    r = Renderer(sc.width, sc.height)
+    scenes.load(r, sc)
+    r.update_settings(max_bounces=5, spp=1, next_event_estimation=1, hdr_env_strength=0.0, frame_seed=0,
+                      rr_start_bounce=3, rr_survival=0.75, accumulate=1, use_hip_graph=0)
+    r.render()
+    keys = ("ms_clear", "ms_generate", "ms_extend", "ms_shade", "ms_connect", "ms_post", "ms_total")
+    acc = {k: 0.0 for k in keys}
+    rays = 0
+    for _ in range(frames):
+        r.render()
+        st = r.frame_stats()
+        for k in keys:
+            acc[k] += getattr(st, k)
+        rays += st.rays_extended + st.shadow_rays
+    r.update_settings(use_hip_graph=1)
+    r.render()
+    gt = 0.0
+    for _ in range(frames):
+        r.render()
+        gt += r.frame_stats().ms_total
+    r.exit()
+    out = {k: round(v / frames, 3) for k, v in acc.items()}
+    out.update({"workload": "256-instance scene, 1920x1080, 1 spp, depth 5, RR from 3, NEE, black environment",
+                "rays_per_frame": rays // frames, "mrays_per_s": round(rays / (acc["ms_total"] * 1e-3) / 1e6, 2),
+                "graph_ms_total": round(gt / frames, 3)})
+    return out
+
+
 def bench_batched(frames: int, tune: bool):
     """64 uneven scenes (soup of 10K..2M triangles in a lit box room, 256^2 /
     512^2 / 1024^2 pixels), 4 spp, depth 4, next-event estimation: the whole
@@ -148,9 +183,11 @@ def bench_batched(frames: int, tune: bool):
 
     tuned = None
     if tune:
-        cfgs = [dict(block=b, lds_stack=l, skip_popped=1) for b in (64, 128, 256) for l in (0, 16)]
+        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, interior_visits=v)
+                for b in (64, 128, 256) for l in (8, 16) for v in (2, 4)]
         tuned, table = tuning.autotune(r, lambda: batched_frame()[0], configs=cfgs, repeats=1)
-        tuned = dict(tuned, sweep={f"{c['block']}/{c['lds_stack']}": round(t * 1e3, 1) for c, t in table})
+        tuned = dict(tuned, sweep={f"{c['block']}/{c['lds_stack']}/{c['interior_visits']}": round(t * 1e3, 1)
+                                   for c, t in table})
     batched_frame()
     bt, brays = 0.0, 0
     for _ in range(frames):
@@ -255,6 +292,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-soup", action="store_true")
+    ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune the traversal kernel on the batched frame")
     ap.add_argument("--sweep", action="store_true",
                     help="autotune the traversal kernel on the incoherent workload first (table on stderr)")
@@ -308,11 +346,11 @@ def main():
             ev_b.synchronize()
             return ev_a.elapsed_time(ev_b) * 1e-3
 
-        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, refill_min=f, tlas_in_global=0, interior_visits=v)
-                for b, l in ((64, 16), (128, 8), (256, 8), (64, 8), (128, 16)) for f in (12, 16, 20) for v in (4, 5, 6, 8)]
+        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, refill_min=f, tlas_in_lds=g, interior_visits=v)
+                for b in (64, 128, 256) for l in (8, 16) for f in (12, 16) for g in (0, 1) for v in (2, 4)]
         best, table = tuning.autotune(r, timed, configs=cfgs, repeats=3)
         for c, t in table:
-            print(f"sweep block={c['block']} lds={c['lds_stack']} refill={c['refill_min']} "
+            print(f"sweep block={c['block']} lds={c['lds_stack']} refill={c['refill_min']} tlas_lds={c['tlas_in_lds']} "
                   f"visits={c['interior_visits']}: {len(rays) / t / 1e6:.1f} Mrays/s", file=sys.stderr)
 
     for _ in range(args.warmup):
@@ -375,6 +413,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, rays)
     r.exit()
+    if not args.no_frame:
+        out["frame"] = bench_frame(frames=5)
     if not args.no_soup:
         out["soup_65k"] = bench_soup(65536, steps=10, cpu=not args.no_cpu)
         out["soup_1m"] = bench_soup(1 << 20, steps=10, cpu=False)

@@ -200,20 +200,21 @@ WF_API wf_status wf_renderer_set_build_options(wf_renderer* r, const wf_build_op
 
 /* Traversal kernel configuration (no reference counterpart: the autotuner's
  * axes).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
- * scratch), 8, 16 or 32 (LDS, spilling to global memory past that depth);
+ * scratch), 4, 8 or 16 (LDS, spilling to global memory past that depth);
  * skip_popped_interior drops popped interior nodes whose entry distance is
  * not below the current hit distance; bvh_layout 0 = BVH2; stack_limit caps
  * the traversal stack below its 128 entries per lane (0 = no cap; a ray that
  * would exceed it reports WF_ERROR_STACK_OVERFLOW); refill_min (1..64, 0 =
  * default 16) is how many lanes of a 64-lane wave must have finished their
- * rays before the wave claims new ones; tlas_in_global 1 keeps the TLAS in
- * global memory even when it fits the 16 KB LDS copy (0 = copy it);
+ * rays before the wave claims new ones; tlas_in_lds 1 copies the TLAS into
+ * each workgroup's LDS when it fits (at most 1024 interior nodes and 64 KB
+ * of LDS with the stack; 0 = read it through the caches);
  * interior_visits (1..8, 0 = default 4) is how many interior nodes a lane may
  * visit per pass of the traversal loop before the pass's instance-entry and
  * leaf steps. */
 typedef struct wf_traversal_config {
     uint32_t workgroup_size; uint32_t lds_stack_entries; uint32_t skip_popped_interior; uint32_t bvh_layout;
-    uint32_t stack_limit; uint32_t refill_min; uint32_t tlas_in_global; uint32_t interior_visits;
+    uint32_t stack_limit; uint32_t refill_min; uint32_t tlas_in_lds; uint32_t interior_visits;
 } wf_traversal_config;
 WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, const wf_traversal_config* config);
 WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* r, wf_traversal_config* out);

@@ -161,19 +161,19 @@ def test_stack_overflow_reports_status(small, lds):
     _assert_bitexact(r.trace_closest(sr), ref, "after the stack cap is lifted")
 
 
-@pytest.mark.parametrize("block,lds,skip,tglobal,refill,visits", [
-    (64, 8, 1, 0, 16, 1), (128, 0, 1, 0, 1, 2), (256, 32, 0, 0, 64, 3), (128, 16, 0, 1, 16, 8), (256, 8, 1, 1, 32, 4),
-    (64, 0, 1, 1, 48, 1)])
-def test_traversal_configs_agree(small, block, lds, skip, tglobal, refill, visits):
+@pytest.mark.parametrize("block,lds,skip,tlds,refill,visits", [
+    (64, 8, 1, 1, 16, 1), (128, 0, 1, 1, 1, 2), (256, 4, 0, 1, 64, 3), (128, 16, 0, 0, 16, 8), (256, 8, 1, 0, 32, 4),
+    (64, 0, 1, 0, 48, 1)])
+def test_traversal_configs_agree(small, block, lds, skip, tlds, refill, visits):
     sc, r, orc = small
     from wavefrontpathtracing_amd import tuning
     bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
     sr = scenes.sphere_rays(11, bmin, bmax, 20000)
     ref = orc.trace_closest(sr, nthreads=8)
-    tuning.set_extend_config(r, block=block, lds_stack=lds, skip_popped=skip, tlas_in_global=tglobal, refill_min=refill,
+    tuning.set_extend_config(r, block=block, lds_stack=lds, skip_popped=skip, tlas_in_lds=tlds, refill_min=refill,
                              interior_visits=visits)
     try:
-        _assert_bitexact(r.trace_closest(sr), ref, f"config {block}/{lds}/{skip}/{tglobal}/{refill}/{visits}")
+        _assert_bitexact(r.trace_closest(sr), ref, f"config {block}/{lds}/{skip}/{tlds}/{refill}/{visits}")
     finally:
         tuning.set_extend_config(r)
 

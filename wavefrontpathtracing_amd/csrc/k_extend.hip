@@ -355,15 +355,18 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
     }
 }
 
+extern __shared__ float4 s_tlas_dyn[];
+
 // TLDS: the workgroup first copies the TLAS records (at most
-// kTlasLdsPairs) to LDS and reads TLAS nodes from there.
+// kTlasLdsPairs, in dynamic LDS sized by the launch) and reads TLAS nodes
+// from there.
 template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS>
 __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
 {
     constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
     __shared__ uint32_t s_ent[kLdsN];
     __shared__ float s_dist[kLdsN];
-    __shared__ float4 s_tl[TLDS ? kTlasLdsPairs * 4 : 1];
+    float4* s_tl = s_tlas_dyn;
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t n = L.d_count ? *L.d_count : L.n;
@@ -398,9 +401,9 @@ void* pick_lds(uint32_t block, uint32_t lds, bool skip)
 #define WF_CASE(B, S) \
     if (block == B && lds == S) \
         return skip ? kernel_ptr<B, S, true, COUNT, ANY, TLDS>() : kernel_ptr<B, S, false, COUNT, ANY, TLDS>();
-    WF_CASE(64, 0) WF_CASE(64, 8) WF_CASE(64, 16) WF_CASE(64, 32)
-    WF_CASE(128, 0) WF_CASE(128, 8) WF_CASE(128, 16) WF_CASE(128, 32)
-    WF_CASE(256, 0) WF_CASE(256, 8) WF_CASE(256, 16) WF_CASE(256, 32)
+    WF_CASE(64, 0) WF_CASE(64, 4) WF_CASE(64, 8) WF_CASE(64, 16)
+    WF_CASE(128, 0) WF_CASE(128, 4) WF_CASE(128, 8) WF_CASE(128, 16)
+    WF_CASE(256, 0) WF_CASE(256, 4) WF_CASE(256, 8) WF_CASE(256, 16)
 #undef WF_CASE
     return nullptr;
 }
@@ -420,21 +423,22 @@ void* pick_kernel(const ExtendConfig& c, bool count, bool any, bool tlds)
 
 } // namespace
 
-hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, int* blocks_per_cu)
+hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, size_t dyn_lds_bytes, int* blocks_per_cu)
 {
     void* k = pick_kernel(cfg, false, false, tlas_lds);
     if (!k) return hipErrorInvalidValue;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, (int)cfg.block, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, (int)cfg.block, dyn_lds_bytes);
 }
 
 hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStream_t stream)
 {
-    const bool tlds = cfg.tlas_lds != 0 && L.scene.n_tpairs <= kTlasLdsPairs;
+    const bool tlds = L.grid_tlds != 0;   // the host sets it when the TLAS records fit
     void* k = pick_kernel(cfg, L.counters != nullptr, L.any_hit != 0, tlds);
     if (!k) return hipErrorInvalidValue;
     ExtendLaunch arg = L;
     void* args[] = {&arg};
-    return hipLaunchKernel(k, dim3(tlds ? L.grid_tlds : L.grid), dim3(cfg.block), args, 0, stream);
+    const size_t dyn = tlds ? (size_t)L.scene.n_tpairs * 64u : 0u;
+    return hipLaunchKernel(k, dim3(tlds ? L.grid_tlds : L.grid), dim3(cfg.block), args, dyn, stream);
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t stream)

@@ -91,17 +91,31 @@ static wf_render_settings default_settings()
     return s;
 }
 
+// Grid of the LDS-TLAS variant for the current TLAS (0 = use the global
+// variant): its dynamic LDS grows with the TLAS, so its occupancy does too.
+wf_status wf_renderer::update_tlds_grid()
+{
+    ext_grid_tlds = 0;
+    if (!ext_cfg.tlas_lds || n_tpairs == 0 || n_tpairs > kTlasLdsPairs) return WF_OK;
+    // stay within 64 KB of LDS per workgroup (stack + TLAS copy)
+    if ((size_t)ext_cfg.lds_stack * ext_cfg.block * 8u + (size_t)n_tpairs * 64u > 65536u) return WF_OK;
+    int per_cu = 0;
+    const hipError_t e = extend_occupancy(ext_cfg, true, (size_t)n_tpairs * 64u, &per_cu);
+    if (e != hipSuccess) return hip_fail(e, "occupancy query");
+    if (per_cu >= 1) ext_grid_tlds = std::min((uint32_t)per_cu * (uint32_t)cu_count, ext_grid);
+    return WF_OK;
+}
+
 wf_status wf_renderer::ensure_trace_scratch()
 {
     // persistent grid: as many workgroups as stay resident on every CU
-    int per_cu = 0, per_cu_t = 0;
-    hipError_t e = extend_occupancy(ext_cfg, false, &per_cu);
-    if (e == hipSuccess) e = extend_occupancy(ext_cfg, true, &per_cu_t);
+    int per_cu = 0;
+    hipError_t e = extend_occupancy(ext_cfg, false, 0, &per_cu);
     if (e != hipSuccess) return hip_fail(e, "occupancy query");
     if (per_cu < 1) per_cu = 1;
-    if (per_cu_t < 1) per_cu_t = 1;
     ext_grid = (uint32_t)per_cu * (uint32_t)cu_count;
-    ext_grid_tlds = std::min((uint32_t)per_cu_t * (uint32_t)cu_count, ext_grid);
+    wf_status s = update_tlds_grid();
+    if (s != WF_OK) return s;
     spill_per_thread = ext_cfg.lds_stack > 0 ? (kStackTotal > ext_cfg.lds_stack ? kStackTotal - ext_cfg.lds_stack : 0) : 0;
     const size_t threads = (size_t)ext_grid * ext_cfg.block;
     if ((e = d_spill.ensure(std::max<size_t>(threads * spill_per_thread * sizeof(uint2), 16))) != hipSuccess)
@@ -208,8 +222,8 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     if (!r || !c) return WF_ERROR_INVALID_ARGUMENT;
     if (c->workgroup_size != 64 && c->workgroup_size != 128 && c->workgroup_size != 256)
         return r->fail(WF_ERROR_INVALID_ARGUMENT, "workgroup_size must be 64, 128 or 256");
-    if (c->lds_stack_entries != 0 && c->lds_stack_entries != 8 && c->lds_stack_entries != 16 && c->lds_stack_entries != 32)
-        return r->fail(WF_ERROR_INVALID_ARGUMENT, "lds_stack_entries must be 0, 8, 16 or 32");
+    if (c->lds_stack_entries != 0 && c->lds_stack_entries != 4 && c->lds_stack_entries != 8 && c->lds_stack_entries != 16)
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "lds_stack_entries must be 0, 4, 8 or 16");
     if (c->bvh_layout != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "bvh_layout must be 0 (BVH2)");
     if (c->refill_min > 64) return r->fail(WF_ERROR_INVALID_ARGUMENT, "refill_min must be in [0, 64]");
     if (c->interior_visits > 8) return r->fail(WF_ERROR_INVALID_ARGUMENT, "interior_visits must be in [0, 8]");
@@ -219,7 +233,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     r->ext_cfg.skip_popped = c->skip_popped_interior ? 1u : 0u;
     r->ext_cfg.stack_limit = c->stack_limit;
     r->ext_cfg.refill_min = c->refill_min ? c->refill_min : ExtendConfig{}.refill_min;
-    r->ext_cfg.tlas_lds = c->tlas_in_global ? 0u : 1u;
+    r->ext_cfg.tlas_lds = c->tlas_in_lds ? 1u : 0u;
     r->ext_cfg.visits = c->interior_visits ? c->interior_visits : ExtendConfig{}.visits;
     r->drop_graph();
     const wf_status s = r->ensure_trace_scratch();
@@ -236,7 +250,7 @@ extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* 
     out->bvh_layout = 0;
     out->stack_limit = r->ext_cfg.stack_limit;
     out->refill_min = r->ext_cfg.refill_min;
-    out->tlas_in_global = r->ext_cfg.tlas_lds ? 0u : 1u;
+    out->tlas_in_lds = r->ext_cfg.tlas_lds;
     out->interior_visits = r->ext_cfg.visits;
     return WF_OK;
 }
@@ -481,6 +495,10 @@ wf_status wf_renderer::upload_scene()
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "TLAS upload");
     n_tpairs = n_rec;
+    if (ext_grid) {
+        const wf_status ts = update_tlds_grid();
+        if (ts != WF_OK) return ts;
+    }
     wf_status s = upload_materials_and_emitters();
     if (s != WF_OK) return s;
     tlas_valid = true;

@@ -121,6 +121,7 @@ struct wf_renderer {
     wf::Mesh* find_mesh(uint64_t h);
     wf::Texture* find_texture(uint64_t h);
     wf_status ensure_trace_scratch();
+    wf_status update_tlds_grid();
     wf_status upload_scene();
     wf_status upload_materials_and_emitters();
     wf_status run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,

@@ -67,8 +67,8 @@ struct TraceScene {
     const GpuInstance* inst;
 };
 
-// TLAS records that fit the LDS copy a workgroup makes (16 KB)
-constexpr uint32_t kTlasLdsPairs = 256;
+// most TLAS records a workgroup copies to LDS (64 B each, dynamic LDS)
+constexpr uint32_t kTlasLdsPairs = 1024;
 
 // Traversal configuration (the autotuner's axes).
 struct ExtendConfig {
@@ -77,7 +77,7 @@ struct ExtendConfig {
     uint32_t skip_popped = 1;   // skip popped interior nodes whose entry distance >= t
     uint32_t stack_limit = 0;   // cap on stack entries per lane (0 = capacity)
     uint32_t refill_min = 16;   // idle lanes of a wave that trigger a refill
-    uint32_t tlas_lds = 1;      // copy the TLAS records to LDS when they fit
+    uint32_t tlas_lds = 0;      // copy the TLAS records to LDS when they fit
     uint32_t visits = 4;        // interior visits per pass of the traversal loop
 };
 
@@ -102,7 +102,7 @@ struct ExtendLaunch {
     uint32_t visits;           // interior visits per pass of the traversal loop (1..8)
 };
 hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStream_t stream);
-hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, int* blocks_per_cu);
+hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, size_t dyn_lds_bytes, int* blocks_per_cu);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t stream);
 
 } // namespace wf

@@ -1,6 +1,6 @@
 """Traversal kernel configuration and the autotuner over it.
 
-Axes: workgroup size {64, 128, 256} x stack placement {LDS 8/16/32 entries
+Axes: workgroup size {64, 128, 256} x stack placement {LDS 4/8/16 entries
 per lane with global spill, private (scratch) stack} x popped-interior skip
 x refill threshold (finished lanes of a wave before it claims new rays).
 """
@@ -11,14 +11,14 @@ import itertools
 
 from . import _lib as L
 
-DEFAULT = dict(block=64, lds_stack=16, skip_popped=1, refill_min=16, tlas_in_global=0, interior_visits=4)
+DEFAULT = dict(block=64, lds_stack=16, skip_popped=1, refill_min=16, tlas_in_lds=0, interior_visits=4)
 
 
 def set_extend_config(renderer, block: int = DEFAULT["block"], lds_stack: int = DEFAULT["lds_stack"],
                       skip_popped: int = DEFAULT["skip_popped"], bvh_layout: int = 0, stack_limit: int = 0,
-                      refill_min: int = DEFAULT["refill_min"], tlas_in_global: int = DEFAULT["tlas_in_global"],
+                      refill_min: int = DEFAULT["refill_min"], tlas_in_lds: int = DEFAULT["tlas_in_lds"],
                       interior_visits: int = DEFAULT["interior_visits"]) -> None:
-    c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout, stack_limit, refill_min, tlas_in_global,
+    c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout, stack_limit, refill_min, tlas_in_lds,
                           interior_visits)
     L.check(L.lib().wf_renderer_set_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "set_traversal_config")
 
@@ -28,11 +28,11 @@ def get_extend_config(renderer) -> dict:
     L.check(L.lib().wf_renderer_get_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "get_traversal_config")
     return dict(block=c.workgroup_size, lds_stack=c.lds_stack_entries, skip_popped=c.skip_popped_interior,
                 bvh_layout=c.bvh_layout, stack_limit=c.stack_limit, refill_min=c.refill_min,
-                tlas_in_global=c.tlas_in_global, interior_visits=c.interior_visits)
+                tlas_in_lds=c.tlas_in_lds, interior_visits=c.interior_visits)
 
 
 def candidates():
-    for block, lds, skip in itertools.product((64, 128, 256), (0, 8, 16, 32), (1, 0)):
+    for block, lds, skip in itertools.product((64, 128, 256), (0, 4, 8, 16), (1, 0)):
         yield dict(block=block, lds_stack=lds, skip_popped=skip)
 
 
