@@ -34,6 +34,30 @@ def bytes_per_ray(counters: np.ndarray) -> float:
     return 32.0 + 20.0 + 32.0 * (c[0] + c[1]) + 92.0 * c[2] + 40.0 * c[3]
 
 
+PROFILE = os.path.join(ROOT, "profiles", "extend_tlas256_incoherent.md")
+
+
+def measured_traffic(cfg: dict):
+    """HBM bytes per extend launch measured with rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (scaled by a plain copy's counts) for this workload, as
+    recorded in the committed profile summary; None when the summary was
+    taken with another traversal configuration."""
+    try:
+        text = open(PROFILE).read()
+    except OSError:
+        return None, None
+    want = " ".join(f"{k}={v}" for k, v in cfg.items())
+    got, traffic = None, None
+    for line in text.splitlines():
+        if line.startswith("traversal_config:"):
+            got = line.split(":", 1)[1].strip()
+        if line.startswith("traffic_bytes_per_launch:"):
+            traffic = float(line.split(":", 1)[1])
+    if got != want or traffic is None:
+        return None, None
+    return traffic, os.path.relpath(PROFILE, ROOT)
+
+
 def cpu_baseline(scene, rays, seconds_target=12.0):
     """The oracle's closest-hit traversal on the host cores: whole passes
     over the same incoherent rays until about `seconds_target` of work."""
@@ -374,6 +398,7 @@ def main():
     b_ray = bytes_per_ray(counters)
     achieved_gbs = mrays * 1e6 * b_ray / 1e9
     cfg = tuning.get_extend_config(r)
+    traffic, traffic_src = measured_traffic(cfg)
 
     out = {
         "metric": METRIC,
@@ -404,7 +429,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": None if traffic is None else round(traffic),
+            "traffic_unit": "B per launch",
+            "traffic_source": traffic_src,
             "bytes_per_ray": round(b_ray, 1),
             "mean_counters": {k: round(float(v), 3) for k, v in
                               zip(("tlas_nodes", "blas_nodes", "instances", "tri_tests"), counters.mean(axis=0))},
