@@ -235,6 +235,11 @@ WF_API wf_status wf_renderer_read_accumulator(wf_renderer* r, float* out_rgba);
 /* tone-mapped sRGB output (post_process.hlsl:53-71): width*height RGBA8 */
 WF_API wf_status wf_renderer_read_final_color(wf_renderer* r, uint8_t* out_rgba8);
 WF_API wf_status wf_renderer_get_frame_stats(const wf_renderer* r, wf_frame_stats* out);
+/* image files (host only; no reference counterpart, which presents to a
+ * swapchain): linear RGBA float -> PFM (RGB, little endian, bottom row
+ * first); RGBA8 -> binary PPM (RGB) */
+WF_API wf_status wf_write_pfm(const char* path, const float* rgba, uint32_t width, uint32_t height);
+WF_API wf_status wf_write_ppm(const char* path, const uint8_t* rgba8, uint32_t width, uint32_t height);
 
 /* ---- the extend stage on caller rays (extend.hlsl:25-40) -------------- */
 /* Builds the TLAS from the instances submitted since begin_scene and uploads

@@ -3,6 +3,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 from wavefrontpathtracing_amd import _lib as L
 
 import oracle_api as O
@@ -38,3 +40,21 @@ def test_host_only_entry_points_run_without_gpu():
 
 def test_oracle_loads():
     assert O.olib() is not None
+
+
+def test_image_files_round_trip(tmp_path):
+    from wavefrontpathtracing_amd import renderer as R
+    rng = np.random.default_rng(2)
+    img = rng.random((5, 7, 4)).astype(np.float32)
+    R.write_pfm(tmp_path / "a.pfm", img)
+    raw = (tmp_path / "a.pfm").read_bytes()
+    head = b"PF\n7 5\n-1.0\n"
+    assert raw.startswith(head)
+    data = np.frombuffer(raw[len(head):], dtype="<f4").reshape(5, 7, 3)
+    assert np.array_equal(data[::-1], img[..., :3])          # bottom row first
+    img8 = rng.integers(0, 256, (3, 4, 4), dtype=np.uint8)
+    R.write_ppm(tmp_path / "b.ppm", img8)
+    raw = (tmp_path / "b.ppm").read_bytes()
+    head = b"P6\n4 3\n255\n"
+    assert raw.startswith(head)
+    assert np.array_equal(np.frombuffer(raw[len(head):], dtype=np.uint8).reshape(3, 4, 3), img8[..., :3])

@@ -141,6 +141,8 @@ SIGNATURES = {
     "wf_renderer_read_accumulator": (_st, [_vp, _vp]),
     "wf_renderer_read_final_color": (_st, [_vp, _vp]),
     "wf_renderer_get_frame_stats": (_st, [_vp, C.POINTER(FrameStats)]),
+    "wf_write_pfm": (_st, [C.c_char_p, _vp, _u32, _u32]),
+    "wf_write_ppm": (_st, [C.c_char_p, _vp, _u32, _u32]),
     "wf_renderer_build_tlas": (_st, [_vp]),
     "wf_renderer_get_scene_bounds": (_st, [_vp, _vp, _vp]),
     "wf_trace_closest": (_st, [_vp, _vp, _u32, _vp]),

@@ -344,3 +344,17 @@ def parse_tlas_blob(blob: bytes):
     nodes = np.frombuffer(blob, dtype=L.TLAS_NODE, count=n_nodes, offset=int(h[0]))
     inst = np.frombuffer(blob, dtype=L.BVH_INSTANCE, count=n_inst, offset=int(h[1]))
     return nodes, inst
+
+
+def write_pfm(path: str, rgba: np.ndarray) -> None:
+    """Linear (h, w, 4) float image -> PFM (RGB, little endian, bottom row first)."""
+    a = np.ascontiguousarray(rgba, dtype=np.float32)
+    h, w = a.shape[:2]
+    check(lib().wf_write_pfm(str(path).encode(), ptr(a), w, h), what="write_pfm")
+
+
+def write_ppm(path: str, rgba8: np.ndarray) -> None:
+    """(h, w, 4) uint8 image -> binary PPM (RGB)."""
+    a = np.ascontiguousarray(rgba8, dtype=np.uint8)
+    h, w = a.shape[:2]
+    check(lib().wf_write_ppm(str(path).encode(), ptr(a), w, h), what="write_ppm")
