@@ -182,6 +182,25 @@ def bench_frame(frames: int):
     return out
 
 
+def bench_inverse(steps: int):
+    """Batched inverse rendering: 64 orbit views of the 256-instance scene at
+    512x512, 8 spp, depth 5, NEE, a 1024x1024 albedo texture and the 4 light
+    strengths as unknowns; each step = forward of all views in one batch +
+    path-replay adjoint + Adam update."""
+    from wavefrontpathtracing_amd import inverse as INV
+    t0 = time.perf_counter()
+    hist = INV.run(steps=steps + 1, views=64, res=512, spp=8, max_bounces=5, tex_size=1024, lr=1e-2, seed=7, log=None)
+    total = time.perf_counter() - t0
+    timed = hist[1:]
+    s = float(np.mean([h["s"] for h in timed]))
+    return {"workload": "64 views x 512x512, 8 spp, depth 5, NEE, 256-instance scene, 1024x1024 texture + 4 emission "
+                        "scalars, forward + adjoint + Adam per step",
+            "ms_per_step": round(s * 1e3, 1), "steps_per_s": round(1.0 / s, 3),
+            "forward_rays_per_step": timed[-1]["forward_rays"],
+            "loss": [round(h["loss"], 6) for h in hist], "texel_mae": [round(h["texel_mae"], 5) for h in hist],
+            "steps_timed": steps, "wall_s_with_setup": round(total, 1)}
+
+
 def bench_batched(frames: int, tune: bool):
     """64 uneven scenes (soup of 10K..2M triangles in a lit box room, 256^2 /
     512^2 / 1024^2 pixels), 4 spp, depth 4, next-event estimation: the whole
@@ -317,6 +336,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-soup", action="store_true")
     ap.add_argument("--no-frame", action="store_true")
+    ap.add_argument("--no-inverse", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune the traversal kernel on the batched frame")
     ap.add_argument("--sweep", action="store_true",
                     help="autotune the traversal kernel on the incoherent workload first (table on stderr)")
@@ -449,6 +469,8 @@ def main():
         out["batched"] = bench_batched(frames=2, tune=args.tune)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(frames=3)
+    if not args.no_inverse:
+        out["inverse"] = bench_inverse(steps=3)
     print(json.dumps(out), flush=True)
 
 
