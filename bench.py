@@ -414,6 +414,13 @@ def main():
 
     hits = d_hits.cpu().numpy().view(L.HIT)
     hit_frac = float((hits["instance_idx"] != L.MISS).mean())
+    # the same rays through the host-array entry point (PCIe copies included)
+    r.trace_closest(rays)
+    t_host = time.perf_counter()
+    for _ in range(3):
+        host_hits = r.trace_closest(rays)
+    host_mrays = 3 * len(rays) / (time.perf_counter() - t_host) / 1e6
+    assert host_hits.tobytes() == hits.tobytes()
     _, counters = r.trace_closest_counters(rays)
     b_ray = bytes_per_ray(counters)
     achieved_gbs = mrays * 1e6 * b_ray / 1e9
@@ -441,6 +448,7 @@ def main():
             "hit_fraction": round(hit_frac, 4),
             "traversal": cfg,
             "scene_build_s": round(t_build, 2),
+            "host_arrays_mrays": round(host_mrays, 2),
             "wall_s_timed": round(t_wall, 4),
         },
         "roofline": {
