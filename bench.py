@@ -390,11 +390,11 @@ def main():
             ev_b.synchronize()
             return ev_a.elapsed_time(ev_b) * 1e-3
 
-        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, refill_min=f, tlas_in_lds=g, interior_visits=v)
-                for b in (64, 128, 256) for l in (8, 16) for f in (12, 16) for g in (0, 1) for v in (2, 4)]
+        cfgs = [dict(block=b, lds_stack=l, skip_popped=s, refill_min=16, tlas_in_lds=0, interior_visits=v)
+                for b in (64, 128, 256) for l in (4, 8, 12, 16) for s in (1,) for v in (4, 5)]
         best, table = tuning.autotune(r, timed, configs=cfgs, repeats=3)
         for c, t in table:
-            print(f"sweep block={c['block']} lds={c['lds_stack']} refill={c['refill_min']} tlas_lds={c['tlas_in_lds']} "
+            print(f"sweep block={c['block']} lds={c['lds_stack']} skip={c['skip_popped']} "
                   f"visits={c['interior_visits']}: {len(rays) / t / 1e6:.1f} Mrays/s", file=sys.stderr)
 
     for _ in range(args.warmup):

@@ -200,7 +200,7 @@ WF_API wf_status wf_renderer_set_build_options(wf_renderer* r, const wf_build_op
 
 /* Traversal kernel configuration (no reference counterpart: the autotuner's
  * axes).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
- * scratch), 4, 8 or 16 (LDS, spilling to global memory past that depth);
+ * scratch), 4, 8, 12 or 16 (LDS, spilling to global memory past that depth);
  * skip_popped_interior drops popped interior nodes whose entry distance is
  * not below the current hit distance; bvh_layout 0 = BVH2; stack_limit caps
  * the traversal stack below its 128 entries per lane (0 = no cap; a ray that

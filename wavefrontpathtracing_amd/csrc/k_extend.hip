@@ -98,26 +98,46 @@ __device__ __forceinline__ bool tri_test(const float4& a, const float4& b, const
 //   leaf child:      0x80000000 | kind << 27 | ref   (kind 1..15, ref < 2^27; TLAS leaves are kind 1)
 //                    0x80000000 | node index          (BLAS leaves with more triangles or a far
 //                                                      first triangle; the node is re-read)
+//
+// In LDS an entry takes 6 bytes: the reference plus its entry distance as a
+// half-precision value rounded toward -inf.  The distance only decides the
+// popped-interior skip (skip when distance >= t); a rounded-down value never
+// skips a node the exact value would keep, and a node it keeps anyway is
+// visited, its children both miss, and the traversal pops on exactly as the
+// reference does.  Spilled entries keep the full float.
+__device__ __forceinline__ uint16_t half_down(float d)
+{
+    const _Float16 h = (_Float16)d;                        // round to nearest even
+    uint16_t b = __builtin_bit_cast(uint16_t, h);
+    if ((float)h > d) b = (b & 0x8000u) ? (uint16_t)(b + 1u) : (b == 0u ? (uint16_t)0x8001u : (uint16_t)(b - 1u));
+    return b;
+}
+__device__ __forceinline__ float half_up_to_float(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+
+// Entries past the LDS depth spill to global memory, level-major: level k of
+// thread g sits at spill[k * spill_stride + g], so lanes of a wave at the same
+// depth touch adjacent 8-byte words.
 template <int BLOCK, int LDS>
 struct LdsStack {
     uint32_t* ent;
-    float* dist;
-    uint2* spill;
-    uint32_t spill_n;
+    uint16_t* dist;
+    uint2* spill;            // this thread's level-0 spill slot
+    uint32_t spill_n;        // spill levels
+    uint32_t spill_stride;   // threads of the grid
     uint32_t cap;
     __device__ __forceinline__ bool push(uint32_t sp, uint32_t e, float d)
     {
         if (sp >= cap) return false;
-        if (sp < (uint32_t)LDS) { ent[sp * BLOCK] = e; dist[sp * BLOCK] = d; return true; }
+        if (sp < (uint32_t)LDS) { ent[sp * BLOCK] = e; dist[sp * BLOCK] = half_down(d); return true; }
         const uint32_t k = sp - LDS;
         if (k >= spill_n) return false;
-        spill[k] = make_uint2(e, __float_as_uint(d));
+        spill[(size_t)k * spill_stride] = make_uint2(e, __float_as_uint(d));
         return true;
     }
     __device__ __forceinline__ void get(uint32_t sp, uint32_t& e, float& d) const
     {
-        if (sp < (uint32_t)LDS) { e = ent[sp * BLOCK]; d = dist[sp * BLOCK]; return; }
-        const uint2 v = spill[sp - LDS];
+        if (sp < (uint32_t)LDS) { e = ent[sp * BLOCK]; d = half_up_to_float(dist[sp * BLOCK]); return; }
+        const uint2 v = spill[(size_t)(sp - LDS) * spill_stride];
         e = v.x; d = __uint_as_float(v.y);
     }
 };
@@ -365,7 +385,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
 {
     constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
     __shared__ uint32_t s_ent[kLdsN];
-    __shared__ float s_dist[kLdsN];
+    __shared__ uint16_t s_dist[kLdsN];
     float4* s_tl = s_tlas_dyn;
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
@@ -376,7 +396,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
         __syncthreads();
     }
     if constexpr (LDS > 0) {
-        LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + (size_t)gtid * L.spill_per_thread, L.spill_per_thread,
+        LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + gtid, L.spill_per_thread, gridDim.x * BLOCK,
                                 L.stack_cap};
         extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS>(L, st, s_tl, tid & 63u, n);
     } else {
@@ -401,9 +421,9 @@ void* pick_lds(uint32_t block, uint32_t lds, bool skip)
 #define WF_CASE(B, S) \
     if (block == B && lds == S) \
         return skip ? kernel_ptr<B, S, true, COUNT, ANY, TLDS>() : kernel_ptr<B, S, false, COUNT, ANY, TLDS>();
-    WF_CASE(64, 0) WF_CASE(64, 4) WF_CASE(64, 8) WF_CASE(64, 16)
-    WF_CASE(128, 0) WF_CASE(128, 4) WF_CASE(128, 8) WF_CASE(128, 16)
-    WF_CASE(256, 0) WF_CASE(256, 4) WF_CASE(256, 8) WF_CASE(256, 16)
+    WF_CASE(64, 0) WF_CASE(64, 4) WF_CASE(64, 8) WF_CASE(64, 12) WF_CASE(64, 16)
+    WF_CASE(128, 0) WF_CASE(128, 4) WF_CASE(128, 8) WF_CASE(128, 12) WF_CASE(128, 16)
+    WF_CASE(256, 0) WF_CASE(256, 4) WF_CASE(256, 8) WF_CASE(256, 12) WF_CASE(256, 16)
 #undef WF_CASE
     return nullptr;
 }

@@ -222,8 +222,9 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     if (!r || !c) return WF_ERROR_INVALID_ARGUMENT;
     if (c->workgroup_size != 64 && c->workgroup_size != 128 && c->workgroup_size != 256)
         return r->fail(WF_ERROR_INVALID_ARGUMENT, "workgroup_size must be 64, 128 or 256");
-    if (c->lds_stack_entries != 0 && c->lds_stack_entries != 4 && c->lds_stack_entries != 8 && c->lds_stack_entries != 16)
-        return r->fail(WF_ERROR_INVALID_ARGUMENT, "lds_stack_entries must be 0, 4, 8 or 16");
+    if (c->lds_stack_entries != 0 && c->lds_stack_entries != 4 && c->lds_stack_entries != 8 && c->lds_stack_entries != 12 &&
+        c->lds_stack_entries != 16)
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "lds_stack_entries must be 0, 4, 8, 12 or 16");
     if (c->bvh_layout != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "bvh_layout must be 0 (BVH2)");
     if (c->refill_min > 64) return r->fail(WF_ERROR_INVALID_ARGUMENT, "refill_min must be in [0, 64]");
     if (c->interior_visits > 8) return r->fail(WF_ERROR_INVALID_ARGUMENT, "interior_visits must be in [0, 8]");

@@ -1,6 +1,6 @@
 """Traversal kernel configuration and the autotuner over it.
 
-Axes: workgroup size {64, 128, 256} x stack placement {LDS 4/8/16 entries
+Axes: workgroup size {64, 128, 256} x stack placement {LDS 4/8/12/16 entries
 per lane with global spill, private (scratch) stack} x popped-interior skip
 x refill threshold (finished lanes of a wave before it claims new rays).
 """
@@ -32,7 +32,7 @@ def get_extend_config(renderer) -> dict:
 
 
 def candidates():
-    for block, lds, skip in itertools.product((64, 128, 256), (0, 4, 8, 16), (1, 0)):
+    for block, lds, skip in itertools.product((64, 128, 256), (0, 4, 8, 12, 16), (1, 0)):
         yield dict(block=block, lds_stack=lds, skip_popped=skip)
 
 
