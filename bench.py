@@ -174,11 +174,18 @@ def bench_frame(frames: int):
     for _ in range(frames):
         r.render()
         gt += r.frame_stats().ms_total
+    # the same frame as one megakernel launch (use_wavefront_pathtracing = 0)
+    r.update_settings(use_hip_graph=0, use_wavefront_pathtracing=0)
+    r.render()
+    mt = 0.0
+    for _ in range(frames):
+        r.render()
+        mt += r.frame_stats().ms_total
     r.exit()
     out = {k: round(v / frames, 3) for k, v in acc.items()}
     out.update({"workload": "256-instance scene, 1920x1080, 1 spp, depth 5, RR from 3, NEE, black environment",
                 "rays_per_frame": rays // frames, "mrays_per_s": round(rays / (acc["ms_total"] * 1e-3) / 1e6, 2),
-                "graph_ms_total": round(gt / frames, 3)})
+                "graph_ms_total": round(gt / frames, 3), "megakernel_ms_total": round(mt / frames, 3)})
     return out
 
 
@@ -262,11 +269,20 @@ def bench_batched(frames: int, tune: bool):
                 st = r.frame_stats()
                 rrays += st.rays_extended + st.shadow_rays
     r.exit()
+    # the batch as one megakernel launch per frame
+    r.update_settings(spp=4, accumulate=0, frame_seed=9, use_wavefront_pathtracing=0)
+    batched_frame()
+    mt, mrays = 0.0, 0
+    for _ in range(frames):
+        dt, nr = batched_frame()
+        mt += dt
+        mrays += nr
     b_mr, p_mr, r_mr = brays / bt / 1e6, prays / pt / 1e6, rrays / rt / 1e6
     return {"workload": f"{n} scenes, {sum(len(i) // 3 for s in sl for _, i in s.meshes[:1]):,} soup tris, "
                         f"{sum(s.width * s.height for s in sl):,} pixels, 4 spp, depth 4, NEE",
             "batched_mrays": round(b_mr, 2), "plain_mrays": round(p_mr, 2), "ratio": round(b_mr / p_mr, 3),
             "reference_loop_mrays": round(r_mr, 2), "ratio_vs_reference_loop": round(b_mr / r_mr, 3),
+            "megakernel_batched_mrays": round(mrays / mt / 1e6, 2),
             "reference_loop_ms_per_batch": round(rt / frames * 1e3, 2),
             "rays_per_frame": brays // frames, "batched_ms_per_frame": round(bt / frames * 1e3, 2),
             "plain_ms_per_frame": round(pt / frames * 1e3, 2), "setup_s": round(setup_s, 1), "frames": frames,

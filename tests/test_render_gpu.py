@@ -113,3 +113,44 @@ def test_gltf_scene_matches_oracle(tmp_path):
     ref = O.OracleScene(sc).render(max_bounces=4, spp=2, nee=1, env=0.1, frame_seed=6, nthreads=8)
     assert np.abs(gpu[..., :3]).sum() > 0
     assert rel_l1(gpu, ref) <= REL_L1_TOL
+
+
+@pytest.mark.parametrize("nee,env", [(0, 1.0), (1, 0.3)])
+def test_megakernel_equals_wavefront(nee, env):
+    """The megakernel frame (use_wavefront_pathtracing = 0) runs the same
+    per-path operations in the same order as the wavefront frame: identical
+    images and ray counts."""
+    sc = scenes.small_tlas_scene()
+    sc.eye = (0.0, 9.0, -14.0)
+    out = []
+    for wavefront in (1, 0):
+        r = Renderer(sc.width, sc.height)
+        try:
+            scenes.load(r, sc)
+            r.update_settings(max_bounces=5, spp=2, next_event_estimation=nee, hdr_env_strength=env, frame_seed=5,
+                              accumulate=0, use_wavefront_pathtracing=wavefront)
+            r.render()
+            st = r.frame_stats()
+            out.append((r.read_energy(), st.rays_extended, st.shadow_rays))
+        finally:
+            r.exit()
+    (wf_img, wf_rays, wf_sh), (mk_img, mk_rays, mk_sh) = out
+    assert np.array_equal(wf_img, mk_img), np.argwhere(np.any(wf_img != mk_img, axis=-1))[:4]
+    assert (wf_rays, wf_sh) == (mk_rays, mk_sh)
+
+
+def test_megakernel_batch_equals_wavefront_batch():
+    sl = scenes.batch_scenes(seed=31, n_scenes=5, tri_min=500, tri_max=20_000, resolutions=(24, 40, 56))
+    imgs = []
+    for wavefront in (1, 0):
+        r = Renderer(64, 64)
+        try:
+            scenes.load_batch(r, sl)
+            r.update_settings(max_bounces=4, spp=2, next_event_estimation=1, hdr_env_strength=0.0, frame_seed=5,
+                              use_wavefront_pathtracing=wavefront)
+            r.render()
+            imgs.append([r.read_scene_energy(k, sc.width, sc.height) for k, sc in enumerate(sl)])
+        finally:
+            r.exit()
+    for k in range(len(sl)):
+        assert np.array_equal(imgs[0][k], imgs[1][k]), f"scene {k}"

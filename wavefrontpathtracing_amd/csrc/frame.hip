@@ -2,6 +2,7 @@
 // and emitter tables the shade and connect stages read, and the frame loop
 // (renderer.cpp:742-939) over one or many scene slots.
 #include "renderer_impl.h"
+#include "mega.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -252,6 +253,27 @@ wf_status wf_renderer::record_frame(const FrameArgs& A, uint32_t grid, bool time
     wf_status s;
     if ((s = reset_overflows()) != WF_OK) return s;
     if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+    if (!settings.use_wavefront_pathtracing) {
+        // megakernel frame: every path from camera to termination in one launch
+        MegaLaunch M{};
+        M.scene.tnodes = (const wf_tlas_node*)d_tnodes.p;
+        M.scene.tpairs = (const float4*)d_tpairs.p;
+        M.scene.n_tpairs = n_tpairs;
+        M.scene.inst = (const GpuInstance*)d_inst.p;
+        M.spill = (uint2*)d_spill.p;
+        M.spill_per_thread = spill_per_thread;
+        M.stack_cap = ext_cfg.stack_limit ? std::min(ext_cfg.stack_limit, kStackTotal) : kStackTotal;
+        M.work = (uint32_t*)d_work.p;
+        M.visits = ext_cfg.visits;
+        M.refill_min = ext_cfg.refill_min;
+        M.grid = mega_grid;
+        if ((e = launch_megakernel(A, M, ext_cfg, stream)) != hipSuccess) return hip_fail(e, "megakernel launch");
+        if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+        if ((e = launch_resolve_post(A, grid, stream)) != hipSuccess) return hip_fail(e, "post-process launch");
+        if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+        *n_events = ei;
+        return WF_OK;
+    }
     if ((e = launch_clear(A, grid, stream)) != hipSuccess) return hip_fail(e, "clear launch");
     if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
     if ((e = launch_generate(A, grid, stream)) != hipSuccess) return hip_fail(e, "generate launch");
@@ -383,7 +405,7 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
         // buffers, scene range, settings and traversal kernel stay the same
         auto bits = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return (uint64_t)u; };
         const std::vector<uint64_t> key = {
-            first, count, settings.max_bounces, settings.spp, settings.next_event_estimation, settings.rr_start_bounce,
+            first, count, settings.use_wavefront_pathtracing, mega_grid, settings.max_bounces, settings.spp, settings.next_event_estimation, settings.rr_start_bounce,
             bits(settings.rr_survival), bits(settings.hdr_env_strength),
             settings.accumulate, ext_cfg.block, ext_cfg.lds_stack, ext_cfg.skip_popped, ext_grid,
             (uint64_t)(uintptr_t)d_tnodes.p, (uint64_t)(uintptr_t)d_tpairs.p, ext_cfg.refill_min, ext_cfg.tlas_lds, ext_cfg.visits, ext_grid_tlds, n_tpairs, (uint64_t)(uintptr_t)d_inst.p, (uint64_t)(uintptr_t)d_shade_inst.p,
@@ -415,7 +437,9 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "frame");
     auto ms = [&](int a, int b) { float t = 0.0f; (void)hipEventElapsedTime(&t, ev[a], ev[b]); return t; };
-    if (!settings.use_hip_graph) {
+    if (!settings.use_hip_graph && !settings.use_wavefront_pathtracing) {
+        st.ms_post = ms(1, 2);   // the megakernel frame has no separate stages
+    } else if (!settings.use_hip_graph) {
         st.ms_clear = ms(0, 1);
         st.ms_generate = ms(1, 2);
         for (uint32_t d = 0; d <= settings.max_bounces; ++d) {

@@ -12,7 +12,7 @@
 // Per-path state (throughput, energy) is FP32 and indexed by path id, so a
 // path's arithmetic does not depend on where the compaction placed it; the
 // random numbers are PCG32 keyed on (path, bounce).
-#include "stages.h"
+#include "shading.h"
 
 #include <cfloat>
 
@@ -21,190 +21,9 @@
 namespace wf {
 namespace {
 
-constexpr float kInvPi = 0.31830988f;   // common.hlsl:23
-constexpr float kRayMinT = 1e-8f;
-
-// ---- RNG: PCG32 (O'Neill 2014) seeded per (path, bounce) ------------------
-struct Pcg { uint64_t state, inc; };
-__device__ __forceinline__ uint32_t pcg_next(Pcg& r)
-{
-    const uint64_t old = r.state;
-    r.state = old * 6364136223846793005ull + r.inc;
-    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
-    const uint32_t rot = (uint32_t)(old >> 59u);
-    return (xs >> rot) | (xs << ((32u - rot) & 31u));
-}
-__device__ __forceinline__ void path_rng(uint32_t path, uint32_t bounce, uint32_t seed, float u[6])
-{
-    Pcg r;
-    r.state = 0u;
-    r.inc = ((uint64_t)bounce << 1u) | 1u;
-    pcg_next(r);
-    r.state += ((uint64_t)seed << 32) | path;
-    pcg_next(r);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) u[k] = (float)(pcg_next(r) >> 8) * (1.0f / 16777216.0f);
-}
-
-// ---- sin/cos of 2*pi*u (quadrant + Taylor, the same operation sequence as
-// the host restatement) ----------------------------------------------------------
-__device__ __forceinline__ void sincos_turns(float u, float& s, float& c)
-{
-    const float q4 = u * 4.0f;
-    const float qf = floorf(q4);
-    const float f = q4 - qf;
-    const int q = ((int)qf) & 3;
-    const float x = f * 1.57079632679489662f;
-    const float y = x * x;
-    float p = (float)(1.0 / 6227020800.0);
-    p = p * y - (float)(1.0 / 39916800.0);
-    p = p * y + (float)(1.0 / 362880.0);
-    p = p * y - (float)(1.0 / 5040.0);
-    p = p * y + (float)(1.0 / 120.0);
-    p = p * y - (float)(1.0 / 6.0);
-    p = p * y + 1.0f;
-    const float sx = x * p;
-    float r = -(float)(1.0 / 87178291200.0);
-    r = r * y + (float)(1.0 / 479001600.0);
-    r = r * y - (float)(1.0 / 3628800.0);
-    r = r * y + (float)(1.0 / 40320.0);
-    r = r * y - (float)(1.0 / 720.0);
-    r = r * y + (float)(1.0 / 24.0);
-    r = r * y - 0.5f;
-    r = r * y + 1.0f;
-    const float cx = r;
-    switch (q) {
-    case 0: s = sx; c = cx; break;
-    case 1: s = cx; c = -sx; break;
-    case 2: s = -sx; c = -cx; break;
-    default: s = -cx; c = sx; break;
-    }
-}
-
-struct V3 { float x, y, z; };
-__device__ __forceinline__ float dot3(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ V3 normalize3(V3 v)
-{
-    const float s = 1.0f / sqrtf(dot3(v, v));
-    return V3{v.x * s, v.y * s, v.z * s};
-}
-__device__ __forceinline__ V3 cross3(V3 a, V3 b)
-{
-    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-// glm mat4 * vec4 association: (c0*x + c1*y) + (c2*z + c3*w), xyz only
-__device__ __forceinline__ V3 xform(const float* m, V3 v, float w)
-{
-    V3 o;
-    o.x = (m[0] * v.x + m[4] * v.y) + (m[8] * v.z + m[12] * w);
-    o.y = (m[1] * v.x + m[5] * v.y) + (m[9] * v.z + m[13] * w);
-    o.z = (m[2] * v.x + m[6] * v.y) + (m[10] * v.z + m[14] * w);
-    return o;
-}
-__device__ __forceinline__ float interp(float a, float b, float c, float bx, float by) { return (a + bx * (b - a)) + by * (c - a); }
-
-// cosine_weighted_hemisphere_sample + create_orthonormal_basis (sample.hlsl:13-27,45-54)
-__device__ __forceinline__ V3 cosine_sample(V3 n, float r0, float r1)
-{
-    const float sin_theta = sqrtf(r0);
-    float sp, cp;
-    sincos_turns(r1, sp, cp);
-    const V3 v = normalize3(V3{sin_theta * cp, sqrtf(1.0f - r0), sin_theta * sp});
-    const V3 t0 = (fabsf(n.x) > fabsf(n.z)) ? V3{-n.y, n.x, 0.0f} : V3{0.0f, -n.z, n.y};
-    const V3 t = normalize3(t0);
-    const V3 b = cross3(n, t);
-    return V3{(v.x * t.x + v.y * n.x) + v.z * b.x, (v.x * t.y + v.y * n.y) + v.z * b.y, (v.x * t.z + v.y * n.z) + v.z * b.z};
-}
-
-__device__ __forceinline__ V3 tex_sample(const GpuTexture& tx, float u, float v)
-{
-    float fx = u * (float)tx.width - 0.5f, fy = v * (float)tx.height - 0.5f;
-    fx = fminf(fmaxf(fx, -1e9f), 1e9f);
-    fy = fminf(fmaxf(fy, -1e9f), 1e9f);
-    const float fx0 = floorf(fx), fy0 = floorf(fy);
-    const float ax = fx - fx0, ay = fy - fy0;
-    const int64_t ix = (int64_t)fx0, iy = (int64_t)fy0;
-    const int64_t w = tx.width, h = tx.height;
-    const uint32_t x0 = (uint32_t)(((ix % w) + w) % w), y0 = (uint32_t)(((iy % h) + h) % h);
-    const uint32_t x1 = (x0 + 1u == (uint32_t)w) ? 0u : x0 + 1u, y1 = (y0 + 1u == (uint32_t)h) ? 0u : y0 + 1u;
-    const float4* t = reinterpret_cast<const float4*>(tx.texels);
-    const float4 a = t[(size_t)y0 * w + x0], b = t[(size_t)y0 * w + x1];
-    const float4 c = t[(size_t)y1 * w + x0], d = t[(size_t)y1 * w + x1];
-    V3 o;
-    float top, bot;
-    top = a.x + ax * (b.x - a.x); bot = c.x + ax * (d.x - c.x); o.x = top + ay * (bot - top);
-    top = a.y + ax * (b.y - a.y); bot = c.y + ax * (d.y - c.y); o.y = top + ay * (bot - top);
-    top = a.z + ax * (b.z - a.z); bot = c.z + ax * (d.z - c.z); o.z = top + ay * (bot - top);
-    return o;
-}
-
-// one atomic per wave64 for all lanes appending to the same queue
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred)
-{
-    const uint64_t mask = __ballot(pred);
-    uint32_t slot = 0;
-    if (pred) {
-        const uint32_t lane = __lane_id();
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-        const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-        base = __shfl(base, (int)leader, 64);
-        slot = base + rank;
-    }
-    return slot;
-}
-
-// atomicAdd(&arr[key], v) for every lane with `pred`, combined per distinct
-// key: the wave sums the values of the lanes that share the first pending
-// lane's key and issues one atomic for them (the emission gradients of a
-// frame all land on a handful of emitter instances).  Called by all lanes.
-__device__ __forceinline__ void wave_add_keyed(float* arr, uint32_t key, float v, bool pred)
-{
-    const uint32_t lane = __lane_id();
-    uint64_t pending = __ballot(pred);
-    while (pending) {
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
-        const uint32_t k = __shfl(key, (int)leader, 64);
-        const bool mine = pred && key == k;
-        const uint64_t sel = __ballot(mine);
-        float sum = mine ? v : 0.0f;
-        for (int off = 32; off > 0; off >>= 1) sum = sum + __shfl_xor(sum, off, 64);
-        if (lane == leader) atomicAdd(arr + k, sum);
-        pending &= ~sel;
-        pred = pred && !mine;
-    }
-}
-
-__device__ __forceinline__ void store_ray(wf_ray* q, uint32_t slot, V3 o, V3 d, float tmin, float tmax)
-{
-    float4* p = reinterpret_cast<float4*>(q + slot);
-    p[0] = make_float4(o.x, o.y, o.z, tmin);
-    p[1] = make_float4(d.x, d.y, d.z, tmax);
-}
+using namespace shading;
 
 // ---- kernels ---------------------------------------------------------------------
-
-// scene slot holding global path id p (slots are sorted by path_off)
-__device__ __forceinline__ uint32_t scene_of_path(const FrameArgs& A, uint32_t p)
-{
-    uint32_t lo = A.first_scene, hi = A.first_scene + A.n_scenes - 1u;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1u) >> 1;
-        if (A.scenes[mid].path_off <= p) lo = mid; else hi = mid - 1u;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t scene_of_pixel(const FrameArgs& A, uint32_t q)
-{
-    uint32_t lo = A.first_scene, hi = A.first_scene + A.n_scenes - 1u;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1u) >> 1;
-        if (A.scenes[mid].pix_off <= q) lo = mid; else hi = mid - 1u;
-    }
-    return lo;
-}
 
 // clear_buffers.hlsl:21-39: throughput 1, energy 0, queue 0 holds every path
 __global__ void k_clear(FrameArgs A)
@@ -228,18 +47,8 @@ __global__ void k_generate(FrameArgs A)
         const uint32_t p = A.path_base + i;
         const uint32_t s = scene_of_path(A, p);
         const GpuScene& sc = A.scenes[s];
-        const uint32_t pixel = (p - sc.path_off) % sc.n_pix;
-        const uint32_t px = pixel % sc.width, py = pixel / sc.width;
-        const float* c2v = sc.view.clip_to_view;
-        const float* v2w = sc.view.view_to_world;
-        const float uvx = ((float)px + 0.5f) / sc.view.render_dim[0];
-        float uvy = ((float)py + 0.5f) / sc.view.render_dim[1];
-        uvy = 1.0f - uvy;
-        const V3 cv = xform(c2v, V3{2.0f * uvx - 1.0f, 2.0f * uvy - 1.0f, 1.0f}, 1.0f);
-        const V3 dv = normalize3(cv);
-        const V3 dw = xform(v2w, dv, 0.0f);
-        const V3 ow = xform(v2w, V3{0.0f, 0.0f, 0.0f}, 1.0f);
-        const V3 o = V3{ow.x + dw.x * kRayMinT, ow.y + dw.y * kRayMinT, ow.z + dw.z * kRayMinT};
+        V3 o, dw;
+        camera_ray(sc, (p - sc.path_off) % sc.n_pix, o, dw);
         store_ray(A.rays[0], i, o, dw, kRayMinT, FLT_MAX);
         A.path_of[0][i] = p;
         A.root_of[0][i] = sc.tlas_root;
@@ -247,34 +56,9 @@ __global__ void k_generate(FrameArgs A)
     }
 }
 
-// bilinear corners and fractions of tex_sample, for scattering gradients
-__device__ __forceinline__ void tex_corners(const GpuTexture& tx, float u, float v, uint32_t c[4], float& ax, float& ay)
-{
-    float fx = u * (float)tx.width - 0.5f, fy = v * (float)tx.height - 0.5f;
-    fx = fminf(fmaxf(fx, -1e9f), 1e9f);
-    fy = fminf(fmaxf(fy, -1e9f), 1e9f);
-    const float fx0 = floorf(fx), fy0 = floorf(fy);
-    ax = fx - fx0;
-    ay = fy - fy0;
-    const int64_t ix = (int64_t)fx0, iy = (int64_t)fy0;
-    const int64_t w = tx.width, h = tx.height;
-    const uint32_t x0 = (uint32_t)(((ix % w) + w) % w), y0 = (uint32_t)(((iy % h) + h) % h);
-    const uint32_t x1 = (x0 + 1u == (uint32_t)w) ? 0u : x0 + 1u, y1 = (y0 + 1u == (uint32_t)h) ? 0u : y0 + 1u;
-    c[0] = y0 * (uint32_t)w + x0; c[1] = y0 * (uint32_t)w + x1;
-    c[2] = y1 * (uint32_t)w + x0; c[3] = y1 * (uint32_t)w + x1;
-}
-
-// shade.hlsl:42-174 with the build's RNG, Russian roulette and next-event
-// estimation (the host restatement has the same operation order).
-//
-// ADJ = true is the path-replay adjoint (Vicini, Speierer, Jakob 2021): the
-// same path is replayed with the same random numbers; instead of adding
-// contributions to the path's energy it subtracts them from lrem (the
-// path's forward estimate minus what was collected before), and since every
-// contribution is linear in the albedo of each vertex it passes,
-// dEstimate/dalbedo_d = lrem / albedo_d at vertex d, scattered to the
-// texture's bilinear corners; emission gradients are the collected
-// contributions per unit emission strength.
+// Shade stage (shade.hlsl:42-174): one queue entry per thread, shade_vertex
+// for the path, then the continuation and shadow rays appended to the next
+// queues (ADJ: the path-replay adjoint of the same vertex).
 template <bool ADJ>
 __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
 {
@@ -302,151 +86,12 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
             float4 e4 = ADJ ? A.lrem[p] : A.energy[p];
             const float4 dlt = ADJ ? A.delta[p] : make_float4(0, 0, 0, 0);
             V3 thr{t4.x, t4.y, t4.z};
-            float u[6];
-            path_rng(p, d, seed, u);
-            bool done = false;
-            if (hit.instance_idx == 0xFFFFFFFFu) {
-                if (ADJ) {
-                    e4.x = e4.x - thr.x * A.env[0];
-                    e4.y = e4.y - thr.y * A.env[1];
-                    e4.z = e4.z - thr.z * A.env[2];
-                } else {
-                    e4.x = e4.x + thr.x * A.env[0];
-                    e4.y = e4.y + thr.y * A.env[1];
-                    e4.z = e4.z + thr.z * A.env[2];
-                }
-                done = true;
-            }
-            if (!done) {
-                const GpuShadeInstance* si = A.shade_inst + hit.instance_idx;
-                const float* l2w = si->local_to_world;
-                const GpuShadeTri* tri = si->tris + hit.primitive_idx;
-                const float bx = hit.bary[0], by = hit.bary[1];
-                const V3 lp{interp(tri->p[0][0], tri->p[1][0], tri->p[2][0], bx, by),
-                            interp(tri->p[0][1], tri->p[1][1], tri->p[2][1], bx, by),
-                            interp(tri->p[0][2], tri->p[1][2], tri->p[2][2], bx, by)};
-                const V3 ln{interp(tri->n[0][0], tri->n[1][0], tri->n[2][0], bx, by),
-                            interp(tri->n[0][1], tri->n[1][1], tri->n[2][1], bx, by),
-                            interp(tri->n[0][2], tri->n[1][2], tri->n[2][2], bx, by)};
-                const float tu = interp(tri->uv[0][0], tri->uv[1][0], tri->uv[2][0], bx, by);
-                const float tv = interp(tri->uv[0][1], tri->uv[1][1], tri->uv[2][1], bx, by);
-                const V3 P = xform(l2w, lp, 1.0f);
-                const V3 N = normalize3(xform(l2w, ln, 0.0f));
-                V3 albedo{si->base_color[0], si->base_color[1], si->base_color[2]};
-                if (si->base_color_tex >= 0) {
-                    const V3 tc = tex_sample(A.textures[si->base_color_tex], tu, tv);
-                    albedo = V3{albedo.x * tc.x, albedo.y * tc.y, albedo.z * tc.z};
-                }
-                const float es = si->emissive_strength;
-                const V3 ef{si->emissive_factor[0], si->emissive_factor[1], si->emissive_factor[2]};
-                const V3 em{ef.x * es, ef.y * es, ef.z * es};
-                if (em.x != 0.0f || em.y != 0.0f || em.z != 0.0f) {
-                    if (!A.nee || d == 0) {
-                        if (ADJ) {
-                            e4.x = e4.x - thr.x * em.x;
-                            e4.y = e4.y - thr.y * em.y;
-                            e4.z = e4.z - thr.z * em.z;
-                            em_g = (dlt.x * (thr.x * ef.x) + dlt.y * (thr.y * ef.y)) + dlt.z * (thr.z * ef.z);
-                            em_key = hit.instance_idx;
-                            em_has = em_g != 0.0f;
-                        } else {
-                            e4.x = e4.x + thr.x * em.x;
-                            e4.y = e4.y + thr.y * em.y;
-                            e4.z = e4.z + thr.z * em.z;
-                        }
-                    }
-                    done = true;
-                }
-                if (ADJ && !done && si->base_color_tex == A.grad_tex_slot && si->base_color_tex >= 0) {
-                    // d estimate / d texel: (lrem / albedo) * base_color * bilinear weight
-                    uint32_t c[4];
-                    float ax, ay;
-                    tex_corners(A.textures[si->base_color_tex], tu, tv, c, ax, ay);
-                    const float wts[4] = {(1.0f - ax) * (1.0f - ay), ax * (1.0f - ay), (1.0f - ax) * ay, ax * ay};
-                    const float g3[3] = {albedo.x > 0.0f ? dlt.x * e4.x / albedo.x * si->base_color[0] : 0.0f,
-                                         albedo.y > 0.0f ? dlt.y * e4.y / albedo.y * si->base_color[1] : 0.0f,
-                                         albedo.z > 0.0f ? dlt.z * e4.z / albedo.z * si->base_color[2] : 0.0f};
-                    if (g3[0] != 0.0f || g3[1] != 0.0f || g3[2] != 0.0f) {
-                        for (int j = 0; j < 4; ++j) {
-                            float* gt = reinterpret_cast<float*>(A.grad_tex + c[j]);
-                            atomicAdd(gt + 0, g3[0] * wts[j]);
-                            atomicAdd(gt + 1, g3[1] * wts[j]);
-                            atomicAdd(gt + 2, g3[2] * wts[j]);
-                        }
-                    }
-                }
-                if (!done && A.nee && d < A.max_bounces) {
-                    const GpuScene& sc = A.scenes[A.path_scene[p]];
-                    if (sc.em_count > 0) {
-                        uint32_t lo = 0, hi = sc.em_count - 1u;
-                        const uint4* em_tab = A.emitters + sc.em_off;
-                        while (lo < hi) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (__uint_as_float(em_tab[mid].z) > u[3]) hi = mid; else lo = mid + 1u;
-                        }
-                        const uint4 er = em_tab[lo];
-                        const GpuShadeInstance* li = A.shade_inst + er.x;
-                        const GpuShadeTri* lt = li->tris + er.y;
-                        const V3 W0 = xform(li->local_to_world, V3{lt->p[0][0], lt->p[0][1], lt->p[0][2]}, 1.0f);
-                        const V3 W1 = xform(li->local_to_world, V3{lt->p[1][0], lt->p[1][1], lt->p[1][2]}, 1.0f);
-                        const V3 W2 = xform(li->local_to_world, V3{lt->p[2][0], lt->p[2][1], lt->p[2][2]}, 1.0f);
-                        const float su = sqrtf(u[4]);
-                        const float bv = su * (1.0f - u[5]), bw = su * u[5];
-                        const V3 PL{interp(W0.x, W1.x, W2.x, bv, bw), interp(W0.y, W1.y, W2.y, bv, bw), interp(W0.z, W1.z, W2.z, bv, bw)};
-                        const V3 dl{PL.x - P.x, PL.y - P.y, PL.z - P.z};
-                        const V3 e1{W1.x - W0.x, W1.y - W0.y, W1.z - W0.z};
-                        const V3 e2{W2.x - W0.x, W2.y - W0.y, W2.z - W0.z};
-                        const float dist2 = dot3(dl, dl);
-                        const float dist = sqrtf(dist2);
-                        const float inv_d = 1.0f / dist;
-                        const V3 wi{dl.x * inv_d, dl.y * inv_d, dl.z * inv_d};
-                        const V3 nl = normalize3(cross3(e1, e2));
-                        const float cos_s = dot3(N, wi);
-                        const float cos_l = -dot3(nl, wi);
-                        if (cos_s > 0.0f && cos_l > 0.0f && dist2 > 0.0f) {
-                            const float G = (cos_s * cos_l) / dist2;
-                            const float wgt = G * sc.em_area;
-                            const float ls = li->emissive_strength;
-                            contrib.x = (((thr.x * albedo.x) * kInvPi) * (li->emissive_factor[0] * ls)) * wgt;
-                            contrib.y = (((thr.y * albedo.y) * kInvPi) * (li->emissive_factor[1] * ls)) * wgt;
-                            contrib.z = (((thr.z * albedo.z) * kInvPi) * (li->emissive_factor[2] * ls)) * wgt;
-                            if (ADJ) {
-                                unit.x = (((thr.x * albedo.x) * kInvPi) * li->emissive_factor[0]) * wgt;
-                                unit.y = (((thr.y * albedo.y) * kInvPi) * li->emissive_factor[1]) * wgt;
-                                unit.z = (((thr.z * albedo.z) * kInvPi) * li->emissive_factor[2]) * wgt;
-                                light = er.x;
-                            }
-                            sh_o = V3{P.x + wi.x * kRayMinT, P.y + wi.y * kRayMinT, P.z + wi.z * kRayMinT};
-                            sh_d = wi;
-                            sh_tmax = dist * 0.9999f;
-                            shadow = true;
-                        }
-                    }
-                }
-                if (!done) {
-                    const V3 L = cosine_sample(N, u[0], u[1]);
-                    float NoL = dot3(N, L);
-                    if (NoL < 0.0f) NoL = 0.0f;
-                    if (!(NoL > 0.0f)) {
-                        done = true;
-                    } else {
-                        const float pdf = NoL * kInvPi;
-                        const float inv_pdf = 1.0f / pdf;
-                        thr.x = thr.x * ((NoL * (albedo.x * kInvPi)) * inv_pdf);
-                        thr.y = thr.y * ((NoL * (albedo.y * kInvPi)) * inv_pdf);
-                        thr.z = thr.z * ((NoL * (albedo.z * kInvPi)) * inv_pdf);
-                        if (d >= A.rr_start) {
-                            if (u[2] >= A.rr_survival) done = true;
-                            else { thr.x = thr.x / A.rr_survival; thr.y = thr.y / A.rr_survival; thr.z = thr.z / A.rr_survival; }
-                        }
-                        if (!done && d < A.max_bounces) {
-                            spawn = true;
-                            new_o = V3{P.x + L.x * kRayMinT, P.y + L.y * kRayMinT, P.z + L.z * kRayMinT};
-                            new_d = L;
-                        }
-                    }
-                }
-            }
+            VertexOut o;
+            shade_vertex<ADJ>(A, p, A.path_scene[p], d, seed, hit, thr, e4, dlt, o);
+            spawn = o.spawn; shadow = o.shadow;
+            new_o = o.new_o; new_d = o.new_d; sh_o = o.sh_o; sh_d = o.sh_d; sh_tmax = o.sh_tmax;
+            contrib = o.contrib; unit = o.unit; light = o.light;
+            em_key = o.em_key; em_g = o.em_g; em_has = o.em_has;
             A.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
             if (ADJ) A.lrem[p] = e4;
             else A.energy[p] = e4;

@@ -5,6 +5,7 @@
 // memory in place of D3D12 resources and an explicit handle in place of the
 // g_renderer singleton.  Software ray tracing is the only traversal mode.
 #include "renderer_impl.h"
+#include "mega.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -116,8 +117,11 @@ wf_status wf_renderer::ensure_trace_scratch()
     ext_grid = (uint32_t)per_cu * (uint32_t)cu_count;
     wf_status s = update_tlds_grid();
     if (s != WF_OK) return s;
+    int per_cu_m = 0;
+    if ((e = mega_occupancy(ext_cfg, &per_cu_m)) != hipSuccess) return hip_fail(e, "occupancy query");
+    mega_grid = (uint32_t)std::max(per_cu_m, 1) * (uint32_t)cu_count;
     spill_per_thread = ext_cfg.lds_stack > 0 ? (kStackTotal > ext_cfg.lds_stack ? kStackTotal - ext_cfg.lds_stack : 0) : 0;
-    const size_t threads = (size_t)ext_grid * ext_cfg.block;
+    const size_t threads = (size_t)std::max(ext_grid, mega_grid) * ext_cfg.block;
     if ((e = d_spill.ensure(std::max<size_t>(threads * spill_per_thread * sizeof(uint2), 16))) != hipSuccess)
         return hip_fail(e, "spill stack allocation");
     if ((e = d_work.ensure(64)) != hipSuccess) return hip_fail(e, "work counter allocation");
@@ -207,7 +211,6 @@ extern "C" WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_re
     if (s->use_software_rt == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only software ray tracing is available");
     if (s->render_view_mode != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "debug view modes are not supported");
     if (s->cosine_weighted_diffuse == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only cosine-weighted diffuse sampling is built");
-    if (s->use_wavefront_pathtracing == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only the wavefront path tracer is built");
     const bool reset = s->max_bounces != r->settings.max_bounces || s->spp != r->settings.spp ||
                        s->next_event_estimation != r->settings.next_event_estimation;
     if (std::memcmp(s, &r->settings, sizeof(*s)) != 0) r->forward_valid = false;

@@ -103,7 +103,7 @@ struct wf_renderer {
 
     // traversal scratch
     wf::ExtendConfig ext_cfg;
-    uint32_t ext_grid = 0, ext_grid_tlds = 0;
+    uint32_t ext_grid = 0, ext_grid_tlds = 0, mega_grid = 0;
     uint32_t n_tpairs = 0;
     uint32_t spill_per_thread = 0;
     wf::DevBuf d_work, d_spill, d_tmp_rays, d_tmp_hits, d_tmp_aux;

@@ -1,0 +1,315 @@
+// Traversal primitives shared by the extend kernel and the megakernel:
+// the slab and ray/triangle tests (intersect.hlsl:7-76), the traversal
+// stacks, and one traversal pass of a lane over the TLAS -> BLAS hierarchy
+// (trace_ray_tlas / trace_ray_bvh_instance / trace_ray_bvh_local,
+// accelstruct.hlsl:58-218).  Device code only; FP contraction is off, so the
+// results are bit-identical to that algorithm in FP32.
+#pragma once
+
+#include "wf_internal.h"
+
+#include <cfloat>
+
+#pragma clang fp contract(off)
+
+namespace wf {
+namespace trav {
+
+constexpr float kRayMaxT = FLT_MAX;
+constexpr float kIntersectEps = 1e-8f;
+
+__device__ __forceinline__ float hmin(float a, float b) { return fminf(a, b); }   // IEEE minNum, as HLSL min
+__device__ __forceinline__ float hmax(float a, float b) { return fmaxf(a, b); }
+
+struct Ray3 { float ox, oy, oz, dx, dy, dz, ix, iy, iz; };
+
+__device__ __forceinline__ float slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
+                                      const Ray3& r, float t)
+{
+    const float tx1 = (mnx - r.ox) * r.ix;
+    const float tx2 = (mxx - r.ox) * r.ix;
+    float tmin = hmin(tx1, tx2);
+    float tmax = hmax(tx1, tx2);
+    const float ty1 = (mny - r.oy) * r.iy;
+    const float ty2 = (mxy - r.oy) * r.iy;
+    tmin = hmax(tmin, hmin(ty1, ty2));
+    tmax = hmin(tmax, hmax(ty1, ty2));
+    const float tz1 = (mnz - r.oz) * r.iz;
+    const float tz2 = (mxz - r.oz) * r.iz;
+    tmin = hmax(tmin, hmin(tz1, tz2));
+    tmax = hmin(tmax, hmax(tz1, tz2));
+    return (tmax >= tmin && tmin < t && tmax > 0.0f) ? tmin : kRayMaxT;
+}
+
+// Moller-Trumbore with back-face culling on precomputed edges.
+__device__ __forceinline__ bool tri_test(const float4& a, const float4& b, const float4& c, const Ray3& r,
+                                         float& t, float& bu, float& bv)
+{
+    const float e1x = b.x, e1y = b.y, e1z = b.z;
+    const float e2x = c.x, e2y = c.y, e2z = c.z;
+    const float px = r.dy * e2z - r.dz * e2y;
+    const float py = r.dz * e2x - r.dx * e2z;
+    const float pz = r.dx * e2y - r.dy * e2x;
+    const float det = (e1x * px + e1y * py) + e1z * pz;
+    if (det < kIntersectEps) return false;
+    const float inv = 1.0f / det;
+    const float sx = r.ox - a.x, sy = r.oy - a.y, sz = r.oz - a.z;
+    const float v = ((sx * px + sy * py) + sz * pz) * inv;
+    if (v < 0.0f || v > 1.0f) return false;
+    const float qx = sy * e1z - sz * e1y;
+    const float qy = sz * e1x - sx * e1z;
+    const float qz = sx * e1y - sy * e1x;
+    const float w = ((r.dx * qx + r.dy * qy) + r.dz * qz) * inv;
+    if (w < 0.0f || v + w > 1.0f) return false;
+    const float tt = ((e2x * qx + e2y * qy) + e2z * qz) * inv;
+    if (tt < 0.0f || tt >= t) return false;
+    t = tt; bu = v; bv = w;
+    return true;
+}
+
+// ---- stacks -------------------------------------------------------------------
+// An entry is a child reference plus its slab entry distance.  Encodings:
+//   interior child:  ref                      (bit 31 clear; the child's pair record index)
+//   leaf child:      0x80000000 | kind << 27 | ref   (kind 1..15, ref < 2^27; TLAS leaves are kind 1)
+//                    0x80000000 | node index          (BLAS leaves with more triangles or a far
+//                                                      first triangle; the node is re-read)
+//
+// In LDS an entry takes 6 bytes: the reference plus its entry distance as a
+// half-precision value rounded toward -inf.  The distance only decides the
+// popped-interior skip (skip when distance >= t); a rounded-down value never
+// skips a node the exact value would keep, and a node it keeps anyway is
+// visited, its children both miss, and the traversal pops on exactly as the
+// reference does.  Spilled entries keep the full float.
+__device__ __forceinline__ uint16_t half_down(float d)
+{
+    const _Float16 h = (_Float16)d;                        // round to nearest even
+    uint16_t b = __builtin_bit_cast(uint16_t, h);
+    if ((float)h > d) b = (b & 0x8000u) ? (uint16_t)(b + 1u) : (b == 0u ? (uint16_t)0x8001u : (uint16_t)(b - 1u));
+    return b;
+}
+__device__ __forceinline__ float half_up_to_float(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+
+// Entries past the LDS depth spill to global memory, level-major: level k of
+// thread g sits at spill[k * spill_stride + g], so lanes of a wave at the same
+// depth touch adjacent 8-byte words.
+template <int BLOCK, int LDS>
+struct LdsStack {
+    uint32_t* ent;
+    uint16_t* dist;
+    uint2* spill;            // this thread's level-0 spill slot
+    uint32_t spill_n;        // spill levels
+    uint32_t spill_stride;   // threads of the grid
+    uint32_t cap;
+    __device__ __forceinline__ bool push(uint32_t sp, uint32_t e, float d)
+    {
+        if (sp >= cap) return false;
+        if (sp < (uint32_t)LDS) { ent[sp * BLOCK] = e; dist[sp * BLOCK] = half_down(d); return true; }
+        const uint32_t k = sp - LDS;
+        if (k >= spill_n) return false;
+        spill[(size_t)k * spill_stride] = make_uint2(e, __float_as_uint(d));
+        return true;
+    }
+    __device__ __forceinline__ void get(uint32_t sp, uint32_t& e, float& d) const
+    {
+        if (sp < (uint32_t)LDS) { e = ent[sp * BLOCK]; d = half_up_to_float(dist[sp * BLOCK]); return; }
+        const uint2 v = spill[(size_t)(sp - LDS) * spill_stride];
+        e = v.x; d = __uint_as_float(v.y);
+    }
+};
+
+template <int N>
+struct PrivStack {
+    uint32_t ent[N];
+    float dist[N];
+    uint32_t cap;
+    __device__ __forceinline__ bool push(uint32_t sp, uint32_t e, float d)
+    {
+        if (sp >= (uint32_t)N || sp >= cap) return false;
+        ent[sp] = e; dist[sp] = d;
+        return true;
+    }
+    __device__ __forceinline__ void get(uint32_t sp, uint32_t& e, float& d) const { e = ent[sp]; d = dist[sp]; }
+};
+
+__device__ __forceinline__ float4 ld4(const void* p) { return *reinterpret_cast<const float4*>(p); }
+
+// Loads through the global address space (the pointers below all point to
+// device memory; generic pointers would compile to flat loads).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef float __attribute__((ext_vector_type(4))) native_float4;
+typedef __attribute__((address_space(1))) const native_float4 gfloat4;
+typedef __attribute__((address_space(1))) const uint32_t guint;
+__device__ __forceinline__ float4 gld4(const void* p)
+{
+    const native_float4 v = *(gfloat4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t gldu(const void* p) { return *(guint*)p; }
+#else
+__device__ __forceinline__ float4 gld4(const void* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ uint32_t gldu(const void* p) { return *reinterpret_cast<const uint32_t*>(p); }
+#endif
+
+enum LaneStatus { kRun = 0, kDone = 1, kOverflow = 2 };
+
+// Per-lane traversal state.  `ref`/`kind` name the lane's node: kind 0 =
+// interior node whose child pair sits at nodes + ref * 32 bytes (`nodes` =
+// the TLAS pair array, or the BLAS node array, whose children are adjacent),
+// kind != 0 = leaf (a TLAS leaf's ref is the instance, a BLAS leaf's
+// ref/kind its first triangle/count).  The world ray and the current
+// (object-space inside a BLAS) ray share t, as in the reference.
+struct TraceLane {
+    Ray3 wr, cr;
+    float t;
+    uint32_t ref, kind, sp, base_sp;
+    bool in_blas;
+    const char* nodes;
+    const GpuTri* tris;
+    uint32_t inst_id, h_inst, h_prim;
+    float h_u, h_v;
+};
+
+// Start a ray: the TLAS root box is tested once (trace_ray_tlas).  Returns
+// false when the root box is missed (the ray is finished, a miss).
+template <bool COUNT>
+__device__ __forceinline__ bool trace_begin(TraceLane& s, const TraceScene& sc, const wf_ray& ray, uint32_t root,
+                                            uint32_t* cnt)
+{
+    s.h_inst = ~0u; s.h_prim = ~0u; s.h_u = 0.0f; s.h_v = 0.0f;
+    s.t = ray.tmax;
+    s.wr.ox = ray.origin[0]; s.wr.oy = ray.origin[1]; s.wr.oz = ray.origin[2];
+    s.wr.dx = ray.direction[0]; s.wr.dy = ray.direction[1]; s.wr.dz = ray.direction[2];
+    s.wr.ix = 1.0f / s.wr.dx; s.wr.iy = 1.0f / s.wr.dy; s.wr.iz = 1.0f / s.wr.dz;
+    s.cr = s.wr;
+    s.sp = 0; s.base_sp = 0; s.in_blas = false;
+    s.nodes = reinterpret_cast<const char*>(sc.tpairs);
+    if (COUNT) { cnt[0] = 1; cnt[1] = 0; cnt[2] = 0; cnt[3] = 0; }
+    const float4 ra = gld4(&sc.tnodes[root]);
+    const float4 rb = gld4(reinterpret_cast<const char*>(&sc.tnodes[root]) + 16);
+    if (slab(ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, s.wr, s.t) == kRayMaxT) return false;
+    const uint32_t lr = __float_as_uint(ra.w);
+    s.ref = lr == 0 ? __float_as_uint(rb.w) : 2u * __float_as_uint(rb.w);   // instance, or 2 * record
+    s.kind = lr == 0 ? 1u : 0u;
+    return true;
+}
+
+// One traversal pass of a lane: up to `visits` interior visits (TLAS or
+// BLAS: both children tested, near child next, far child pushed; each with
+// its pops), then an instance entry or a triangle leaf if that is what the
+// lane now points at, then pops until its next node is known.  Returns kRun,
+// kDone (hit or miss found; any-hit rays stop at their first hit) or
+// kOverflow (the stack was full).
+template <bool SKIP, bool COUNT, bool TLDS, class Stack>
+__device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, Stack& st, const float4* s_tl,
+                                          uint32_t visits, bool any, uint32_t* cnt)
+{
+    int status = kRun;
+    // pop until the next node to process is known (or the stack is empty)
+    auto pop_next = [&]() {
+        for (;;) {
+            if (s.in_blas && s.sp == s.base_sp) {
+                s.in_blas = false;
+                s.cr = s.wr;
+                s.nodes = reinterpret_cast<const char*>(sc.tpairs);
+            }
+            if (s.sp == 0) { status = kDone; return; }
+            uint32_t e; float dd;
+            --s.sp;
+            st.get(s.sp, e, dd);
+            if (e & 0x80000000u) {
+                const uint32_t k4 = (e >> 27) & 15u;
+                if (k4) { s.kind = k4; s.ref = e & 0x07FFFFFFu; }
+                else {
+                    const char* nd = s.nodes + (size_t)(e & 0x07FFFFFFu) * 32u;
+                    s.ref = gldu(nd + 12); s.kind = gldu(nd + 28);
+                }
+                return;
+            }
+            if (SKIP && dd >= s.t) { if (COUNT) cnt[s.in_blas ? 1 : 0] += 2; continue; }
+            s.ref = e; s.kind = 0;
+            return;
+        }
+    };
+    // up to `visits` interior nodes per pass, so the rarer instance and leaf
+    // steps below are shared by more work
+    for (uint32_t v = 0; v < visits; ++v) {
+        if (status != kRun || s.kind != 0) break;
+        // interior node: near child next, far child pushed (swap only when
+        // dist_left > dist_right)
+        float4 a, b, c, d;
+        if (TLDS && !s.in_blas) {
+            const float4* q = s_tl + 2u * s.ref;
+            a = q[0]; b = q[1]; c = q[2]; d = q[3];
+        } else {
+            const char* cp = s.nodes + (size_t)s.ref * 32u;
+            a = gld4(cp); b = gld4(cp + 16); c = gld4(cp + 32); d = gld4(cp + 48);
+        }
+        if (COUNT) cnt[s.in_blas ? 1 : 0] += 2;
+        const float dl = slab(a.x, a.y, a.z, b.x, b.y, b.z, s.cr, s.t);
+        const float dr = slab(c.x, c.y, c.z, d.x, d.y, d.z, s.cr, s.t);
+        const bool sw = dl > dr;
+        const float dn = sw ? dr : dl, df = sw ? dl : dr;
+        const uint32_t n_ref = __float_as_uint(sw ? c.w : a.w), n_kind = __float_as_uint(sw ? d.w : b.w);
+        const uint32_t f_ref = __float_as_uint(sw ? a.w : c.w), f_kind = __float_as_uint(sw ? b.w : d.w);
+        const uint32_t f_id = s.ref + (sw ? 0u : 1u);
+        if (dn == kRayMaxT) {
+            pop_next();
+        } else {
+            s.ref = n_ref; s.kind = n_kind;
+            if (df != kRayMaxT) {
+                uint32_t e;
+                if (f_kind == 0) e = f_ref;
+                else if (f_kind < 16u && f_ref < (1u << 27)) e = 0x80000000u | (f_kind << 27) | f_ref;
+                else e = 0x80000000u | f_id;
+                if (st.push(s.sp, e, df)) ++s.sp;
+                else status = kOverflow;
+            }
+        }
+    }
+    if (status == kRun && s.kind != 0 && !s.in_blas) {
+        // TLAS leaf: enter the instance (trace_ray_bvh_instance); the BLAS
+        // root is entered without a box test
+        if (COUNT) { cnt[2]++; cnt[1]++; }
+        const GpuInstance* in = sc.inst + s.ref;
+        const float4 m0 = gld4(&in->m[0]), m1 = gld4(&in->m[4]), m2 = gld4(&in->m[8]);
+        const float4 pp = gld4(&in->nodes);
+        const uint64_t u0 = (uint64_t)__float_as_uint(pp.x) | ((uint64_t)__float_as_uint(pp.y) << 32);
+        const uint64_t u1 = (uint64_t)__float_as_uint(pp.z) | ((uint64_t)__float_as_uint(pp.w) << 32);
+        const char* bn = reinterpret_cast<const char*>(u0);
+        s.nodes = bn;
+        s.tris = reinterpret_cast<const GpuTri*>(u1);
+        const Ray3& w = s.wr;
+        // m[c*3+r]: m0 = (c0r0 c0r1 c0r2 c1r0), m1 = (c1r1 c1r2 c2r0 c2r1), m2 = (c2r2 c3r0 c3r1 c3r2)
+        s.cr.ox = (m0.x * w.ox + m0.w * w.oy) + (m1.z * w.oz + m2.y * 1.0f);
+        s.cr.oy = (m0.y * w.ox + m1.x * w.oy) + (m1.w * w.oz + m2.z * 1.0f);
+        s.cr.oz = (m0.z * w.ox + m1.y * w.oy) + (m2.x * w.oz + m2.w * 1.0f);
+        s.cr.dx = (m0.x * w.dx + m0.w * w.dy) + (m1.z * w.dz + m2.y * 0.0f);
+        s.cr.dy = (m0.y * w.dx + m1.x * w.dy) + (m1.w * w.dz + m2.z * 0.0f);
+        s.cr.dz = (m0.z * w.dx + m1.y * w.dy) + (m2.x * w.dz + m2.w * 0.0f);
+        s.cr.ix = 1.0f / s.cr.dx; s.cr.iy = 1.0f / s.cr.dy; s.cr.iz = 1.0f / s.cr.dz;
+        s.inst_id = s.ref;
+        s.ref = gldu(bn + 12);    // root left_first
+        s.kind = gldu(bn + 28);   // root prim_count
+        s.base_sp = s.sp;
+        s.in_blas = true;
+    }
+    if (status == kRun && s.kind != 0 && s.in_blas) {
+        // BLAS leaf: triangles in index order; a hit replaces the closest
+        // only if t < ray.t
+        for (uint32_t k = s.ref; k < s.ref + s.kind; ++k) {
+            const GpuTri* tr = s.tris + k;
+            const float4 a = gld4(&tr->v0[0]), b = gld4(&tr->e1[0]), c = gld4(&tr->e2[0]);
+            if (COUNT) cnt[3]++;
+            if (tri_test(a, b, c, s.cr, s.t, s.h_u, s.h_v)) {
+                s.h_prim = __float_as_uint(a.w);
+                s.h_inst = s.inst_id;
+                if (any) { status = kDone; break; }
+            }
+        }
+        if (status == kRun) pop_next();
+    }
+    return status;
+}
+
+} // namespace trav
+} // namespace wf
