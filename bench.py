@@ -268,7 +268,6 @@ def bench_batched(frames: int, tune: bool):
                 rt += time.perf_counter() - t
                 st = r.frame_stats()
                 rrays += st.rays_extended + st.shadow_rays
-    r.exit()
     # the batch as one megakernel launch per frame
     r.update_settings(spp=4, accumulate=0, frame_seed=9, use_wavefront_pathtracing=0)
     batched_frame()
@@ -277,6 +276,7 @@ def bench_batched(frames: int, tune: bool):
         dt, nr = batched_frame()
         mt += dt
         mrays += nr
+    r.exit()
     b_mr, p_mr, r_mr = brays / bt / 1e6, prays / pt / 1e6, rrays / rt / 1e6
     return {"workload": f"{n} scenes, {sum(len(i) // 3 for s in sl for _, i in s.meshes[:1]):,} soup tris, "
                         f"{sum(s.width * s.height for s in sl):,} pixels, 4 spp, depth 4, NEE",

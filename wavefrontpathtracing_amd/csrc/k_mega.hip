@@ -135,7 +135,7 @@ __device__ __forceinline__ void mega_loop(const FrameArgs& A, const MegaLaunch& 
 }
 
 template <int BLOCK, int LDS>
-__global__ __launch_bounds__(BLOCK) void k_megakernel(FrameArgs A, MegaLaunch M)
+__global__ __launch_bounds__(BLOCK, 4) void k_megakernel(FrameArgs A, MegaLaunch M)
 {
     constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
     __shared__ uint32_t s_ent[kLdsN];
