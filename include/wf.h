@@ -128,6 +128,9 @@ typedef struct wf_frame_stats {
     uint32_t rays_extended;      /* closest-hit rays traced this frame */
     uint32_t shadow_rays;        /* occlusion rays traced this frame */
     uint32_t stack_overflows;
+    /* per bounce (wavefront frames; the megakernel reports totals in [0]) */
+    uint32_t rays_per_bounce[9];
+    uint32_t shadow_rays_per_bounce[9];
 } wf_frame_stats;
 
 /* ---- lifecycle: renderer::init / exit (renderer.cpp:222-541) ---------- */

@@ -85,7 +85,8 @@ class BuildOptions(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("ms_clear", C.c_float), ("ms_generate", C.c_float), ("ms_extend", C.c_float), ("ms_shade", C.c_float),
                 ("ms_connect", C.c_float), ("ms_post", C.c_float), ("ms_total", C.c_float),
-                ("rays_extended", C.c_uint32), ("shadow_rays", C.c_uint32), ("stack_overflows", C.c_uint32)]
+                ("rays_extended", C.c_uint32), ("shadow_rays", C.c_uint32), ("stack_overflows", C.c_uint32),
+                ("rays_per_bounce", C.c_uint32 * 9), ("shadow_rays_per_bounce", C.c_uint32 * 9)]
 
 
 class TraversalConfig(C.Structure):

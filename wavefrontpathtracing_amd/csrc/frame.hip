@@ -453,7 +453,11 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
     st.ms_total = ms(0, n_ev - 1);
     for (uint32_t d = 0; d <= settings.max_bounces; ++d) {
         st.rays_extended += counts[d];
-        if (d < kMaxBounces) st.shadow_rays += counts[kShadowCountBase + d];
+        st.rays_per_bounce[d] = counts[d];
+        if (d < kMaxBounces) {
+            st.shadow_rays += counts[kShadowCountBase + d];
+            st.shadow_rays_per_bounce[d] = counts[kShadowCountBase + d];
+        }
     }
     uint32_t ov = 0;
     if ((s = read_overflows(&ov)) != WF_OK) return s;
