@@ -200,6 +200,12 @@ WF_API wf_status wf_renderer_get_settings(const wf_renderer* r, wf_render_settin
 WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_render_settings* settings);
 /* options for meshes created after the call (renderer.cpp:93-96) */
 WF_API wf_status wf_renderer_set_build_options(wf_renderer* r, const wf_build_options* options);
+/* Where create_render_mesh builds a BLAS (no reference counterpart; the
+ * reference builds on the CPU at load time, renderer.cpp:1266-1269).  Both
+ * builders produce the same nodes and triangle order.  AUTO uses the GPU for
+ * meshes of at least 16,384 triangles when interval_count is 2..64. */
+typedef enum wf_blas_builder { WF_BLAS_BUILDER_AUTO = 0, WF_BLAS_BUILDER_HOST = 1, WF_BLAS_BUILDER_DEVICE = 2 } wf_blas_builder;
+WF_API wf_status wf_renderer_set_blas_builder(wf_renderer* r, uint32_t builder);
 
 /* Traversal kernel configuration (no reference counterpart: the autotuner's
  * axes).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
@@ -269,6 +275,16 @@ WF_API wf_status wf_trace_closest_counters(wf_renderer* r, const wf_ray* rays, u
  * *out_size always receives the blob size, *out_node_count node_at. */
 WF_API wf_status wf_bvh_build(const float* tri_positions, uint32_t n, const wf_build_options* options,
                               void* out_blob, uint64_t capacity, uint64_t* out_size, uint32_t* out_node_count);
+/* The same BLAS built on the GPU (bvh_builder_t::build, bvh_builder.cpp:6-168,
+ * run as a device build): d_tri_positions holds n triangles of 9 floats in
+ * device memory; d_nodes receives the node array of the blob above
+ * (node_capacity >= 2n nodes) and d_indices its index array (n entries), the
+ * same bytes wf_bvh_build writes.  interval_count must be 2..64.  stream is a
+ * hipStream_t (0 = null stream); the call returns after the build finished
+ * and *out_node_count holds node_at. */
+WF_API wf_status wf_bvh_build_device(const float* d_tri_positions, uint32_t n, const wf_build_options* options,
+                                     wf_bvh_node* d_nodes, uint64_t node_capacity, uint32_t* d_indices,
+                                     uint32_t* out_node_count, void* stream);
 /* Agglomerative TLAS blob (tlas_header_t ‖ nodes ‖ instances). */
 WF_API wf_status wf_tlas_build(const wf_bvh_instance* instances, uint32_t m,
                                void* out_blob, uint64_t capacity, uint64_t* out_size, uint32_t* out_node_count);

@@ -1,0 +1,39 @@
+// Device BLAS builder (bvh_device.hip): the binned-SAH build of
+// bvh_builder.cpp:6-168 on the GPU, with the host builder's output bytes.
+#pragma once
+
+#include <string>
+
+#include "wf_internal.h"
+
+namespace wf {
+
+constexpr uint32_t kDevBuildMaxBins = 64;   // interval_count the device builder accepts (2..64)
+
+// Device scratch of one build, grown on demand and reusable across builds.
+struct DevBuildScratch {
+    DevBuf rc[2], rl[2], rh[2];     // per-triangle records in position order (ping-pong)
+    DevBuf tfr, tbl;                // partition tables, node-relative positions
+    DevBuf nodes, lists, small, blocks, info, bins, chunks, cnt, off, slot_ch, ctr;
+    void release();
+};
+
+struct DevBuildResult {
+    uint32_t node_count = 0;
+    uint32_t max_leaf = 0;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    uint32_t levels = 0;            // tree levels built a level at a time
+    uint32_t subtrees = 0;          // subtrees built one per wave
+};
+
+// Builds the BLAS of n triangles (9 floats each, device memory) into
+// d_nodes (capacity >= 2n nodes) and d_idx (n entries).  Synchronises the
+// stream a few times per tree level to size the next level's launches.
+wf_status device_build_blas(const float* d_pos, uint32_t n, const wf_build_options& opt, wf_bvh_node* d_nodes,
+                            uint32_t* d_idx, DevBuildScratch& S, hipStream_t stream, DevBuildResult& res,
+                            std::string& err);
+
+// GpuTri records in leaf order from the positions and the index array.
+hipError_t launch_tris_from_index(const float* d_pos, const uint32_t* d_idx, uint32_t n, GpuTri* d_tris, hipStream_t stream);
+
+} // namespace wf

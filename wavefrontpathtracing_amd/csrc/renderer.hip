@@ -166,6 +166,9 @@ extern "C" WF_API wf_status wf_renderer_exit(wf_renderer* r)
     if (!r) return WF_ERROR_INVALID_ARGUMENT;
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (auto& m : r->meshes) { m.nodes.release(); m.tris.release(); m.shade.release(); }
+    r->build_scratch.release();
+    r->build_pos.release();
+    r->build_idx.release();
     for (auto& t : r->textures) t.texels.release();
     r->release_frame_buffers();
     r->d_tnodes.release(); r->d_tpairs.release(); r->d_inst.release(); r->d_shade_inst.release(); r->d_textures.release();
@@ -259,6 +262,14 @@ extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* 
     return WF_OK;
 }
 
+extern "C" WF_API wf_status wf_renderer_set_blas_builder(wf_renderer* r, uint32_t builder)
+{
+    if (!r) return WF_ERROR_INVALID_ARGUMENT;
+    if (builder > WF_BLAS_BUILDER_DEVICE) return r->fail(WF_ERROR_INVALID_ARGUMENT, "unknown BLAS builder %u", builder);
+    r->blas_builder = builder;
+    return WF_OK;
+}
+
 extern "C" WF_API wf_status wf_renderer_set_build_options(wf_renderer* r, const wf_build_options* o)
 {
     if (!r || !o) return WF_ERROR_INVALID_ARGUMENT;
@@ -339,6 +350,39 @@ extern "C" WF_API wf_status wf_renderer_create_render_mesh(wf_renderer* r, const
             std::memcpy(shade[t].n[v], vx.normal, 12);
             std::memcpy(shade[t].uv[v], vx.uv, 8);
         }
+    }
+    const uint32_t bins = r->build_opts.interval_count;
+    const bool bins_ok = bins >= 2 && bins <= kDevBuildMaxBins;
+    if (r->blas_builder == WF_BLAS_BUILDER_DEVICE && !bins_ok)
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "the device BLAS builder takes 2..%u intervals", kDevBuildMaxBins);
+    if (r->blas_builder == WF_BLAS_BUILDER_DEVICE || (r->blas_builder == WF_BLAS_BUILDER_AUTO && bins_ok && n >= kAutoDeviceBuildMin)) {
+        // build on the GPU: positions up, nodes and leaf-ordered triangles made in place
+        hipError_t e = m.nodes.ensure(sizeof(wf_bvh_node) * 2ull * n);
+        if (e == hipSuccess) e = m.tris.ensure(sizeof(GpuTri) * n);
+        if (e == hipSuccess) e = m.shade.ensure(sizeof(GpuShadeTri) * n);
+        if (e == hipSuccess) e = r->build_pos.ensure(36ull * n);
+        if (e == hipSuccess) e = r->build_idx.ensure(4ull * n);
+        if (e == hipSuccess) e = hipMemcpyAsync(r->build_pos.p, m.pos9.data(), 36ull * n, hipMemcpyHostToDevice, r->stream);
+        if (e != hipSuccess) { m.nodes.release(); m.tris.release(); m.shade.release(); return r->hip_fail(e, "mesh upload"); }
+        DevBuildResult res;
+        std::string berr;
+        const wf_status s = device_build_blas((const float*)r->build_pos.p, n, r->build_opts, (wf_bvh_node*)m.nodes.p,
+                                              (uint32_t*)r->build_idx.p, r->build_scratch, r->stream, res, berr);
+        if (s != WF_OK) { m.nodes.release(); m.tris.release(); m.shade.release(); return r->fail(s, "device BLAS build: %s", berr.c_str()); }
+        e = launch_tris_from_index((const float*)r->build_pos.p, (const uint32_t*)r->build_idx.p, n, (GpuTri*)m.tris.p, r->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(m.shade.p, shade.data(), sizeof(GpuShadeTri) * n, hipMemcpyHostToDevice, r->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+        if (e != hipSuccess) { m.nodes.release(); m.tris.release(); m.shade.release(); return r->hip_fail(e, "mesh upload"); }
+        m.tri_count = n;
+        m.node_count = res.node_count;
+        m.max_leaf = res.max_leaf;
+        std::memcpy(m.blas_min, res.bmin, 12);   // renderer.cpp:107-111
+        std::memcpy(m.blas_max, res.bmax, 12);
+        m.alive = true;
+        m.version = 1;
+        r->meshes.push_back(std::move(m));
+        *out = make_handle((uint32_t)r->meshes.size() - 1, 1);
+        return WF_OK;
     }
     BlasData b;
     wf_status s = build_blas(m.pos9.data(), n, r->build_opts, b);

@@ -2,6 +2,7 @@
 // re-expressed over HIP device memory).
 #pragma once
 
+#include "bvh_device.h"
 #include "stages.h"
 #include "wf_internal.h"
 
@@ -15,12 +16,6 @@ constexpr uint32_t kMaxTextures = 65536;   // renderer.cpp:232 texture slotmap s
 constexpr uint32_t kStackTotal = 128;      // LDS + spill entries per lane (reference: 64 TLAS + 64 BLAS)
 constexpr int kNumStageEvents = 48;
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t n);
-    void release();
-};
 
 struct Mesh {
     bool alive = false;
@@ -71,6 +66,9 @@ struct wf_renderer {
     hipEvent_t ev[wf::kNumStageEvents] = {};
     wf_render_settings settings{};
     wf_build_options build_opts{8, 0};
+    uint32_t blas_builder = WF_BLAS_BUILDER_AUTO;
+    wf::DevBuildScratch build_scratch;   // device BLAS builder scratch, kept across meshes
+    wf::DevBuf build_pos, build_idx;
     std::string err;
 
     std::vector<wf::Mesh> meshes;

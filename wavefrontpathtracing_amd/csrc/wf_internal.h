@@ -7,6 +7,14 @@
 
 namespace wf {
 
+// Device allocation that only grows (hipMalloc / hipFree).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n);
+    void release();
+};
+
 // ---- device layouts ------------------------------------------------------------
 // Triangle in leaf order (pre-permuted through the index array), with the two
 // edges precomputed in FP32 (v1-v0 and v2-v0 are exactly what the

@@ -235,6 +235,13 @@ class Renderer:
         o = L.BuildOptions(interval_count, int(subdivide_single_prim))
         self._check(lib().wf_renderer_set_build_options(self.handle, C.byref(o)), "set_build_options")
 
+    BLAS_BUILDERS = {"auto": 0, "host": 1, "device": 2}
+
+    def set_blas_builder(self, builder: str) -> None:
+        """Where create_render_mesh builds BLASes: "auto" (the GPU for meshes of at
+        least 16,384 triangles), "host" or "device"; all give the same nodes."""
+        self._check(lib().wf_renderer_set_blas_builder(self.handle, self.BLAS_BUILDERS[builder]), "set_blas_builder")
+
     def view(self) -> L.View:
         v = L.View()
         self._check(lib().wf_renderer_get_view(self.handle, C.byref(v)), "get_view")
@@ -305,6 +312,28 @@ def bvh_build(tri_positions: np.ndarray, interval_count: int = 8, subdivide_sing
     out = np.empty(size.value, dtype=np.uint8)
     check(lib().wf_bvh_build(ptr(pos), len(pos), C.byref(opt), ptr(out), size.value, C.byref(size), C.byref(nodes)), what="bvh_build")
     return out.tobytes()
+
+
+def bvh_build_device(tri_positions: np.ndarray, interval_count: int = 8, subdivide_single_prim: bool = False) -> bytes:
+    """The blob of bvh_build with the tree built on the GPU (wf_bvh_build_device): the
+    positions are uploaded, the node and index arrays read back, and the header and
+    triangles (original order) assembled as bvh_builder_t::extract does."""
+    import struct
+    import torch
+    pos = np.ascontiguousarray(tri_positions, dtype=np.float32).reshape(-1, 9)
+    n = len(pos)
+    d_pos = torch.from_numpy(pos.copy()).to("cuda") if n else torch.empty((1, 9), dtype=torch.float32, device="cuda")
+    d_nodes = torch.empty((max(2 * n, 2), 8), dtype=torch.int32, device="cuda")
+    d_idx = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+    opt = L.BuildOptions(interval_count, int(subdivide_single_prim))
+    count = C.c_uint32()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(lib().wf_bvh_build_device(C.c_void_p(d_pos.data_ptr()), n, C.byref(opt), C.c_void_p(d_nodes.data_ptr()),
+                                    2 * n, C.c_void_p(d_idx.data_ptr()), C.byref(count), stream), what="bvh_build_device")
+    nodes = d_nodes[:count.value].cpu().numpy().tobytes()
+    idx = d_idx[:n].cpu().numpy().tobytes()
+    k = count.value
+    return struct.pack("<III", 12, 12 + 32 * k, 12 + 32 * k + 36 * n) + nodes + pos.tobytes() + idx
 
 
 def tlas_build(instances: np.ndarray) -> bytes:
