@@ -147,6 +147,63 @@ def bench_soup(n_tris: int, steps: int, cpu: bool):
     return out
 
 
+def bench_build(cpu: bool):
+    """BLAS build on the GPU (wf_bvh_builder_build_device, scratch kept across
+    builds) over random soups: 65,536 triangles at 16 intervals (the CPU
+    reference case), 1M at 16 and 2M at 8 (the largest meshes of the batch).
+    Each device build is checked equal, node for node and index for index, to
+    the host builder's; the host builder (threads) and the oracle's restated
+    reference builder (one thread) are timed beside it."""
+    import torch
+    from wavefrontpathtracing_amd import renderer as R, scenes
+    b = R.DeviceBvhBuilder()
+    out = {"workload": "random soups (centroids in [-1,1]^3, offsets +-U(0.005,0.02)), SAH binned BLAS; "
+                       "GPU time = wall clock of the whole build call, positions already in HBM"}
+    stream = torch.cuda.current_stream().cuda_stream
+    for n, bins, key in ((65536, 16, "soup_65k_16"), (1 << 20, 16, "soup_1m_16"), (2_000_000, 8, "soup_2m_8")):
+        v, i = scenes.random_soup(n % 1000 + 1, n)
+        pos9 = np.ascontiguousarray(v["position"][i].reshape(-1, 9), dtype=np.float32)
+        d_pos = torch.from_numpy(pos9).to("cuda")
+        d_nodes = torch.empty((2 * n, 8), dtype=torch.int32, device="cuda")
+        d_idx = torch.empty(n, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        args = (d_pos.data_ptr(), n, d_nodes.data_ptr(), 2 * n, d_idx.data_ptr(), bins, False, stream)
+        b.build(*args)
+        reps = 10 if n <= 65536 else 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            count = b.build(*args)
+        gpu_s = (time.perf_counter() - t0) / reps
+        levels, subtrees, max_leaf = b.last_build()
+        t0 = time.perf_counter()
+        blob = R.bvh_build(pos9, bins)
+        host_s = time.perf_counter() - t0
+        nodes_h, _, idx_h = R.parse_bvh_blob(blob)
+        assert count == len(nodes_h)
+        assert d_nodes[:count].cpu().numpy().tobytes() == nodes_h.tobytes(), "device BLAS nodes differ from the host build"
+        assert d_idx.cpu().numpy().tobytes() == idx_h.tobytes(), "device BLAS indices differ from the host build"
+        e = {"tris": n, "intervals": bins, "gpu_ms": round(gpu_s * 1e3, 3), "gpu_mtris_per_s": round(n / gpu_s / 1e6, 1),
+             "host_builder_ms": round(host_s * 1e3, 1), "host_threads": min(os.cpu_count() or 1, 16),
+             "nodes": count, "level_passes": levels, "wave_subtrees": subtrees, "max_leaf": max_leaf,
+             "equal_to_host_build": True}
+        if cpu and n <= (1 << 20):
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_api as O
+            reps_c = 20 if n <= 65536 else 1
+            t0 = time.perf_counter()
+            for _ in range(reps_c):
+                O.blas_blob(pos9, bins)
+            c_s = (time.perf_counter() - t0) / reps_c
+            e["cpu_oracle_ms"] = round(c_s * 1e3, 1)
+            e["cpu_oracle"] = {"kind": "port", "cores": 1,
+                               "sample": f"{reps_c} build(s) of the same soup by the oracle's restated builder"}
+            e["speedup_vs_cpu_oracle"] = round(c_s / gpu_s, 1)
+        out[key] = e
+        del d_pos, d_nodes, d_idx
+    b.close()
+    return out
+
+
 def bench_frame(frames: int):
     """One wavefront frame of the 256-instance scene at 1920x1080, 1 spp,
     depth 5, Russian roulette from bounce 3, next-event estimation to the 4
@@ -353,6 +410,7 @@ def main():
     ap.add_argument("--no-soup", action="store_true")
     ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--no-inverse", action="store_true")
+    ap.add_argument("--no-build", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune the traversal kernel on the batched frame")
     ap.add_argument("--sweep", action="store_true",
                     help="autotune the traversal kernel on the incoherent workload first (table on stderr)")
@@ -489,6 +547,8 @@ def main():
     if not args.no_soup:
         out["soup_65k"] = bench_soup(65536, steps=10, cpu=not args.no_cpu)
         out["soup_1m"] = bench_soup(1 << 20, steps=10, cpu=False)
+    if not args.no_build:
+        out["bvh_build"] = bench_build(cpu=not args.no_cpu)
     if not args.no_batch:
         out["batched"] = bench_batched(frames=2, tune=args.tune)
     if not args.no_adjoint:

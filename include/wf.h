@@ -285,6 +285,19 @@ WF_API wf_status wf_bvh_build(const float* tri_positions, uint32_t n, const wf_b
 WF_API wf_status wf_bvh_build_device(const float* d_tri_positions, uint32_t n, const wf_build_options* options,
                                      wf_bvh_node* d_nodes, uint64_t node_capacity, uint32_t* d_indices,
                                      uint32_t* out_node_count, void* stream);
+/* A reusable device builder (bvh_builder_t, bvh/bvh_builder.h:14-37): the same
+ * build as wf_bvh_build_device with its scratch kept across builds (it grows to
+ * the largest mesh and is freed by destroy).  last_build reports the tree
+ * levels built a level at a time, the subtrees built one per wave and the
+ * largest leaf of the last build. */
+typedef struct wf_bvh_builder wf_bvh_builder;
+WF_API wf_status wf_bvh_builder_create(wf_bvh_builder** out_builder);
+WF_API wf_status wf_bvh_builder_destroy(wf_bvh_builder* b);
+WF_API wf_status wf_bvh_builder_build_device(wf_bvh_builder* b, const float* d_tri_positions, uint32_t n,
+                                             const wf_build_options* options, wf_bvh_node* d_nodes, uint64_t node_capacity,
+                                             uint32_t* d_indices, uint32_t* out_node_count, void* stream);
+WF_API wf_status wf_bvh_builder_last_build(const wf_bvh_builder* b, uint32_t* levels, uint32_t* subtrees, uint32_t* max_leaf);
+WF_API const char* wf_bvh_builder_last_error(const wf_bvh_builder* b);
 /* Agglomerative TLAS blob (tlas_header_t ‖ nodes ‖ instances). */
 WF_API wf_status wf_tlas_build(const wf_bvh_instance* instances, uint32_t m,
                                void* out_blob, uint64_t capacity, uint64_t* out_size, uint32_t* out_node_count);

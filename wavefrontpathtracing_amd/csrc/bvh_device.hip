@@ -39,6 +39,7 @@
 #include "bvh_device.h"
 
 #include <cfloat>
+#include <new>
 
 #pragma clang fp contract(off)
 
@@ -1150,4 +1151,55 @@ extern "C" WF_API wf_status wf_bvh_build_device(const float* d_tri_positions, ui
     S.release();
     if (s == WF_OK) *out_node_count = res.node_count;
     return s;
+}
+
+// A reusable device builder (bvh_builder_t, bvh_builder.h:14-37): its scratch
+// grows to the largest mesh built and is kept until destroy.
+struct wf_bvh_builder {
+    wf::DevBuildScratch scratch;
+    wf::DevBuildResult last;
+    std::string err;
+};
+
+extern "C" WF_API wf_status wf_bvh_builder_create(wf_bvh_builder** out)
+{
+    if (!out) return WF_ERROR_INVALID_ARGUMENT;
+    *out = new (std::nothrow) wf_bvh_builder();
+    return *out ? WF_OK : WF_ERROR_OUT_OF_MEMORY;
+}
+
+extern "C" WF_API wf_status wf_bvh_builder_destroy(wf_bvh_builder* b)
+{
+    if (!b) return WF_ERROR_INVALID_ARGUMENT;
+    b->scratch.release();
+    delete b;
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_bvh_builder_build_device(wf_bvh_builder* b, const float* d_tri_positions, uint32_t n,
+                                                        const wf_build_options* options, wf_bvh_node* d_nodes,
+                                                        uint64_t node_capacity, uint32_t* d_indices, uint32_t* out_node_count,
+                                                        void* stream)
+{
+    if (!b || !d_tri_positions || !d_nodes || !d_indices || !out_node_count) return WF_ERROR_INVALID_ARGUMENT;
+    if (node_capacity < 2ull * n) return WF_ERROR_CAPACITY;
+    const wf_build_options opt = options ? *options : wf_build_options{8, 0};
+    b->err.clear();
+    const wf_status s = wf::device_build_blas(d_tri_positions, n, opt, d_nodes, d_indices, b->scratch, (hipStream_t)stream, b->last, b->err);
+    if (s == WF_OK) *out_node_count = b->last.node_count;
+    return s;
+}
+
+extern "C" WF_API wf_status wf_bvh_builder_last_build(const wf_bvh_builder* b, uint32_t* levels, uint32_t* subtrees, uint32_t* max_leaf)
+{
+    if (!b) return WF_ERROR_INVALID_ARGUMENT;
+    if (levels) *levels = b->last.levels;
+    if (subtrees) *subtrees = b->last.subtrees;
+    if (max_leaf) *max_leaf = b->last.max_leaf;
+    return WF_OK;
+}
+
+extern "C" WF_API const char* wf_bvh_builder_last_error(const wf_bvh_builder* b)
+{
+    return b ? b->err.c_str() : "null builder";
 }

@@ -336,6 +336,37 @@ def bvh_build_device(tri_positions: np.ndarray, interval_count: int = 8, subdivi
     return struct.pack("<III", 12, 12 + 32 * k, 12 + 32 * k + 36 * n) + nodes + pos.tobytes() + idx
 
 
+class DeviceBvhBuilder:
+    """wf_bvh_builder: device BLAS builds (bvh_builder_t on the GPU) whose scratch is
+    kept across builds.  Arguments are device addresses (torch data_ptr())."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().wf_bvh_builder_create(C.byref(h)), what="bvh_builder_create")
+        self.handle = h
+
+    def build(self, d_pos: int, n: int, d_nodes: int, node_capacity: int, d_idx: int, interval_count: int = 8,
+              subdivide_single_prim: bool = False, stream: int = 0) -> int:
+        opt = L.BuildOptions(interval_count, int(subdivide_single_prim))
+        count = C.c_uint32()
+        s = lib().wf_bvh_builder_build_device(self.handle, C.c_void_p(d_pos), n, C.byref(opt), C.c_void_p(d_nodes),
+                                              node_capacity, C.c_void_p(d_idx), C.byref(count), C.c_void_p(stream))
+        if s != 0:
+            raise WfError(s, f"bvh_builder_build_device: {lib().wf_bvh_builder_last_error(self.handle).decode()}")
+        return count.value
+
+    def last_build(self):
+        """(levels built a level at a time, subtrees built one per wave, largest leaf)"""
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().wf_bvh_builder_last_build(self.handle, C.byref(a), C.byref(b), C.byref(c)), what="bvh_builder_last_build")
+        return a.value, b.value, c.value
+
+    def close(self) -> None:
+        if self.handle:
+            lib().wf_bvh_builder_destroy(self.handle)
+            self.handle = None
+
+
 def tlas_build(instances: np.ndarray) -> bytes:
     inst = np.ascontiguousarray(instances, dtype=L.BVH_INSTANCE)
     size = C.c_uint64()
