@@ -201,6 +201,30 @@ def bench_build(cpu: bool):
         out[key] = e
         del d_pos, d_nodes, d_idx
     b.close()
+    # TLAS: the agglomerative build over random instances (the 256-instance
+    # scene's transform distribution, spread so the density stays constant)
+    from wavefrontpathtracing_amd import _lib as L
+    bmin, bmax = np.array([-1, -0.5, -1], np.float32), np.array([1, 0.5, 1], np.float32)
+    for m in (256, 4096, 32767):
+        rng = scenes.Rng(m)
+        spread = 10.0 * max(1.0, (m / 256) ** (1 / 3))
+        inst = np.array([R.make_bvh_instance(rng.transform(0.5, 2.0, spread), bmin, bmax, k % 4) for k in range(m)],
+                        dtype=L.BVH_INSTANCE)
+        R.tlas_build_device(inst)
+        reps = 5 if m <= 4096 else 1
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dblob = R.tlas_build_device(inst)
+        gpu_s = (time.perf_counter() - t0) / reps
+        e = {"instances": m, "gpu_ms": round(gpu_s * 1e3, 2)}
+        if m <= 4096 or cpu:
+            t0 = time.perf_counter()
+            hblob = R.tlas_build(inst)
+            host_s = time.perf_counter() - t0
+            assert dblob == hblob, "device TLAS differs from the host build"
+            e.update({"host_ms": round(host_s * 1e3, 2), "host_kind": "the library's host build (one thread)",
+                      "equal_to_host_build": True, "speedup_vs_host": round(host_s / gpu_s, 2)})
+        out[f"tlas_{m}"] = e
     return out
 
 

@@ -206,6 +206,10 @@ WF_API wf_status wf_renderer_set_build_options(wf_renderer* r, const wf_build_op
  * meshes of at least 16,384 triangles when interval_count is 2..64. */
 typedef enum wf_blas_builder { WF_BLAS_BUILDER_AUTO = 0, WF_BLAS_BUILDER_HOST = 1, WF_BLAS_BUILDER_DEVICE = 2 } wf_blas_builder;
 WF_API wf_status wf_renderer_set_blas_builder(wf_renderer* r, uint32_t builder);
+/* Where the renderer builds each scene's TLAS when its instances change (the
+ * reference rebuilds it on the CPU every frame, renderer.cpp:637-645); AUTO
+ * uses the GPU from 4,096 instances.  Same nodes either way. */
+WF_API wf_status wf_renderer_set_tlas_builder(wf_renderer* r, uint32_t builder);
 
 /* Traversal kernel configuration (no reference counterpart: the autotuner's
  * axes).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
@@ -301,6 +305,12 @@ WF_API const char* wf_bvh_builder_last_error(const wf_bvh_builder* b);
 /* Agglomerative TLAS blob (tlas_header_t ‖ nodes ‖ instances). */
 WF_API wf_status wf_tlas_build(const wf_bvh_instance* instances, uint32_t m,
                                void* out_blob, uint64_t capacity, uint64_t* out_size, uint32_t* out_node_count);
+/* The same TLAS built on the GPU (tlas_builder_t::build, tlas_builder.cpp:6-110):
+ * d_instances holds m records in device memory; d_nodes (capacity >= 2m
+ * nodes) receives the node array of the blob above, the same bytes
+ * wf_tlas_build writes.  Returns after the build; *out_node_count = node count. */
+WF_API wf_status wf_tlas_build_device(const wf_bvh_instance* d_instances, uint32_t m, wf_tlas_node* d_nodes,
+                                      uint64_t node_capacity, uint32_t* out_node_count, void* stream);
 /* The software instance record submit_render_mesh derives (renderer.cpp:1318-1336). */
 WF_API wf_status wf_make_bvh_instance(const float transform[16], const float blas_min[3], const float blas_max[3],
                                       uint32_t bvh_index, wf_bvh_instance* out);

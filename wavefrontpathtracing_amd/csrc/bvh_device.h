@@ -33,6 +33,12 @@ wf_status device_build_blas(const float* d_pos, uint32_t n, const wf_build_optio
                             uint32_t* d_idx, DevBuildScratch& S, hipStream_t stream, DevBuildResult& res,
                             std::string& err);
 
+// The TLAS of m instance records (device memory) into d_nodes (capacity >= 2m):
+// the reference's agglomerative build, node for node.  list/ids hold the
+// candidate list when m exceeds what one workgroup keeps in LDS.
+wf_status device_build_tlas(const wf_bvh_instance* d_inst, uint32_t m, wf_tlas_node* d_nodes, DevBuf& list, DevBuf& ids,
+                            DevBuf& status, hipStream_t stream, uint32_t& node_count, std::string& err);
+
 // GpuTri records in leaf order from the positions and the index array.
 hipError_t launch_tris_from_index(const float* d_pos, const uint32_t* d_idx, uint32_t n, GpuTri* d_tris, hipStream_t stream);
 

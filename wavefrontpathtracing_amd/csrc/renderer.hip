@@ -167,6 +167,7 @@ extern "C" WF_API wf_status wf_renderer_exit(wf_renderer* r)
     if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (auto& m : r->meshes) { m.nodes.release(); m.tris.release(); m.shade.release(); }
     r->build_scratch.release();
+    for (wf::DevBuf* d : {&r->tbuild_inst, &r->tbuild_nodes, &r->tbuild_list, &r->tbuild_ids, &r->tbuild_status}) d->release();
     r->build_pos.release();
     r->build_idx.release();
     for (auto& t : r->textures) t.texels.release();
@@ -267,6 +268,15 @@ extern "C" WF_API wf_status wf_renderer_set_blas_builder(wf_renderer* r, uint32_
     if (!r) return WF_ERROR_INVALID_ARGUMENT;
     if (builder > WF_BLAS_BUILDER_DEVICE) return r->fail(WF_ERROR_INVALID_ARGUMENT, "unknown BLAS builder %u", builder);
     r->blas_builder = builder;
+    return WF_OK;
+}
+
+extern "C" WF_API wf_status wf_renderer_set_tlas_builder(wf_renderer* r, uint32_t builder)
+{
+    if (!r) return WF_ERROR_INVALID_ARGUMENT;
+    if (builder > WF_BLAS_BUILDER_DEVICE) return r->fail(WF_ERROR_INVALID_ARGUMENT, "unknown TLAS builder %u", builder);
+    r->tlas_builder = builder;
+    r->tlas_valid = false;
     return WF_OK;
 }
 
@@ -465,8 +475,27 @@ wf_status wf_renderer::upload_scene()
         std::vector<wf_bvh_instance> sw(sl.instances.size());
         for (size_t i = 0; i < sl.instances.size(); ++i) sw[i] = sl.instances[i].sw;
         TlasData t;
-        wf_status s = build_tlas(sw.data(), (uint32_t)sw.size(), t);
-        if (s != WF_OK) return fail(s, "TLAS build failed");
+        const uint32_t m = (uint32_t)sw.size();
+        if (tlas_builder == WF_BLAS_BUILDER_DEVICE || (tlas_builder == WF_BLAS_BUILDER_AUTO && m >= kAutoDeviceTlasMin)) {
+            hipError_t e = tbuild_inst.ensure(sizeof(wf_bvh_instance) * m);
+            if (e == hipSuccess) e = tbuild_nodes.ensure(sizeof(wf_tlas_node) * 2ull * m);
+            if (e == hipSuccess) e = hipMemcpyAsync(tbuild_inst.p, sw.data(), sizeof(wf_bvh_instance) * m, hipMemcpyHostToDevice, stream);
+            if (e != hipSuccess) return hip_fail(e, "TLAS instance upload");
+            uint32_t count = 0;
+            std::string berr;
+            const wf_status s = device_build_tlas((const wf_bvh_instance*)tbuild_inst.p, m, (wf_tlas_node*)tbuild_nodes.p, tbuild_list,
+                                                  tbuild_ids, tbuild_status, stream, count, berr);
+            if (s != WF_OK) return fail(s, "device TLAS build: %s", berr.c_str());
+            t.nodes.resize(count);
+            e = hipMemcpyAsync(t.nodes.data(), tbuild_nodes.p, sizeof(wf_tlas_node) * count, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return hip_fail(e, "TLAS readback");
+            t.node_count = count;
+            t.instances = sw;
+        } else {
+            wf_status s = build_tlas(sw.data(), m, t);
+            if (s != WF_OK) return fail(s, "TLAS build failed");
+        }
         const uint32_t node_base = (uint32_t)all_nodes.size();
         const uint32_t inst_base = (uint32_t)gi.size();
         if (node_base + t.node_count > 0xFFFFu)

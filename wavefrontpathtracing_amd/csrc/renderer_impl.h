@@ -69,6 +69,8 @@ struct wf_renderer {
     uint32_t blas_builder = WF_BLAS_BUILDER_AUTO;
     wf::DevBuildScratch build_scratch;   // device BLAS builder scratch, kept across meshes
     wf::DevBuf build_pos, build_idx;
+    uint32_t tlas_builder = WF_BLAS_BUILDER_AUTO;
+    wf::DevBuf tbuild_inst, tbuild_nodes, tbuild_list, tbuild_ids, tbuild_status;
     std::string err;
 
     std::vector<wf::Mesh> meshes;

@@ -1,0 +1,170 @@
+// Device TLAS builder: the reference's agglomerative build
+// (tlas_builder.cpp:6-68 with find_best_match :86-110, including its
+// "max of mins / max of maxes" match metric) in one workgroup.
+//
+// The merge sequence is the reference's and stays sequential: A's best match
+// B, B's best match C, a merge when C == A, otherwise the walk moves on, with
+// swap-remove of the merged pair from the candidate list.  Each best-match
+// search (every remaining candidate against one node) is the parallel part: a
+// 1024-lane argmin that keeps the reference's first-index tie-break, so the
+// nodes, their numbering and the list order are those of the host build.
+// The candidate list (node ids and bounds) lives in LDS up to kTlasLdsMax
+// entries and in device memory beyond.
+#include "bvh_device.h"
+
+#include <cfloat>
+
+#pragma clang fp contract(off)
+
+namespace wf {
+namespace {
+
+constexpr uint32_t kTWG = 1024;
+constexpr uint32_t kTlasLdsMax = 2048;
+
+__device__ __forceinline__ float tmin_(float x, float y) { return (y < x) ? y : x; }
+__device__ __forceinline__ float tmax_(float x, float y) { return (x < y) ? y : x; }
+__device__ __forceinline__ float half_area3(float ex, float ey, float ez) { return ex * ey + ey * ez + ez * ex; }
+
+// status[0] = node count, status[1] = 1 if the walk found no match or did not end
+template <bool LDS>
+__global__ __launch_bounds__(kTWG) void k_tlas_build(const wf_bvh_instance* __restrict__ inst, uint32_t m, wf_tlas_node* nodes,
+                                                     float* g_list, uint32_t* g_ids, uint32_t* status)
+{
+    constexpr uint32_t CAP = LDS ? kTlasLdsMax : 1;
+    __shared__ float s_list[6][CAP];
+    __shared__ uint32_t s_ids[CAP];
+    __shared__ float s_ra[kTWG / 64];
+    __shared__ uint32_t s_ri[kTWG / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    float* lst[6];
+    for (int k = 0; k < 6; ++k) lst[k] = LDS ? s_list[k] : g_list + (size_t)k * m;
+    uint32_t* ids = LDS ? s_ids : g_ids;
+
+    // leaves at nodes 1..m (tlas_builder.cpp:18-31)
+    for (uint32_t i = tid; i < m; i += kTWG) {
+        const wf_bvh_instance& in = inst[i];
+        wf_tlas_node nd;
+        for (int k = 0; k < 3; ++k) {
+            nd.aabb_min[k] = in.aabb_min[k];
+            nd.aabb_max[k] = in.aabb_max[k];
+            lst[k][i] = in.aabb_min[k];
+            lst[3 + k][i] = in.aabb_max[k];
+        }
+        nd.left_right = 0;
+        nd.instance_idx = i;
+        nodes[1 + i] = nd;
+        ids[i] = 1 + i;
+    }
+    __syncthreads();
+
+    // find_best_match: the first candidate of least metric, FLT_MAX and NaN never chosen
+    auto best_match = [&](uint32_t a, uint32_t count) -> uint32_t {
+        const float amn0 = lst[0][a], amn1 = lst[1][a], amn2 = lst[2][a];
+        const float amx0 = lst[3][a], amx1 = lst[4][a], amx2 = lst[5][a];
+        float best = FLT_MAX;
+        uint32_t bi = 0xFFFFFFFFu;
+        for (uint32_t b = tid; b < count; b += kTWG) {
+            if (b == a) continue;
+            const float h0 = tmax_(amn0, lst[0][b]), h1 = tmax_(amn1, lst[1][b]), h2 = tmax_(amn2, lst[2][b]);
+            const float l0 = tmax_(amx0, lst[3][b]), l1 = tmax_(amx1, lst[4][b]), l2 = tmax_(amx2, lst[5][b]);
+            const float area = half_area3(h0 - l0, h1 - l1, h2 - l2);
+            if (area < best) { best = area; bi = b; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ob = __shfl_xor(best, o, 64);
+            const uint32_t oi = __shfl_xor(bi, o, 64);
+            if (ob < best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        if (lane == 0) { s_ra[wave] = best; s_ri[wave] = bi; }
+        __syncthreads();
+        best = s_ra[0];
+        bi = s_ri[0];
+        for (uint32_t w = 1; w < kTWG / 64; ++w) {
+            const float ob = s_ra[w];
+            const uint32_t oi = s_ri[w];
+            if (ob < best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        __syncthreads();
+        return bi;
+    };
+
+    // the walk (tlas_builder.cpp:33-66); every lane holds the same state
+    uint32_t count = m, at = m + 1, A = 0;
+    uint32_t B = best_match(A, count);
+    uint64_t guard = 0;
+    const uint64_t guard_max = 4ull * m * m + 16;
+    uint32_t err = 0;
+    while (count > 1) {
+        if (B == 0xFFFFFFFFu || ++guard > guard_max) { err = 1; break; }
+        const uint32_t C = best_match(B, count);
+        if (C != A) { A = B; B = C; continue; }
+        if (tid == 0) {
+            const uint32_t ia = ids[A], ib = ids[B];
+            wf_tlas_node nn;
+            nn.left_right = ia + (ib << 16);
+            nn.instance_idx = 0;
+            for (int k = 0; k < 3; ++k) {
+                nn.aabb_min[k] = tmin_(lst[k][A], lst[k][B]);
+                nn.aabb_max[k] = tmax_(lst[3 + k][A], lst[3 + k][B]);
+            }
+            nodes[at] = nn;
+            for (int k = 0; k < 3; ++k) { lst[k][A] = nn.aabb_min[k]; lst[3 + k][A] = nn.aabb_max[k]; }
+            ids[A] = at;
+            for (int k = 0; k < 6; ++k) lst[k][B] = lst[k][count - 1];
+            ids[B] = ids[count - 1];
+        }
+        __syncthreads();
+        ++at;
+        --count;
+        B = best_match(A, count);
+    }
+    if (tid == 0) {
+        if (!err) nodes[0] = nodes[ids[A]];
+        status[0] = at;
+        status[1] = err;
+    }
+}
+
+} // namespace
+
+wf_status device_build_tlas(const wf_bvh_instance* d_inst, uint32_t m, wf_tlas_node* d_nodes, DevBuf& list, DevBuf& ids,
+                            DevBuf& status, hipStream_t st, uint32_t& node_count, std::string& err)
+{
+    if (!d_inst || !d_nodes || m == 0) { err = "null pointer or no instances"; return WF_ERROR_INVALID_ARGUMENT; }
+    if (m > kMaxTlasInstances) { err = "too many instances for 16-bit TLAS child indices"; return WF_ERROR_CAPACITY; }
+    hipError_t e = status.ensure(8);
+    if (e == hipSuccess && m > kTlasLdsMax) e = list.ensure(24ull * m);
+    if (e == hipSuccess && m > kTlasLdsMax) e = ids.ensure(4ull * m);
+    if (e != hipSuccess) { err = "TLAS builder scratch allocation failed"; return WF_ERROR_OUT_OF_MEMORY; }
+    if (m <= kTlasLdsMax)
+        k_tlas_build<true><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, nullptr, nullptr, (uint32_t*)status.p);
+    else
+        k_tlas_build<false><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, (float*)list.p, (uint32_t*)ids.p, (uint32_t*)status.p);
+    uint32_t h[2] = {0, 0};
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h, status.p, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { err = std::string("TLAS build: ") + hipGetErrorString(e); return WF_ERROR_DEVICE; }
+    if (h[1]) { err = "TLAS build: the match walk found no partner (non-finite bounds?)"; return WF_ERROR_INVALID_ARGUMENT; }
+    node_count = h[0];
+    return WF_OK;
+}
+
+} // namespace wf
+
+extern "C" WF_API wf_status wf_tlas_build_device(const wf_bvh_instance* d_instances, uint32_t m, wf_tlas_node* d_nodes,
+                                                 uint64_t node_capacity, uint32_t* out_node_count, void* stream)
+{
+    if (!d_instances || !d_nodes || !out_node_count) return WF_ERROR_INVALID_ARGUMENT;
+    if (node_capacity < 2ull * m) return WF_ERROR_CAPACITY;
+    wf::DevBuf list, ids, status;
+    std::string err;
+    uint32_t count = 0;
+    const wf_status s = wf::device_build_tlas(d_instances, m, d_nodes, list, ids, status, (hipStream_t)stream, count, err);
+    list.release();
+    ids.release();
+    status.release();
+    if (s == WF_OK) *out_node_count = count;
+    return s;
+}

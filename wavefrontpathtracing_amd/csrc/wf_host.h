@@ -24,6 +24,7 @@ constexpr uint32_t kMaxTlasInstances = 32767;
 // BLAS stack entries pack a leaf's node index / first triangle in 27 bits.
 constexpr uint32_t kMaxBlasTriangles = 1u << 26;
 constexpr uint32_t kAutoDeviceBuildMin = 16384;   // create_render_mesh builds larger meshes on the GPU by default
+constexpr uint32_t kAutoDeviceTlasMin = 4096;     // scenes with at least this many instances build their TLAS on the GPU by default
 constexpr uint32_t kMaxInstances = 65536;   // renderer_common.h:23
 constexpr uint32_t kMaxBounces = 8;         // renderer.cpp:433-441, ImGui limit :1140
 

@@ -242,6 +242,11 @@ class Renderer:
         least 16,384 triangles), "host" or "device"; all give the same nodes."""
         self._check(lib().wf_renderer_set_blas_builder(self.handle, self.BLAS_BUILDERS[builder]), "set_blas_builder")
 
+    def set_tlas_builder(self, builder: str) -> None:
+        """Where each scene's TLAS is built: "auto" (the GPU from 4,096 instances),
+        "host" or "device"; all give the same nodes."""
+        self._check(lib().wf_renderer_set_tlas_builder(self.handle, self.BLAS_BUILDERS[builder]), "set_tlas_builder")
+
     def view(self) -> L.View:
         v = L.View()
         self._check(lib().wf_renderer_get_view(self.handle, C.byref(v)), "get_view")
@@ -375,6 +380,22 @@ def tlas_build(instances: np.ndarray) -> bytes:
     out = np.empty(size.value, dtype=np.uint8)
     check(lib().wf_tlas_build(ptr(inst), len(inst), ptr(out), size.value, C.byref(size), C.byref(nodes)), what="tlas_build")
     return out.tobytes()
+
+
+def tlas_build_device(instances: np.ndarray) -> bytes:
+    """The blob of tlas_build with the TLAS built on the GPU (wf_tlas_build_device)."""
+    import struct
+    import torch
+    inst = np.ascontiguousarray(instances, dtype=L.BVH_INSTANCE)
+    m = len(inst)
+    d_inst = torch.from_numpy(inst.view(np.uint8).copy()).to("cuda") if m else torch.empty(1, dtype=torch.uint8, device="cuda")
+    d_nodes = torch.empty((max(2 * m, 2), 8), dtype=torch.int32, device="cuda")
+    count = C.c_uint32()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(lib().wf_tlas_build_device(C.c_void_p(d_inst.data_ptr()), m, C.c_void_p(d_nodes.data_ptr()), 2 * m,
+                                     C.byref(count), stream), what="tlas_build_device")
+    nodes = d_nodes[:count.value].cpu().numpy().tobytes()
+    return struct.pack("<II", 8, 8 + 32 * count.value) + nodes + inst.tobytes()
 
 
 def make_bvh_instance(transform, blas_min, blas_max, bvh_index: int) -> np.ndarray:
