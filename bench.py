@@ -314,10 +314,10 @@ def bench_batched(frames: int, tune: bool):
 
     tuned = None
     if tune:
-        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, interior_visits=v)
-                for b in (64, 128, 256) for l in (8, 16) for v in (2, 4)]
+        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, interior_visits=v, bvh_layout=lay)
+                for b in (64, 128, 256) for l in (8, 16) for v in (2, 4) for lay in (0, 1)]
         tuned, table = tuning.autotune(r, lambda: batched_frame()[0], configs=cfgs, repeats=1)
-        tuned = dict(tuned, sweep={f"{c['block']}/{c['lds_stack']}/{c['interior_visits']}": round(t * 1e3, 1)
+        tuned = dict(tuned, sweep={f"{c['block']}/{c['lds_stack']}/{c['interior_visits']}/{c['bvh_layout']}": round(t * 1e3, 1)
                                    for c, t in table})
     batched_frame()
     bt, brays = 0.0, 0
@@ -488,11 +488,11 @@ def main():
             ev_b.synchronize()
             return ev_a.elapsed_time(ev_b) * 1e-3
 
-        cfgs = [dict(block=b, lds_stack=l, skip_popped=s, refill_min=16, tlas_in_lds=0, interior_visits=v)
-                for b in (64, 128, 256) for l in (4, 8, 12, 16) for s in (1,) for v in (4, 5)]
+        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, refill_min=16, tlas_in_lds=0, interior_visits=v, bvh_layout=lay)
+                for b in (64, 128, 256) for l in (8, 12, 16) for v in (2, 4) for lay in (0, 1)]
         best, table = tuning.autotune(r, timed, configs=cfgs, repeats=3)
         for c, t in table:
-            print(f"sweep block={c['block']} lds={c['lds_stack']} skip={c['skip_popped']} "
+            print(f"sweep block={c['block']} lds={c['lds_stack']} layout={c['bvh_layout']} "
                   f"visits={c['interior_visits']}: {len(rays) / t / 1e6:.1f} Mrays/s", file=sys.stderr)
 
     for _ in range(args.warmup):

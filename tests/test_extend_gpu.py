@@ -161,21 +161,61 @@ def test_stack_overflow_reports_status(small, lds):
     _assert_bitexact(r.trace_closest(sr), ref, "after the stack cap is lifted")
 
 
-@pytest.mark.parametrize("block,lds,skip,tlds,refill,visits", [
-    (64, 8, 1, 1, 16, 1), (128, 0, 1, 1, 1, 2), (256, 4, 0, 1, 64, 3), (128, 16, 0, 0, 16, 8), (256, 8, 1, 0, 32, 4),
-    (64, 0, 1, 0, 48, 1)])
-def test_traversal_configs_agree(small, block, lds, skip, tlds, refill, visits):
+@pytest.mark.parametrize("block,lds,skip,tlds,refill,visits,layout", [
+    (64, 8, 1, 1, 16, 1, 0), (128, 0, 1, 1, 1, 2, 0), (256, 4, 0, 1, 64, 3, 0), (128, 16, 0, 0, 16, 8, 0),
+    (256, 8, 1, 0, 32, 4, 0), (64, 0, 1, 0, 48, 1, 0),
+    (64, 16, 1, 0, 16, 4, 1), (128, 0, 1, 0, 1, 2, 1), (256, 8, 1, 0, 64, 1, 1), (64, 4, 1, 0, 32, 8, 1)])
+def test_traversal_configs_agree(small, block, lds, skip, tlds, refill, visits, layout):
     sc, r, orc = small
     from wavefrontpathtracing_amd import tuning
     bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
     sr = scenes.sphere_rays(11, bmin, bmax, 20000)
     ref = orc.trace_closest(sr, nthreads=8)
     tuning.set_extend_config(r, block=block, lds_stack=lds, skip_popped=skip, tlas_in_lds=tlds, refill_min=refill,
-                             interior_visits=visits)
+                             interior_visits=visits, bvh_layout=layout)
     try:
-        _assert_bitexact(r.trace_closest(sr), ref, f"config {block}/{lds}/{skip}/{tlds}/{refill}/{visits}")
+        _assert_bitexact(r.trace_closest(sr), ref, f"config {block}/{lds}/{skip}/{tlds}/{refill}/{visits}/{layout}")
     finally:
         tuning.set_extend_config(r)
+
+
+def test_quad_layout_matches_oracle(soup, small):
+    """The 4-wide layout on the soup (camera and incoherent rays), on the TLAS
+    scene (closest hit, occlusion, a stack cap that overflows) and on a
+    one-triangle BLAS whose root is a leaf: bit-exact against the oracle."""
+    from wavefrontpathtracing_amd import tuning
+    sc, r, orc = soup
+    tuning.set_extend_config(r, bvh_layout=1)
+    try:
+        rays = scenes.primary_rays(sc)
+        _assert_bitexact(r.trace_closest(rays), orc.trace_closest(rays, nthreads=8), "soup primary, 4-wide")
+    finally:
+        tuning.set_extend_config(r)
+    sc, r, orc = small
+    bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
+    sr = scenes.sphere_rays(17, bmin, bmax, 20000)
+    tuning.set_extend_config(r, bvh_layout=1)
+    try:
+        _assert_bitexact(r.trace_closest(sr), orc.trace_closest(sr, nthreads=8), "tlas incoherent, 4-wide")
+        sr2 = sr.copy()
+        sr2["tmax"] = np.float32(3.0)
+        assert np.array_equal(r.trace_occluded(sr2), orc.trace_any(sr2, nthreads=8))
+        tuning.set_extend_config(r, bvh_layout=1, stack_limit=2)
+        with pytest.raises(WfError) as ei:
+            r.trace_closest(sr)
+        assert ei.value.status == 5
+    finally:
+        tuning.set_extend_config(r)
+    _assert_bitexact(r.trace_closest(sr), orc.trace_closest(sr, nthreads=8), "back to 2-wide")
+
+
+def test_quad_layout_rejects_unsupported_combinations(small):
+    from wavefrontpathtracing_amd import tuning
+    sc, r, orc = small
+    for kw in (dict(bvh_layout=1, skip_popped=0), dict(bvh_layout=1, tlas_in_lds=1), dict(bvh_layout=2)):
+        with pytest.raises(WfError):
+            tuning.set_extend_config(r, **kw)
+    tuning.set_extend_config(r)
 
 
 @pytest.mark.slow

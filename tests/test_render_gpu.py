@@ -154,3 +154,40 @@ def test_megakernel_batch_equals_wavefront_batch():
             r.exit()
     for k in range(len(sl)):
         assert np.array_equal(imgs[0][k], imgs[1][k]), f"scene {k}"
+
+
+@pytest.mark.parametrize("wavefront", [1, 0])
+def test_quad_layout_frames_equal_pair_layout(wavefront):
+    """Frames traced through the 4-wide node layout equal the 2-wide ones bit
+    for bit, in the wavefront and the megakernel frame, for one scene and for
+    a batch (with the hipGraph replay on)."""
+    from wavefrontpathtracing_amd import tuning
+    sc = scenes.small_tlas_scene()
+    sl = scenes.batch_scenes(seed=33, n_scenes=4, tri_min=500, tri_max=20_000, resolutions=(24, 40))
+    out = []
+    for layout in (0, 1):
+        r = Renderer(sc.width, sc.height)
+        try:
+            tuning.set_extend_config(r, bvh_layout=layout)
+            scenes.load(r, sc)
+            r.update_settings(max_bounces=4, spp=2, next_event_estimation=1, hdr_env_strength=0.3, frame_seed=9,
+                              accumulate=0, use_wavefront_pathtracing=wavefront)
+            r.render()
+            single = r.read_energy()
+        finally:
+            r.exit()
+        r = Renderer(64, 64)
+        try:
+            tuning.set_extend_config(r, bvh_layout=layout)
+            scenes.load_batch(r, sl)
+            r.update_settings(max_bounces=3, spp=2, next_event_estimation=1, hdr_env_strength=0.0, frame_seed=5,
+                              use_wavefront_pathtracing=wavefront, use_hip_graph=1)
+            r.render()
+            r.render()
+            batch = [r.read_scene_energy(k, s.width, s.height) for k, s in enumerate(sl)]
+        finally:
+            r.exit()
+        out.append((single, batch))
+    assert np.array_equal(out[0][0], out[1][0])
+    for k in range(len(sl)):
+        assert np.array_equal(out[0][1][k], out[1][1][k]), f"batch scene {k}"

@@ -39,6 +39,9 @@ wf_status device_build_blas(const float* d_pos, uint32_t n, const wf_build_optio
 wf_status device_build_tlas(const wf_bvh_instance* d_inst, uint32_t m, wf_tlas_node* d_nodes, DevBuf& list, DevBuf& ids,
                             DevBuf& status, hipStream_t stream, uint32_t& node_count, std::string& err);
 
+// The 4-wide records (kQuadBytes each) of a BLAS node array, node_count / 2 of them.
+hipError_t launch_make_quads(const wf_bvh_node* d_nodes, uint32_t node_count, void* d_quads, hipStream_t stream);
+
 // GpuTri records in leaf order from the positions and the index array.
 hipError_t launch_tris_from_index(const float* d_pos, const uint32_t* d_idx, uint32_t n, GpuTri* d_tris, hipStream_t stream);
 

@@ -1203,3 +1203,48 @@ extern "C" WF_API const char* wf_bvh_builder_last_error(const wf_bvh_builder* b)
 {
     return b ? b->err.c_str() : "null builder";
 }
+
+namespace wf {
+namespace {
+
+// 4-wide records of a BLAS node array (see kQuadBytes): record 0 holds the
+// root, record r >= 1 the pair (2r, 2r+1) and the pairs of its interior
+// members; interior references become record numbers (left_first / 2).
+__global__ void k_make_quads(const wf_bvh_node* __restrict__ nodes, uint32_t node_count, wf_bvh_node* quads)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= node_count / 2u) return;
+    auto rw = [](wf_bvh_node n) {
+        if (n.prim_count == 0) n.left_first >>= 1;
+        return n;
+    };
+    wf_bvh_node* q = quads + 6ull * r;
+    wf_bvh_node z;
+    for (int k = 0; k < 3; ++k) { z.aabb_min[k] = 0.0f; z.aabb_max[k] = 0.0f; }
+    z.left_first = 0;
+    z.prim_count = 0;
+    if (r == 0) {
+        q[0] = rw(nodes[0]);
+        for (int k = 1; k < 6; ++k) q[k] = z;
+        return;
+    }
+    const wf_bvh_node a = nodes[2u * r], b = nodes[2u * r + 1u];
+    q[0] = rw(a);
+    q[1] = rw(b);
+    q[2] = a.prim_count == 0 ? rw(nodes[a.left_first]) : z;
+    q[3] = a.prim_count == 0 ? rw(nodes[a.left_first + 1u]) : z;
+    q[4] = b.prim_count == 0 ? rw(nodes[b.left_first]) : z;
+    q[5] = b.prim_count == 0 ? rw(nodes[b.left_first + 1u]) : z;
+}
+
+} // namespace
+
+hipError_t launch_make_quads(const wf_bvh_node* d_nodes, uint32_t node_count, void* d_quads, hipStream_t stream)
+{
+    const uint32_t n = node_count / 2u;
+    if (n == 0) return hipErrorInvalidValue;
+    k_make_quads<<<(n + 255) / 256, 256, 0, stream>>>(d_nodes, node_count, (wf_bvh_node*)d_quads);
+    return hipGetLastError();
+}
+
+} // namespace wf

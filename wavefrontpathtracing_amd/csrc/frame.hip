@@ -260,6 +260,7 @@ wf_status wf_renderer::record_frame(const FrameArgs& A, uint32_t grid, bool time
         M.scene.tpairs = (const float4*)d_tpairs.p;
         M.scene.n_tpairs = n_tpairs;
         M.scene.inst = (const GpuInstance*)d_inst.p;
+        M.scene.tquads = (const float4*)d_tquads.p;
         M.spill = (uint2*)d_spill.p;
         M.spill_per_thread = spill_per_thread;
         M.stack_cap = ext_cfg.stack_limit ? std::min(ext_cfg.stack_limit, kStackTotal) : kStackTotal;
@@ -410,7 +411,8 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
             settings.accumulate, ext_cfg.block, ext_cfg.lds_stack, ext_cfg.skip_popped, ext_grid,
             (uint64_t)(uintptr_t)d_tnodes.p, (uint64_t)(uintptr_t)d_tpairs.p, ext_cfg.refill_min, ext_cfg.tlas_lds, ext_cfg.visits, ext_grid_tlds, n_tpairs, (uint64_t)(uintptr_t)d_inst.p, (uint64_t)(uintptr_t)d_shade_inst.p,
             (uint64_t)(uintptr_t)d_emitters.p, (uint64_t)(uintptr_t)d_textures.p, (uint64_t)(uintptr_t)d_spill.p,
-            (uint64_t)(uintptr_t)d_rays[0].p, (uint64_t)(uintptr_t)d_image.p, slots.size()};
+            (uint64_t)(uintptr_t)d_rays[0].p, (uint64_t)(uintptr_t)d_image.p, slots.size(), ext_cfg.layout,
+            (uint64_t)(uintptr_t)d_tquads.p};
         if (!graph_exec || key != graph_key) {
             drop_graph();
             hipGraph_t g = nullptr;

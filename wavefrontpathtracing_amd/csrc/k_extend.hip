@@ -49,7 +49,7 @@ using namespace trav;
 // from the global cursor with one atomic (ballot + mbcnt give each idle lane
 // its index).  The loop ends when every lane is idle and the queue is
 // drained.
-template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, class Stack>
+template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4, class Stack>
 __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, const float4* s_tl, uint32_t lane, uint32_t n)
 {
     const TraceScene sc = L.scene;
@@ -77,9 +77,9 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
             if (i >= n) continue;
             ray_i = i;
             const uint32_t root = L.ray_root ? L.ray_root[i] : 0u;
-            if (!trace_begin<COUNT>(s, sc, L.rays[i], root, cnt)) status = kDone;
+            if (!trace_begin<COUNT, L4>(s, sc, L.rays[i], root, cnt)) status = kDone;
         } else if (ray_i != ~0u) {
-            status = trace_pass<SKIP, COUNT, TLDS>(s, sc, st, s_tl, visits, ANY, cnt);
+            status = trace_pass<SKIP, COUNT, TLDS, L4>(s, sc, st, s_tl, visits, ANY, cnt);
         }
         if (status != kRun) {
             if (status == kOverflow) {
@@ -106,7 +106,7 @@ extern __shared__ float4 s_tlas_dyn[];
 // TLDS: the workgroup first copies the TLAS records (at most
 // kTlasLdsPairs, in dynamic LDS sized by the launch) and reads TLAS nodes
 // from there.
-template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS>
+template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4 = false>
 __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
 {
     constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
@@ -124,11 +124,11 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
     if constexpr (LDS > 0) {
         LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + gtid, L.spill_per_thread, gridDim.x * BLOCK,
                                 L.stack_cap};
-        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS>(L, st, s_tl, tid & 63u, n);
+        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4>(L, st, s_tl, tid & 63u, n);
     } else {
         PrivStack<128> st;
         st.cap = L.stack_cap;
-        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS>(L, st, s_tl, tid & 63u, n);
+        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4>(L, st, s_tl, tid & 63u, n);
     }
 }
 
@@ -162,8 +162,25 @@ void* pick_tlds(const ExtendConfig& c, bool count, bool any)
                : pick_lds<false, false, TLDS>(c.block, c.lds_stack, c.skip_popped != 0);
 }
 
+// 4-wide layout: popped-interior skip on, no counters, TLAS in global memory
+template <bool ANY>
+void* pick_quad(uint32_t block, uint32_t lds)
+{
+#define WF_CASE(B, S) \
+    if (block == B && lds == S) return reinterpret_cast<void*>(&k_extend<B, S, true, false, ANY, false, true>);
+    WF_CASE(64, 0) WF_CASE(64, 4) WF_CASE(64, 8) WF_CASE(64, 12) WF_CASE(64, 16)
+    WF_CASE(128, 0) WF_CASE(128, 4) WF_CASE(128, 8) WF_CASE(128, 12) WF_CASE(128, 16)
+    WF_CASE(256, 0) WF_CASE(256, 4) WF_CASE(256, 8) WF_CASE(256, 12) WF_CASE(256, 16)
+#undef WF_CASE
+    return nullptr;
+}
+
 void* pick_kernel(const ExtendConfig& c, bool count, bool any, bool tlds)
 {
+    if (c.layout == 1) {
+        if (count || tlds || !c.skip_popped) return nullptr;
+        return any ? pick_quad<true>(c.block, c.lds_stack) : pick_quad<false>(c.block, c.lds_stack);
+    }
     return tlds ? pick_tlds<true>(c, count, any) : pick_tlds<false>(c, count, any);
 }
 
@@ -179,7 +196,9 @@ hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, size_t dyn_l
 hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStream_t stream)
 {
     const bool tlds = L.grid_tlds != 0;   // the host sets it when the TLAS records fit
-    void* k = pick_kernel(cfg, L.counters != nullptr, L.any_hit != 0, tlds);
+    ExtendConfig c = cfg;
+    if (L.counters) c.layout = 0;         // traversal counters come from the 2-wide kernels
+    void* k = pick_kernel(c, L.counters != nullptr, L.any_hit != 0, tlds);
     if (!k) return hipErrorInvalidValue;
     ExtendLaunch arg = L;
     void* args[] = {&arg};

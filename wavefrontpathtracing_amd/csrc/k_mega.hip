@@ -24,7 +24,7 @@ namespace {
 using namespace trav;
 using namespace shading;
 
-template <int BLOCK, int LDS, class Stack>
+template <int BLOCK, int LDS, bool L4, class Stack>
 __device__ __forceinline__ void mega_loop(const FrameArgs& A, const MegaLaunch& M, Stack& st, uint32_t lane)
 {
     const TraceScene sc = M.scene;
@@ -48,7 +48,7 @@ __device__ __forceinline__ void mega_loop(const FrameArgs& A, const MegaLaunch& 
         r.direction[0] = dir.x; r.direction[1] = dir.y; r.direction[2] = dir.z; r.tmax = tmax;
         any = shadow;
         if (shadow) ++n_shadow; else ++n_closest;
-        return trace_begin<false>(s, sc, r, root, cnt_dummy);
+        return trace_begin<false, L4>(s, sc, r, root, cnt_dummy);
     };
 
     for (;;) {
@@ -79,7 +79,7 @@ __device__ __forceinline__ void mega_loop(const FrameArgs& A, const MegaLaunch& 
             camera_ray(gs, (p - gs.path_off) % gs.n_pix, o, dir);
             if (!start(o, dir, FLT_MAX, false)) status = kDone;
         } else if (p != ~0u) {
-            status = trace_pass<true, false, false>(s, sc, st, nullptr, M.visits, any, cnt_dummy);
+            status = trace_pass<true, false, false, L4>(s, sc, st, nullptr, M.visits, any, cnt_dummy);
         }
         // a finished ray advances its path; the path may start another ray
         // that finishes at once (its TLAS root box missed), hence the loop
@@ -134,7 +134,7 @@ __device__ __forceinline__ void mega_loop(const FrameArgs& A, const MegaLaunch& 
     }
 }
 
-template <int BLOCK, int LDS>
+template <int BLOCK, int LDS, bool L4>
 __global__ __launch_bounds__(BLOCK, 4) void k_megakernel(FrameArgs A, MegaLaunch M)
 {
     constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
@@ -144,20 +144,20 @@ __global__ __launch_bounds__(BLOCK, 4) void k_megakernel(FrameArgs A, MegaLaunch
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     if constexpr (LDS > 0) {
         LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, M.spill + gtid, M.spill_per_thread, gridDim.x * BLOCK, M.stack_cap};
-        mega_loop<BLOCK, LDS>(A, M, st, tid & 63u);
+        mega_loop<BLOCK, LDS, L4>(A, M, st, tid & 63u);
     } else {
         PrivStack<128> st;
         st.cap = M.stack_cap;
-        mega_loop<BLOCK, LDS>(A, M, st, tid & 63u);
+        mega_loop<BLOCK, LDS, L4>(A, M, st, tid & 63u);
     }
 }
 
 template <int BLOCK, int LDS>
-void* mega_ptr() { return reinterpret_cast<void*>(&k_megakernel<BLOCK, LDS>); }
+void* mega_ptr(bool l4) { return l4 ? reinterpret_cast<void*>(&k_megakernel<BLOCK, LDS, true>) : reinterpret_cast<void*>(&k_megakernel<BLOCK, LDS, false>); }
 
-void* pick_mega(uint32_t block, uint32_t lds)
+void* pick_mega(uint32_t block, uint32_t lds, bool l4)
 {
-#define WF_CASE(B, S) if (block == B && lds == S) return mega_ptr<B, S>();
+#define WF_CASE(B, S) if (block == B && lds == S) return mega_ptr<B, S>(l4);
     WF_CASE(64, 0) WF_CASE(64, 4) WF_CASE(64, 8) WF_CASE(64, 12) WF_CASE(64, 16)
     WF_CASE(128, 0) WF_CASE(128, 4) WF_CASE(128, 8) WF_CASE(128, 12) WF_CASE(128, 16)
     WF_CASE(256, 0) WF_CASE(256, 4) WF_CASE(256, 8) WF_CASE(256, 12) WF_CASE(256, 16)
@@ -175,14 +175,14 @@ __global__ void k_mega_counts_clear(uint32_t* counts)
 
 hipError_t mega_occupancy(const ExtendConfig& cfg, int* blocks_per_cu)
 {
-    void* k = pick_mega(cfg.block, cfg.lds_stack);
+    void* k = pick_mega(cfg.block, cfg.lds_stack, cfg.layout == 1);
     if (!k) return hipErrorInvalidValue;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, (int)cfg.block, 0);
 }
 
 hipError_t launch_megakernel(const FrameArgs& A, const MegaLaunch& M, const ExtendConfig& cfg, hipStream_t stream)
 {
-    void* k = pick_mega(cfg.block, cfg.lds_stack);
+    void* k = pick_mega(cfg.block, cfg.lds_stack, cfg.layout == 1);
     if (!k) return hipErrorInvalidValue;
     k_mega_counts_clear<<<1, 64, 0, stream>>>(A.counts);
     hipError_t e = hipGetLastError();

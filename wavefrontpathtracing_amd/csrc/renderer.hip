@@ -165,7 +165,8 @@ extern "C" WF_API wf_status wf_renderer_exit(wf_renderer* r)
 {
     if (!r) return WF_ERROR_INVALID_ARGUMENT;
     if (r->stream) (void)hipStreamSynchronize(r->stream);
-    for (auto& m : r->meshes) { m.nodes.release(); m.tris.release(); m.shade.release(); }
+    for (auto& m : r->meshes) { m.nodes.release(); m.tris.release(); m.shade.release(); m.quads.release(); }
+    r->d_tquads.release();
     r->build_scratch.release();
     for (wf::DevBuf* d : {&r->tbuild_inst, &r->tbuild_nodes, &r->tbuild_list, &r->tbuild_ids, &r->tbuild_status}) d->release();
     r->build_pos.release();
@@ -232,7 +233,9 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     if (c->lds_stack_entries != 0 && c->lds_stack_entries != 4 && c->lds_stack_entries != 8 && c->lds_stack_entries != 12 &&
         c->lds_stack_entries != 16)
         return r->fail(WF_ERROR_INVALID_ARGUMENT, "lds_stack_entries must be 0, 4, 8, 12 or 16");
-    if (c->bvh_layout != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "bvh_layout must be 0 (BVH2)");
+    if (c->bvh_layout > 1) return r->fail(WF_ERROR_INVALID_ARGUMENT, "bvh_layout must be 0 (2-wide) or 1 (4-wide)");
+    if (c->bvh_layout == 1 && (!c->skip_popped_interior || c->tlas_in_lds))
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "the 4-wide layout needs skip_popped_interior = 1 and tlas_in_lds = 0");
     if (c->refill_min > 64) return r->fail(WF_ERROR_INVALID_ARGUMENT, "refill_min must be in [0, 64]");
     if (c->interior_visits > 8) return r->fail(WF_ERROR_INVALID_ARGUMENT, "interior_visits must be in [0, 8]");
     const ExtendConfig old = r->ext_cfg;
@@ -243,6 +246,8 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     r->ext_cfg.refill_min = c->refill_min ? c->refill_min : ExtendConfig{}.refill_min;
     r->ext_cfg.tlas_lds = c->tlas_in_lds ? 1u : 0u;
     r->ext_cfg.visits = c->interior_visits ? c->interior_visits : ExtendConfig{}.visits;
+    r->ext_cfg.layout = c->bvh_layout;
+    if (r->ext_cfg.layout != old.layout) r->tlas_valid = false;   // instance records point at the layout's arrays
     r->drop_graph();
     const wf_status s = r->ensure_trace_scratch();
     if (s != WF_OK) { r->ext_cfg = old; return s; }
@@ -255,7 +260,7 @@ extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* 
     out->workgroup_size = r->ext_cfg.block;
     out->lds_stack_entries = r->ext_cfg.lds_stack;
     out->skip_popped_interior = r->ext_cfg.skip_popped;
-    out->bvh_layout = 0;
+    out->bvh_layout = r->ext_cfg.layout;
     out->stack_limit = r->ext_cfg.stack_limit;
     out->refill_min = r->ext_cfg.refill_min;
     out->tlas_in_lds = r->ext_cfg.tlas_lds;
@@ -465,6 +470,16 @@ wf_status wf_renderer::upload_scene()
 {
     if (tlas_valid) return WF_OK;
     if (slots.empty()) return fail(WF_ERROR_INVALID_STATE, "no scene (call begin_scene)");
+    if (ext_cfg.layout == 1) {
+        for (auto& sl : slots)
+            for (const InstanceRec& rec : sl.instances) {
+                Mesh& m = meshes[rec.mesh];
+                if (m.quads.p) continue;
+                hipError_t e = m.quads.ensure((size_t)kQuadBytes * (m.node_count / 2u));
+                if (e == hipSuccess) e = launch_make_quads((const wf_bvh_node*)m.nodes.p, m.node_count, m.quads.p, stream);
+                if (e != hipSuccess) return hip_fail(e, "4-wide BLAS records");
+            }
+    }
     std::vector<wf_tlas_node> all_nodes;
     std::vector<GpuInstance> gi;
     std::vector<GpuShadeInstance> si;
@@ -517,6 +532,8 @@ wf_status wf_renderer::upload_scene()
                 for (int rr = 0; rr < 3; ++rr) g.m[c * 3 + rr] = rec.sw.world_to_local[c * 4 + rr];
             g.nodes = (const wf_bvh_node*)m.nodes.p;
             g.tris = (const GpuTri*)m.tris.p;
+            g.quads = (const char*)m.quads.p;
+            g.tris4 = g.tris;
             gi.push_back(g);
             GpuShadeInstance sh;
             std::memset(&sh, 0, sizeof(sh));
@@ -560,8 +577,38 @@ wf_status wf_renderer::upload_scene()
             std::memcpy(slot + 7, &kind, 4);
         }
     }
+    // 4-wide TLAS records: record k = pair k, then the pairs of its interior
+    // children; interior references become record numbers (see kQuadBytes)
+    std::vector<float> quads;
+    if (ext_cfg.layout == 1) {
+        quads.assign(48 * std::max<uint32_t>(n_rec, 1), 0.0f);
+        auto put = [](float* dst, const float* src) {
+            std::memcpy(dst, src, 32);
+            uint32_t ref, kind;
+            std::memcpy(&ref, src + 3, 4);
+            std::memcpy(&kind, src + 7, 4);
+            if (kind == 0) { ref >>= 1; std::memcpy(dst + 3, &ref, 4); }
+        };
+        for (uint32_t k = 0; k < n_rec; ++k) {
+            float* q = &quads[48 * (size_t)k];
+            for (int c = 0; c < 2; ++c) {
+                const float* src = &pairs[16 * (size_t)k + 8 * c];
+                put(q + 8 * c, src);
+                uint32_t ref, kind;
+                std::memcpy(&ref, src + 3, 4);
+                std::memcpy(&kind, src + 7, 4);
+                if (kind == 0) {
+                    put(q + 16 + 16 * c, &pairs[16 * (size_t)(ref >> 1)]);
+                    put(q + 24 + 16 * c, &pairs[16 * (size_t)(ref >> 1) + 8]);
+                }
+            }
+        }
+    }
     drop_graph();
     hipError_t e = d_tnodes.ensure(sizeof(wf_tlas_node) * all_nodes.size());
+    if (e == hipSuccess && !quads.empty()) e = d_tquads.ensure(sizeof(float) * quads.size());
+    if (e == hipSuccess && !quads.empty())
+        e = hipMemcpyAsync(d_tquads.p, quads.data(), sizeof(float) * quads.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = d_tpairs.ensure(sizeof(float) * pairs.size());
     if (e == hipSuccess) e = hipMemcpyAsync(d_tpairs.p, pairs.data(), sizeof(float) * pairs.size(), hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = d_inst.ensure(sizeof(GpuInstance) * gi.size());
@@ -612,6 +659,7 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     L.scene.tpairs = (const float4*)d_tpairs.p;
     L.scene.n_tpairs = n_tpairs;
     L.scene.inst = (const GpuInstance*)d_inst.p;
+    L.scene.tquads = (const float4*)d_tquads.p;
     L.rays = d_rays;
     L.ray_root = d_roots;
     L.d_count = d_count;

@@ -24,6 +24,7 @@ struct Mesh {
     float blas_min[3] = {0, 0, 0}, blas_max[3] = {0, 0, 0};
     std::vector<float> pos9;   // host copy of positions (emitter sampling)
     DevBuf nodes, tris, shade;
+    DevBuf quads;              // 4-wide records, built on first use of that layout
 };
 
 struct Texture {
@@ -87,6 +88,7 @@ struct wf_renderer {
     wf_frame_stats stats{};
 
     // TLAS + instance records of every slot, concatenated on the device
+    wf::DevBuf d_tquads;
     wf::DevBuf d_tnodes, d_tpairs, d_inst, d_shade_inst, d_textures, d_emitters, d_scenes, d_dyn;
     bool tlas_valid = false;
     std::vector<uint8_t> h_scenes;   // staging of the per-frame scene table

@@ -171,7 +171,7 @@ struct TraceLane {
 
 // Start a ray: the TLAS root box is tested once (trace_ray_tlas).  Returns
 // false when the root box is missed (the ray is finished, a miss).
-template <bool COUNT>
+template <bool COUNT, bool L4 = false>
 __device__ __forceinline__ bool trace_begin(TraceLane& s, const TraceScene& sc, const wf_ray& ray, uint32_t root,
                                             uint32_t* cnt)
 {
@@ -182,13 +182,14 @@ __device__ __forceinline__ bool trace_begin(TraceLane& s, const TraceScene& sc, 
     s.wr.ix = 1.0f / s.wr.dx; s.wr.iy = 1.0f / s.wr.dy; s.wr.iz = 1.0f / s.wr.dz;
     s.cr = s.wr;
     s.sp = 0; s.base_sp = 0; s.in_blas = false;
-    s.nodes = reinterpret_cast<const char*>(sc.tpairs);
+    s.nodes = reinterpret_cast<const char*>(L4 ? sc.tquads : sc.tpairs);
     if (COUNT) { cnt[0] = 1; cnt[1] = 0; cnt[2] = 0; cnt[3] = 0; }
     const float4 ra = gld4(&sc.tnodes[root]);
     const float4 rb = gld4(reinterpret_cast<const char*>(&sc.tnodes[root]) + 16);
     if (slab(ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, s.wr, s.t) == kRayMaxT) return false;
     const uint32_t lr = __float_as_uint(ra.w);
-    s.ref = lr == 0 ? __float_as_uint(rb.w) : 2u * __float_as_uint(rb.w);   // instance, or 2 * record
+    // instance, or the root's record: 2 * record (pair records) / record (4-wide)
+    s.ref = lr == 0 ? __float_as_uint(rb.w) : (L4 ? 1u : 2u) * __float_as_uint(rb.w);
     s.kind = lr == 0 ? 1u : 0u;
     return true;
 }
@@ -199,10 +200,11 @@ __device__ __forceinline__ bool trace_begin(TraceLane& s, const TraceScene& sc, 
 // lane now points at, then pops until its next node is known.  Returns kRun,
 // kDone (hit or miss found; any-hit rays stop at their first hit) or
 // kOverflow (the stack was full).
-template <bool SKIP, bool COUNT, bool TLDS, class Stack>
+template <bool SKIP, bool COUNT, bool TLDS, bool L4 = false, class Stack>
 __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, Stack& st, const float4* s_tl,
                                           uint32_t visits, bool any, uint32_t* cnt)
 {
+    static_assert(!L4 || (!COUNT && !TLDS), "the 4-wide layout has no counting or LDS-TLAS variant");
     int status = kRun;
     // pop until the next node to process is known (or the stack is empty)
     auto pop_next = [&]() {
@@ -210,7 +212,7 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
             if (s.in_blas && s.sp == s.base_sp) {
                 s.in_blas = false;
                 s.cr = s.wr;
-                s.nodes = reinterpret_cast<const char*>(sc.tpairs);
+                s.nodes = reinterpret_cast<const char*>(L4 ? sc.tquads : sc.tpairs);
             }
             if (s.sp == 0) { status = kDone; return; }
             uint32_t e; float dd;
@@ -232,8 +234,47 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
     };
     // up to `visits` interior nodes per pass, so the rarer instance and leaf
     // steps below are shared by more work
+    // push a far child (an interior child's reference; a leaf's ref/kind when
+    // they fit the entry, else the node's index to read them back at the pop)
+    auto push_far = [&](uint32_t f_ref, uint32_t f_kind, uint32_t f_id, float df) {
+        uint32_t e;
+        if (f_kind == 0) e = f_ref;
+        else if (f_kind < 16u && f_ref < (1u << 27)) e = 0x80000000u | (f_kind << 27) | f_ref;
+        else e = 0x80000000u | f_id;
+        if (st.push(s.sp, e, df)) ++s.sp;
+        else status = kOverflow;
+    };
     for (uint32_t v = 0; v < visits; ++v) {
         if (status != kRun || s.kind != 0) break;
+        if constexpr (L4) {
+            // 4-wide record: this node's children, then (same record) the
+            // near child's children when it is interior
+            const uint32_t rec = s.ref;
+            const char* rp = s.nodes + (size_t)rec * kQuadBytes;
+            const float4 a = gld4(rp), b = gld4(rp + 16), c = gld4(rp + 32), d = gld4(rp + 48);
+            const float dl = slab(a.x, a.y, a.z, b.x, b.y, b.z, s.cr, s.t);
+            const float dr = slab(c.x, c.y, c.z, d.x, d.y, d.z, s.cr, s.t);
+            const bool sw = dl > dr;
+            const float dn = sw ? dr : dl, df = sw ? dl : dr;
+            const uint32_t n_ref = __float_as_uint(sw ? c.w : a.w), n_kind = __float_as_uint(sw ? d.w : b.w);
+            if (dn == kRayMaxT) { pop_next(); continue; }
+            if (df != kRayMaxT)
+                push_far(__float_as_uint(sw ? a.w : c.w), __float_as_uint(sw ? b.w : d.w), rec * 6u + (sw ? 0u : 1u), df);
+            if (n_kind != 0 || status != kRun) { s.ref = n_ref; s.kind = n_kind; continue; }
+            const char* np = rp + (sw ? 128u : 64u);
+            const float4 e = gld4(np), f = gld4(np + 16), g = gld4(np + 32), h = gld4(np + 48);
+            const float dl2 = slab(e.x, e.y, e.z, f.x, f.y, f.z, s.cr, s.t);
+            const float dr2 = slab(g.x, g.y, g.z, h.x, h.y, h.z, s.cr, s.t);
+            const bool sw2 = dl2 > dr2;
+            const float dn2 = sw2 ? dr2 : dl2, df2 = sw2 ? dl2 : dr2;
+            if (dn2 == kRayMaxT) { pop_next(); continue; }
+            s.ref = __float_as_uint(sw2 ? g.w : e.w);
+            s.kind = __float_as_uint(sw2 ? h.w : f.w);
+            if (df2 != kRayMaxT)
+                push_far(__float_as_uint(sw2 ? e.w : g.w), __float_as_uint(sw2 ? f.w : h.w),
+                         rec * 6u + (sw ? 4u : 2u) + (sw2 ? 0u : 1u), df2);
+            continue;
+        }
         // interior node: near child next, far child pushed (swap only when
         // dist_left > dist_right)
         float4 a, b, c, d;
@@ -272,7 +313,7 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
         if (COUNT) { cnt[2]++; cnt[1]++; }
         const GpuInstance* in = sc.inst + s.ref;
         const float4 m0 = gld4(&in->m[0]), m1 = gld4(&in->m[4]), m2 = gld4(&in->m[8]);
-        const float4 pp = gld4(&in->nodes);
+        const float4 pp = L4 ? gld4(&in->quads) : gld4(&in->nodes);   // (nodes or 4-wide records, triangles)
         const uint64_t u0 = (uint64_t)__float_as_uint(pp.x) | ((uint64_t)__float_as_uint(pp.y) << 32);
         const uint64_t u1 = (uint64_t)__float_as_uint(pp.z) | ((uint64_t)__float_as_uint(pp.w) << 32);
         const char* bn = reinterpret_cast<const char*>(u0);

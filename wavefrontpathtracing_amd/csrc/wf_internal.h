@@ -27,13 +27,29 @@ struct alignas(16) GpuTri {
 static_assert(sizeof(GpuTri) == 48, "GpuTri is 48 B");
 
 // Instance record read at every TLAS leaf: rows 0..2 of world_to_local
-// (column-major, m[c*3+r]) and the mesh's device arrays.
+// (column-major, m[c*3+r]) and the mesh's device arrays: the BLAS nodes and
+// triangles as one 16-byte load, and the 4-wide records with the triangles as
+// another (null until the 4-wide layout is in use).
 struct alignas(16) GpuInstance {
     float m[12];
     const wf_bvh_node* nodes;
     const GpuTri* tris;
+    const char* quads;
+    const GpuTri* tris4;
 };
-static_assert(sizeof(GpuInstance) == 64, "GpuInstance is 64 B");
+static_assert(sizeof(GpuInstance) == 80, "GpuInstance is 80 B");
+
+// 4-wide records (traversal layout 1).  Record r of a hierarchy holds, for
+// the child pair r of the 2-wide layout (nodes 2r and 2r+1), those two
+// children followed by the children of the first and then of the second
+// (zeros where a child is a leaf): six 32-byte nodes, 192 B.  An interior
+// child's reference is its own record number (its left_first / 2 for a BLAS;
+// its pair-record number for the TLAS).  BLAS record 0 holds the root alone.
+// One visit decides the near child from the first pair and, when the near
+// child is interior, its near child from that child's pair in the same
+// record; every push and every box test is the 2-wide traversal's, at the
+// same ray.t, so results are identical.
+constexpr uint32_t kQuadBytes = 192;
 
 // Per-triangle shading data (triangle_t without tangents): positions,
 // normals and uvs of the three vertices.
@@ -73,6 +89,7 @@ struct TraceScene {
     const float4* tpairs;
     uint32_t n_tpairs;
     const GpuInstance* inst;
+    const float4* tquads;     // TLAS 4-wide records (layout 1), or null
 };
 
 // most TLAS records a workgroup copies to LDS (64 B each, dynamic LDS)
@@ -87,6 +104,7 @@ struct ExtendConfig {
     uint32_t refill_min = 16;   // idle lanes of a wave that trigger a refill
     uint32_t tlas_lds = 0;      // copy the TLAS records to LDS when they fit
     uint32_t visits = 4;        // interior visits per pass of the traversal loop
+    uint32_t layout = 0;        // 0: 2-wide pair records, 1: 4-wide records
 };
 
 // Launch the closest-hit kernel.  d_count (device) overrides n when non-null.

@@ -2,7 +2,8 @@
 
 Axes: workgroup size {64, 128, 256} x stack placement {LDS 4/8/12/16 entries
 per lane with global spill, private (scratch) stack} x popped-interior skip
-x refill threshold (finished lanes of a wave before it claims new rays).
+x node layout {2-wide pair records, 4-wide records} x refill threshold
+(finished lanes of a wave before it claims new rays).
 """
 from __future__ import annotations
 
@@ -11,11 +12,11 @@ import itertools
 
 from . import _lib as L
 
-DEFAULT = dict(block=64, lds_stack=16, skip_popped=1, refill_min=16, tlas_in_lds=0, interior_visits=4)
+DEFAULT = dict(block=64, lds_stack=16, skip_popped=1, bvh_layout=0, refill_min=16, tlas_in_lds=0, interior_visits=4)
 
 
 def set_extend_config(renderer, block: int = DEFAULT["block"], lds_stack: int = DEFAULT["lds_stack"],
-                      skip_popped: int = DEFAULT["skip_popped"], bvh_layout: int = 0, stack_limit: int = 0,
+                      skip_popped: int = DEFAULT["skip_popped"], bvh_layout: int = DEFAULT["bvh_layout"], stack_limit: int = 0,
                       refill_min: int = DEFAULT["refill_min"], tlas_in_lds: int = DEFAULT["tlas_in_lds"],
                       interior_visits: int = DEFAULT["interior_visits"]) -> None:
     c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout, stack_limit, refill_min, tlas_in_lds,
@@ -32,8 +33,12 @@ def get_extend_config(renderer) -> dict:
 
 
 def candidates():
-    for block, lds, skip in itertools.product((64, 128, 256), (0, 4, 8, 12, 16), (1, 0)):
-        yield dict(block=block, lds_stack=lds, skip_popped=skip)
+    """Every workgroup size x stack placement x skip x layout the kernels
+    support (the 4-wide layout runs with the popped-interior skip only)."""
+    for block, lds, skip, layout in itertools.product((64, 128, 256), (0, 4, 8, 12, 16), (1, 0), (0, 1)):
+        if layout == 1 and not skip:
+            continue
+        yield dict(block=block, lds_stack=lds, skip_popped=skip, bvh_layout=layout)
 
 
 def autotune(renderer, time_fn, configs=None, repeats: int = 3):
