@@ -30,7 +30,9 @@
 //     reference would test both children, find both missing, and pop again;
 //     the traversal counters still count those two child fetches;
 //   * triangles are pre-permuted into leaf order with edges precomputed
-//     (48-byte rows), instances are 64-byte records with 3x4 matrices.
+//     (48-byte rows), instances are 80-byte records with 3x4 matrices;
+//   * an optional 4-wide node layout (L4) visits two levels per step in the
+//     same order (see kQuadBytes).
 #include "traverse.h"
 
 #include <cfloat>
