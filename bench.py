@@ -435,6 +435,8 @@ def main():
     ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--no-inverse", action="store_true")
     ap.add_argument("--no-build", action="store_true")
+    ap.add_argument("--build-only", action="store_true",
+                    help="only the BLAS/TLAS build section (for profiling the builders); prints its JSON")
     ap.add_argument("--tune", action="store_true", help="autotune the traversal kernel on the batched frame")
     ap.add_argument("--sweep", action="store_true",
                     help="autotune the traversal kernel on the incoherent workload first (table on stderr)")
@@ -449,6 +451,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": src.numel() * 4, "launches": 5}))
+        return
+    if args.build_only:
+        print(json.dumps({"bvh_build": bench_build(cpu=not args.no_cpu)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the path tracer is a single-GPU program (asked for --gpus {args.gpus})",

@@ -111,21 +111,20 @@ __device__ __forceinline__ uint32_t bin_of(float c, float lo, float scale, uint3
 
 __device__ __forceinline__ float rec_get(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
-__device__ __forceinline__ uint32_t wave_min_u(uint32_t v)
-{
-    for (int o = 32; o > 0; o >>= 1) { const uint32_t w = __shfl_xor(v, o, 64); v = w < v ? w : v; }
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_max_u(uint32_t v)
-{
-    for (int o = 32; o > 0; o >>= 1) { const uint32_t w = __shfl_xor(v, o, 64); v = w > v ? w : v; }
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+// Wave-wide reductions and inclusive scans over all 64 lanes (every call
+// site runs with the whole wave active), from the ROCm device library: DPP
+// row operations instead of LDS permutes.
+extern "C" __device__ uint32_t __ockl_wfred_min_u32(uint32_t);
+extern "C" __device__ uint32_t __ockl_wfred_max_u32(uint32_t);
+extern "C" __device__ uint32_t __ockl_wfred_add_u32(uint32_t);
+extern "C" __device__ uint64_t __ockl_wfred_min_u64(uint64_t);
+extern "C" __device__ uint32_t __ockl_wfscan_add_u32(uint32_t, bool);
+extern "C" __device__ uint32_t __ockl_wfscan_min_u32(uint32_t, bool);
+extern "C" __device__ uint32_t __ockl_wfscan_max_u32(uint32_t, bool);
+
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) { return __ockl_wfred_min_u32(v); }
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t v) { return __ockl_wfred_max_u32(v); }
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) { return __ockl_wfred_add_u32(v); }
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m, uint32_t lane)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -160,16 +159,20 @@ __device__ __forceinline__ void block_scan2(bool f0, bool f1, uint32_t& ex0, uin
 
 // ---- per-triangle records and the root ------------------------------------------
 
+// Grid-stride over the triangles; the root bounds are reduced per lane, per
+// wave and per workgroup before one set of atomics per workgroup.
 __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ pos, uint32_t n, float4* rc, float4* rl, float4* rh,
                                               Counters* ctr)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    float cmn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, cmx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    __shared__ uint32_t s_k[4][14];
+    uint32_t keys[12];
+    for (int k = 0; k < 3; ++k) {
+        keys[k] = KEY_FMAX; keys[3 + k] = KEY_NFMAX; keys[6 + k] = KEY_FMAX; keys[9 + k] = KEY_NFMAX;
+    }
     uint32_t bad = 0, negz = 0;
-    if (t < n) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const float* p = pos + 9ull * t;
-        float c[3];
+        float mn[3], mx[3], c[3];
         for (int k = 0; k < 3; ++k) {
             const float a = p[k], b = p[3 + k], d = p[6 + k];
             if (!isfinite(a) || !isfinite(b) || !isfinite(d)) bad = 1;
@@ -178,31 +181,46 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ pos, uin
             mx[k] = gmax(gmax(a, b), d);
             c[k] = ((a + b) + d) * 0.3333f;
             if (__float_as_uint(mn[k]) == 0x80000000u || __float_as_uint(mx[k]) == 0x80000000u) negz = 1;
-            cmn[k] = c[k];
-            cmx[k] = c[k];
+            const uint32_t kl = fkey(mn[k]), kh = fkey(mx[k]), kc = fkey(c[k]);
+            keys[k] = kl < keys[k] ? kl : keys[k];
+            keys[3 + k] = kh > keys[3 + k] ? kh : keys[3 + k];
+            keys[6 + k] = kc < keys[6 + k] ? kc : keys[6 + k];
+            keys[9 + k] = kc > keys[9 + k] ? kc : keys[9 + k];
         }
         rc[t] = make_float4(c[0], c[1], c[2], __uint_as_float(t));
         rl[t] = make_float4(mn[0], mn[1], mn[2], 0.0f);
         rh[t] = make_float4(mx[0], mx[1], mx[2], 0.0f);
     }
-    uint32_t keys[12];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (int k = 0; k < 3; ++k) {
-        keys[k] = wave_min_u(fkey(mn[k]));
-        keys[3 + k] = wave_max_u(fkey(mx[k]));
-        keys[6 + k] = wave_min_u(fkey(cmn[k]));
-        keys[9 + k] = wave_max_u(fkey(cmx[k]));
+        keys[k] = wave_min_u(keys[k]);
+        keys[3 + k] = wave_max_u(keys[3 + k]);
+        keys[6 + k] = wave_min_u(keys[6 + k]);
+        keys[9 + k] = wave_max_u(keys[9 + k]);
     }
     bad = wave_max_u(bad);
     negz = wave_max_u(negz);
-    if ((threadIdx.x & 63u) == 0) {
+    if (lane == 0) {
+        for (int k = 0; k < 12; ++k) s_k[wave][k] = keys[k];
+        s_k[wave][12] = bad;
+        s_k[wave][13] = negz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / 64u; ++w)
+            for (int k = 0; k < 14; ++k) {
+                const uint32_t u = s_k[w][k];
+                const bool take_min = (k % 6) < 3 && k < 12;
+                s_k[0][k] = k >= 12 ? (s_k[0][k] | u) : (take_min ? (u < s_k[0][k] ? u : s_k[0][k]) : (u > s_k[0][k] ? u : s_k[0][k]));
+            }
         for (int k = 0; k < 3; ++k) {
-            atomicMin(&ctr->root_keys[k], keys[k]);
-            atomicMax(&ctr->root_keys[3 + k], keys[3 + k]);
-            atomicMin(&ctr->root_keys[6 + k], keys[6 + k]);
-            atomicMax(&ctr->root_keys[9 + k], keys[9 + k]);
+            atomicMin(&ctr->root_keys[k], s_k[0][k]);
+            atomicMax(&ctr->root_keys[3 + k], s_k[0][3 + k]);
+            atomicMin(&ctr->root_keys[6 + k], s_k[0][6 + k]);
+            atomicMax(&ctr->root_keys[9 + k], s_k[0][9 + k]);
         }
-        if (bad) atomicOr(&ctr->nonfinite, 1u);
-        if (negz) atomicOr(&ctr->negzero, 1u);
+        if (s_k[0][12]) atomicOr(&ctr->nonfinite, 1u);
+        if (s_k[0][13]) atomicOr(&ctr->negzero, 1u);
     }
 }
 
@@ -759,53 +777,54 @@ __global__ __launch_bounds__(kWG) void k_wave_build(const uint32_t* __restrict__
                     atomicMax(q + 4, fkey(vh.x)); atomicMax(q + 5, fkey(vh.y)); atomicMax(q + 6, fkey(vh.z));
                 }
                 wave_sync();
-                uint32_t pc = 0;
-                float pmn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, pmx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+                // lane j: bin j (prefix side) and bin B-1-j (suffix side), as
+                // order keys; inclusive scans give the left unions of the
+                // planes and, reversed, the right ones
+                uint32_t pc = 0, rc = 0;
+                uint32_t pk[6] = {KEY_FMAX, KEY_FMAX, KEY_FMAX, KEY_NFMAX, KEY_NFMAX, KEY_NFMAX};
+                uint32_t rk[6] = {KEY_FMAX, KEY_FMAX, KEY_FMAX, KEY_NFMAX, KEY_NFMAX, KEY_NFMAX};
                 if (lane < B) {
-                    pc = s_bins[wave][lane * 7u];
-                    for (int k = 0; k < 3; ++k) {
-                        pmn[k] = kfloat(s_bins[wave][lane * 7u + 1 + k]);
-                        pmx[k] = kfloat(s_bins[wave][lane * 7u + 4 + k]);
-                    }
+                    const uint32_t* bp = &s_bins[wave][lane * 7u];
+                    const uint32_t* bq = &s_bins[wave][(B - 1u - lane) * 7u];
+                    pc = bp[0];
+                    rc = bq[0];
+                    for (int k = 0; k < 6; ++k) { pk[k] = bp[1 + k]; rk[k] = bq[1 + k]; }
                 }
                 wave_sync();
-                uint32_t qc = pc;
-                float qmn[3] = {pmn[0], pmn[1], pmn[2]}, qmx[3] = {pmx[0], pmx[1], pmx[2]};
-                for (int o = 1; o < 64; o <<= 1) {
-                    const uint32_t tc = __shfl_up(pc, o, 64), uc = __shfl_down(qc, o, 64);
-                    float tmn[3], tmx[3], umn[3], umx[3];
-                    for (int k = 0; k < 3; ++k) {
-                        tmn[k] = __shfl_up(pmn[k], o, 64); tmx[k] = __shfl_up(pmx[k], o, 64);
-                        umn[k] = __shfl_down(qmn[k], o, 64); umx[k] = __shfl_down(qmx[k], o, 64);
-                    }
-                    if (lane >= (uint32_t)o) {
-                        pc += tc;
-                        for (int k = 0; k < 3; ++k) { pmn[k] = gmin(tmn[k], pmn[k]); pmx[k] = gmax(tmx[k], pmx[k]); }
-                    }
-                    if (lane + (uint32_t)o < 64u) {
-                        qc += uc;
-                        for (int k = 0; k < 3; ++k) { qmn[k] = gmin(qmn[k], umn[k]); qmx[k] = gmax(qmx[k], umx[k]); }
-                    }
+                pc = __ockl_wfscan_add_u32(pc, true);
+                rc = __ockl_wfscan_add_u32(rc, true);
+                for (int k = 0; k < 3; ++k) {
+                    pk[k] = __ockl_wfscan_min_u32(pk[k], true);
+                    pk[3 + k] = __ockl_wfscan_max_u32(pk[3 + k], true);
+                    rk[k] = __ockl_wfscan_min_u32(rk[k], true);
+                    rk[3 + k] = __ockl_wfscan_max_u32(rk[3 + k], true);
                 }
-                // right side of plane `lane`: bins lane+1 .. B-1
-                const uint32_t rs = __shfl_down(qc, 1, 64);
-                float rmn[3], rmx[3];
-                for (int k = 0; k < 3; ++k) { rmn[k] = __shfl_down(qmn[k], 1, 64); rmx[k] = __shfl_down(qmx[k], 1, 64); }
-                float cost = INFINITY;
-                uint32_t ci = kNone;
+                // right side of plane `lane` (bins lane+1 .. B-1) = reversed scan at lane B-2-lane
+                const int src = (int)B - 2 - (int)lane;
+                const int srcl = src >= 0 ? src : 0;
+                const uint32_t rs = __shfl(rc, srcl, 64);
+                float rmn[3], rmx[3], pmn[3], pmx[3];
+                for (int k = 0; k < 3; ++k) {
+                    rmn[k] = kfloat(__shfl(rk[k], srcl, 64));
+                    rmx[k] = kfloat(__shfl(rk[3 + k], srcl, 64));
+                    pmn[k] = kfloat(pk[k]);
+                    pmx[k] = kfloat(pk[3 + k]);
+                }
+                // the first plane of least cost (the reference's strict '<'
+                // scan): costs are >= 0, so (cost key, plane) orders as the
+                // scan does once -0 is made +0
+                uint64_t key = ~0ull;
                 if (lane + 1u < B) {
                     const float la = (float)pc * half_area(pmx[0] - pmn[0], pmx[1] - pmn[1], pmx[2] - pmn[2]);
                     const float ra = (float)rs * half_area(rmx[0] - rmn[0], rmx[1] - rmn[1], rmx[2] - rmn[2]);
                     const float cst = la + ra;
-                    if (cst < FLT_MAX) { cost = cst; ci = lane; }
+                    if (cst < FLT_MAX) key = ((uint64_t)fkey(cst + 0.0f) << 32) | lane;
                 }
-                // the first plane of least cost (the reference's strict '<' scan)
-                for (int o = 32; o > 0; o >>= 1) {
-                    const float oc = __shfl_xor(cost, o, 64);
-                    const uint32_t oi = __shfl_xor(ci, o, 64);
-                    if (oc < cost || (oc == cost && oi < ci)) { cost = oc; ci = oi; }
+                key = __ockl_wfred_min_u64(key);
+                if (key != ~0ull) {
+                    const float cost = kfloat((uint32_t)(key >> 32));
+                    if (cost < best) { best = cost; baxis = axis; bsplit = (uint32_t)key + 1u; }
                 }
-                if (ci != kNone && cost < best) { best = cost; baxis = axis; bsplit = ci + 1u; }
             }
             // subdivide_node's decisions (bvh_builder.cpp:105-117)
             bool leaf;
@@ -1066,7 +1085,7 @@ wf_status device_build_blas(const float* d_pos, uint32_t n, const wf_build_optio
     };
 
     k_ctr_init<<<1, 64, 0, st>>>(ctr);
-    k_prep<<<grid_for(n, 256, 0x7FFFFFFF), 256, 0, st>>>(d_pos, n, rc[0], rl[0], rh[0], ctr);
+    k_prep<<<grid_for(n, 256, 2048), 256, 0, st>>>(d_pos, n, rc[0], rl[0], rh[0], ctr);
     if (!read_ctr()) return WF_ERROR_DEVICE;
     if (hc.nonfinite) { err = "non-finite vertex position"; return WF_ERROR_INVALID_ARGUMENT; }
     const bool negzero = hc.negzero != 0;
