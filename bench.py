@@ -232,7 +232,9 @@ def bench_frame(frames: int):
     """One wavefront frame of the 256-instance scene at 1920x1080, 1 spp,
     depth 5, Russian roulette from bounce 3, next-event estimation to the 4
     emissive quads, black environment: stage times from the library's own
-    events (direct launches), then the same frame as a captured graph."""
+    events (direct launches; shadow rays traced in the next bounce's extend
+    launch, so ms_extend includes them), then the same frame as a captured
+    graph, with one launch per stage, and as one megakernel."""
     from wavefrontpathtracing_amd import Renderer, scenes
     sc = scenes.tlas_scene()
     r = Renderer(sc.width, sc.height)
@@ -255,8 +257,17 @@ def bench_frame(frames: int):
     for _ in range(frames):
         r.render()
         gt += r.frame_stats().ms_total
+    # one launch per stage (shadow rays in their own connect launch)
+    r.update_settings(use_hip_graph=0, separate_connect=1)
+    r.render()
+    sep = {k: 0.0 for k in ("ms_extend", "ms_connect", "ms_total")}
+    for _ in range(frames):
+        r.render()
+        st = r.frame_stats()
+        for k in sep:
+            sep[k] += getattr(st, k)
     # the same frame as one megakernel launch (use_wavefront_pathtracing = 0)
-    r.update_settings(use_hip_graph=0, use_wavefront_pathtracing=0)
+    r.update_settings(separate_connect=0, use_wavefront_pathtracing=0)
     r.render()
     mt = 0.0
     for _ in range(frames):
@@ -266,7 +277,8 @@ def bench_frame(frames: int):
     out = {k: round(v / frames, 3) for k, v in acc.items()}
     out.update({"workload": "256-instance scene, 1920x1080, 1 spp, depth 5, RR from 3, NEE, black environment",
                 "rays_per_frame": rays // frames, "mrays_per_s": round(rays / (acc["ms_total"] * 1e-3) / 1e6, 2),
-                "graph_ms_total": round(gt / frames, 3), "megakernel_ms_total": round(mt / frames, 3)})
+                "graph_ms_total": round(gt / frames, 3), "megakernel_ms_total": round(mt / frames, 3),
+                "separate_connect": {k: round(v / frames, 3) for k, v in sep.items()}})
     return out
 
 
@@ -338,8 +350,9 @@ def bench_batched(frames: int, tune: bool):
             st = r.frame_stats()
             prays += st.rays_extended + st.shadow_rays
     # the reference's frame structure: one scene, one sample per pixel per
-    # frame, frames accumulated until the scene has 4 samples
-    r.update_settings(spp=1, accumulate=1, frame_seed=0)
+    # frame, frames accumulated until the scene has 4 samples, one launch per
+    # stage (shadow rays in their own launch)
+    r.update_settings(spp=1, accumulate=1, frame_seed=0, separate_connect=1)
     rt, rrays = 0.0, 0
     for _ in range(frames):
         for k in range(n):
@@ -350,7 +363,7 @@ def bench_batched(frames: int, tune: bool):
                 st = r.frame_stats()
                 rrays += st.rays_extended + st.shadow_rays
     # the batch as one megakernel launch per frame
-    r.update_settings(spp=4, accumulate=0, frame_seed=9, use_wavefront_pathtracing=0)
+    r.update_settings(spp=4, accumulate=0, frame_seed=9, use_wavefront_pathtracing=0, separate_connect=0)
     batched_frame()
     mt, mrays = 0.0, 0
     for _ in range(frames):

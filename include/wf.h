@@ -115,6 +115,9 @@ typedef struct wf_render_settings {
     uint32_t next_event_estimation;
     uint32_t frame_seed;
     uint32_t use_hip_graph;      /* 1: the frame's launches replay as one captured hipGraph */
+    uint32_t separate_connect;   /* 1: shadow rays in their own launch after each shade (the reference's
+                                    one launch per stage); 0: traced in the next bounce's extend launch.
+                                    Same image either way. */
 } wf_render_settings;
 
 /* bvh_builder_t::build_options_t (bvh/bvh_builder.h:17-21); the reference

@@ -191,3 +191,31 @@ def test_quad_layout_frames_equal_pair_layout(wavefront):
     assert np.array_equal(out[0][0], out[1][0])
     for k in range(len(sl)):
         assert np.array_equal(out[0][1][k], out[1][1][k]), f"batch scene {k}"
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_fused_connect_equals_separate_connect(batch):
+    """Shadow rays traced in the next bounce's extend launch (the default)
+    give the image of one launch per stage, bit for bit, with the same ray
+    counts; for one scene and for a batch."""
+    out = []
+    for sep in (1, 0):
+        r = Renderer(64, 64)
+        try:
+            if batch:
+                sl = scenes.batch_scenes(seed=35, n_scenes=4, tri_min=500, tri_max=20_000, resolutions=(24, 40))
+                scenes.load_batch(r, sl)
+            else:
+                sc = scenes.small_tlas_scene()
+                scenes.load(r, sc)
+            r.update_settings(max_bounces=5, spp=2, next_event_estimation=1, hdr_env_strength=0.2, frame_seed=7,
+                              accumulate=0, separate_connect=sep)
+            r.render()
+            st = r.frame_stats()
+            img = ([r.read_scene_energy(k, s.width, s.height) for k, s in enumerate(sl)] if batch else [r.read_energy()])
+            out.append((img, st.rays_extended, st.shadow_rays))
+        finally:
+            r.exit()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert out[0][1:] == out[1][1:]

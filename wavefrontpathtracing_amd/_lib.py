@@ -75,7 +75,7 @@ class RenderSettings(C.Structure):
                 ("max_bounces", C.c_uint32), ("cosine_weighted_diffuse", C.c_uint32), ("accumulate", C.c_uint32),
                 ("hdr_env_strength", C.c_float), ("spp", C.c_uint32), ("rr_start_bounce", C.c_uint32),
                 ("rr_survival", C.c_float), ("next_event_estimation", C.c_uint32), ("frame_seed", C.c_uint32),
-                ("use_hip_graph", C.c_uint32)]
+                ("use_hip_graph", C.c_uint32), ("separate_connect", C.c_uint32)]
 
 
 class BuildOptions(C.Structure):

@@ -279,15 +279,27 @@ wf_status wf_renderer::record_frame(const FrameArgs& A, uint32_t grid, bool time
     if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
     if ((e = launch_generate(A, grid, stream)) != hipSuccess) return hip_fail(e, "generate launch");
     if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
+    // The shadow rays of bounce d-1 are traced in the launch that extends
+    // bounce d (one launch per bounce instead of two half-full ones), and
+    // their contributions are added before bounce d is shaded: every path
+    // sees the same operations in the same order as with separate launches.
+    // separate_connect keeps the reference's one launch per stage.
+    const bool batched = slots.size() > 1;
+    // (the fused kernels exist for the popped-interior skip with the TLAS in global memory)
+    const bool fuse = settings.next_event_estimation && !settings.separate_connect && ext_cfg.skip_popped && ext_grid_tlds == 0;
     for (uint32_t d = 0; d <= settings.max_bounces; ++d) {
-        const bool batched = slots.size() > 1;
+        const bool with_shadow = fuse && d > 0;
+        OcclusionQueue q{A.shadow_rays, A.counts + kShadowCountBase + (d > 0 ? d - 1 : 0), A.shadow_visible,
+                         batched ? A.shadow_root : nullptr};
         s = run_extend(A.rays[d & 1u], A.counts + d, A.n_paths, A.hits, nullptr, nullptr, false,
-                       batched ? A.root_of[d & 1u] : nullptr);
+                       batched ? A.root_of[d & 1u] : nullptr, with_shadow ? &q : nullptr);
         if (s != WF_OK) return s;
+        if (with_shadow && (e = launch_connect_accumulate(A, d - 1, grid, stream)) != hipSuccess)
+            return hip_fail(e, "connect launch");
         if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
         if ((e = launch_shade(A, d, grid, stream)) != hipSuccess) return hip_fail(e, "shade launch");
         if ((e = mark()) != hipSuccess) return hip_fail(e, "event");
-        if (settings.next_event_estimation && d < settings.max_bounces) {
+        if (settings.next_event_estimation && !fuse && d < settings.max_bounces) {
             s = run_extend(A.shadow_rays, A.counts + kShadowCountBase + d, A.n_paths, nullptr, A.shadow_visible, nullptr, true,
                            batched ? A.shadow_root : nullptr);
             if (s != WF_OK) return s;
@@ -411,7 +423,7 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
             settings.accumulate, ext_cfg.block, ext_cfg.lds_stack, ext_cfg.skip_popped, ext_grid,
             (uint64_t)(uintptr_t)d_tnodes.p, (uint64_t)(uintptr_t)d_tpairs.p, ext_cfg.refill_min, ext_cfg.tlas_lds, ext_cfg.visits, ext_grid_tlds, n_tpairs, (uint64_t)(uintptr_t)d_inst.p, (uint64_t)(uintptr_t)d_shade_inst.p,
             (uint64_t)(uintptr_t)d_emitters.p, (uint64_t)(uintptr_t)d_textures.p, (uint64_t)(uintptr_t)d_spill.p,
-            (uint64_t)(uintptr_t)d_rays[0].p, (uint64_t)(uintptr_t)d_image.p, slots.size(), ext_cfg.layout,
+            (uint64_t)(uintptr_t)d_rays[0].p, (uint64_t)(uintptr_t)d_image.p, slots.size(), ext_cfg.layout, settings.separate_connect,
             (uint64_t)(uintptr_t)d_tquads.p};
         if (!graph_exec || key != graph_key) {
             drop_graph();

@@ -51,7 +51,7 @@ using namespace trav;
 // from the global cursor with one atomic (ballot + mbcnt give each idle lane
 // its index).  The loop ends when every lane is idle and the queue is
 // drained.
-template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4, class Stack>
+template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4, bool MIX, class Stack>
 __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, const float4* s_tl, uint32_t lane, uint32_t n)
 {
     const TraceScene sc = L.scene;
@@ -61,6 +61,10 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
     uint32_t cnt[4] = {0, 0, 0, 0};
     uint32_t ray_i = ~0u;
     bool drained = false;
+    // MIX: a second queue of occlusion rays follows the first one
+    const uint32_t n2 = MIX ? (L.d_count2 ? *L.d_count2 : L.n2) : 0u;
+    const uint64_t ntot = (uint64_t)n + n2;
+    bool lane_any = ANY;
 
     for (;;) {
         const uint64_t idle = __ballot(ray_i == ~0u);
@@ -72,16 +76,25 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(&L.work[0], n_idle);
             base = __shfl(base, (int)leader, 64);
-            if ((uint64_t)base + n_idle >= (uint64_t)n) drained = true;
+            if ((uint64_t)base + n_idle >= ntot) drained = true;
             if (ray_i != ~0u) continue;
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-            const uint32_t i = base + rank;
-            if (i >= n) continue;
-            ray_i = i;
-            const uint32_t root = L.ray_root ? L.ray_root[i] : 0u;
-            if (!trace_begin<COUNT, L4>(s, sc, L.rays[i], root, cnt)) status = kDone;
+            const uint32_t q = base + rank;
+            if ((uint64_t)q >= ntot) continue;
+            if constexpr (MIX) {
+                const bool second = q >= n;
+                const uint32_t i = second ? q - n : q;
+                ray_i = i;
+                lane_any = second;
+                const uint32_t root = second ? (L.ray_root2 ? L.ray_root2[i] : 0u) : (L.ray_root ? L.ray_root[i] : 0u);
+                if (!trace_begin<COUNT, L4>(s, sc, second ? L.rays2[i] : L.rays[i], root, cnt)) status = kDone;
+            } else {
+                ray_i = q;
+                const uint32_t root = L.ray_root ? L.ray_root[q] : 0u;
+                if (!trace_begin<COUNT, L4>(s, sc, L.rays[q], root, cnt)) status = kDone;
+            }
         } else if (ray_i != ~0u) {
-            status = trace_pass<SKIP, COUNT, TLDS, L4>(s, sc, st, s_tl, visits, ANY, cnt);
+            status = trace_pass<SKIP, COUNT, TLDS, L4>(s, sc, st, s_tl, visits, MIX ? lane_any : ANY, cnt);
         }
         if (status != kRun) {
             if (status == kOverflow) {
@@ -90,6 +103,8 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
             }
             if (ANY) {
                 L.visible[ray_i] = (s.h_inst == ~0u && status != kOverflow) ? 1u : 0u;
+            } else if (MIX && lane_any) {
+                L.visible2[ray_i] = (s.h_inst == ~0u && status != kOverflow) ? 1u : 0u;
             } else {
                 wf_hit o;
                 o.instance_idx = s.h_inst; o.primitive_idx = s.h_prim;
@@ -108,7 +123,7 @@ extern __shared__ float4 s_tlas_dyn[];
 // TLDS: the workgroup first copies the TLAS records (at most
 // kTlasLdsPairs, in dynamic LDS sized by the launch) and reads TLAS nodes
 // from there.
-template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4 = false>
+template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4 = false, bool MIX = false>
 __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
 {
     constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
@@ -126,11 +141,11 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
     if constexpr (LDS > 0) {
         LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + gtid, L.spill_per_thread, gridDim.x * BLOCK,
                                 L.stack_cap};
-        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4>(L, st, s_tl, tid & 63u, n);
+        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4, MIX>(L, st, s_tl, tid & 63u, n);
     } else {
         PrivStack<128> st;
         st.cap = L.stack_cap;
-        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4>(L, st, s_tl, tid & 63u, n);
+        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4, MIX>(L, st, s_tl, tid & 63u, n);
     }
 }
 
@@ -177,8 +192,26 @@ void* pick_quad(uint32_t block, uint32_t lds)
     return nullptr;
 }
 
-void* pick_kernel(const ExtendConfig& c, bool count, bool any, bool tlds)
+// closest-hit launches with a trailing occlusion queue: popped-interior skip
+// on, no counters, TLAS in global memory, either layout
+template <bool L4>
+void* pick_mix(uint32_t block, uint32_t lds)
 {
+#define WF_CASE(B, S) \
+    if (block == B && lds == S) return reinterpret_cast<void*>(&k_extend<B, S, true, false, false, false, L4, true>);
+    WF_CASE(64, 0) WF_CASE(64, 4) WF_CASE(64, 8) WF_CASE(64, 12) WF_CASE(64, 16)
+    WF_CASE(128, 0) WF_CASE(128, 4) WF_CASE(128, 8) WF_CASE(128, 12) WF_CASE(128, 16)
+    WF_CASE(256, 0) WF_CASE(256, 4) WF_CASE(256, 8) WF_CASE(256, 12) WF_CASE(256, 16)
+#undef WF_CASE
+    return nullptr;
+}
+
+void* pick_kernel(const ExtendConfig& c, bool count, bool any, bool tlds, bool mix = false)
+{
+    if (mix) {
+        if (count || any || tlds || !c.skip_popped) return nullptr;
+        return c.layout == 1 ? pick_mix<true>(c.block, c.lds_stack) : pick_mix<false>(c.block, c.lds_stack);
+    }
     if (c.layout == 1) {
         if (count || tlds || !c.skip_popped) return nullptr;
         return any ? pick_quad<true>(c.block, c.lds_stack) : pick_quad<false>(c.block, c.lds_stack);
@@ -200,7 +233,7 @@ hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStre
     const bool tlds = L.grid_tlds != 0;   // the host sets it when the TLAS records fit
     ExtendConfig c = cfg;
     if (L.counters) c.layout = 0;         // traversal counters come from the 2-wide kernels
-    void* k = pick_kernel(c, L.counters != nullptr, L.any_hit != 0, tlds);
+    void* k = pick_kernel(c, L.counters != nullptr, L.any_hit != 0, tlds, L.rays2 != nullptr);
     if (!k) return hipErrorInvalidValue;
     ExtendLaunch arg = L;
     void* args[] = {&arg};

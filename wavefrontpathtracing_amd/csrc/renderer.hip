@@ -89,6 +89,7 @@ static wf_render_settings default_settings()
     s.next_event_estimation = 0;
     s.frame_seed = 0;
     s.use_hip_graph = 0;
+    s.separate_connect = 0;
     return s;
 }
 
@@ -650,8 +651,10 @@ extern "C" WF_API wf_status wf_renderer_get_scene_bounds(const wf_renderer* r, f
 // ---------------------------------------------------------------------------
 
 wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
-                                  uint32_t* d_visible, uint32_t* d_counters, bool any, const uint32_t* d_roots)
+                                  uint32_t* d_visible, uint32_t* d_counters, bool any, const uint32_t* d_roots,
+                                  const OcclusionQueue* second)
 {
+    if (second && (any || d_counters)) return fail(WF_ERROR_INVALID_ARGUMENT, "a second queue needs a closest-hit launch");
     hipError_t e = launch_fill_u32((uint32_t*)d_work.p, 0u, 1, stream);   // work cursor; word 1 counts overflows
     if (e != hipSuccess) return hip_fail(e, "work counter reset");
     ExtendLaunch L{};
@@ -676,6 +679,13 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     L.any_hit = any ? 1 : 0;
     L.refill_min = ext_cfg.refill_min;
     L.visits = ext_cfg.visits;
+    if (second) {
+        L.rays2 = second->rays;
+        L.ray_root2 = second->roots;
+        L.d_count2 = second->d_count;
+        L.n2 = 0;
+        L.visible2 = second->visible;
+    }
     e = launch_extend(L, ext_cfg, stream);
     if (e != hipSuccess) return hip_fail(e, "extend launch");
     return WF_OK;

@@ -126,8 +126,16 @@ struct wf_renderer {
     wf_status update_tlds_grid();
     wf_status upload_scene();
     wf_status upload_materials_and_emitters();
+    // an occlusion queue traced in the same launch after a closest-hit queue
+    struct OcclusionQueue {
+        const wf_ray* rays;
+        const uint32_t* d_count;
+        uint32_t* visible;
+        const uint32_t* roots;
+    };
     wf_status run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
-                         uint32_t* d_visible, uint32_t* d_counters, bool any, const uint32_t* d_roots = nullptr);
+                         uint32_t* d_visible, uint32_t* d_counters, bool any, const uint32_t* d_roots = nullptr,
+                         const OcclusionQueue* second = nullptr);
     wf_status read_overflows(uint32_t* out);
     wf_status reset_overflows();
     wf_status ensure_frame_buffers(size_t n_paths, size_t n_pixels);

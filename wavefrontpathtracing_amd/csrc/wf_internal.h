@@ -126,6 +126,13 @@ struct ExtendLaunch {
     int any_hit;
     uint32_t refill_min;       // idle lanes of a wave that trigger a refill (1..64)
     uint32_t visits;           // interior visits per pass of the traversal loop (1..8)
+    // optional second queue, traced for occlusion after the first (closest-hit
+    // launches only): rays2[0 .. *d_count2 or n2) with their roots, visible2 out
+    const wf_ray* rays2;
+    const uint32_t* ray_root2;
+    const uint32_t* d_count2;
+    uint32_t n2;
+    uint32_t* visible2;
 };
 hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStream_t stream);
 hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, size_t dyn_lds_bytes, int* blocks_per_cu);
