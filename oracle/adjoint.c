@@ -66,7 +66,12 @@ static void eval_path(const orc_shading* sh, const orc_render_params* p, const o
         const orc_vertex* v = &rec[d];
         if (v->kind == ORC_V_END) break;
         if (v->kind == ORC_V_MISS) {
-            for (int k = 0; k < 3; ++k) e[k] += thr[k] * (double)sh->env[k];
+            if (sh->env_tex >= 0) {
+                const float* c = sh->tex_rgba + sh->tex_off[sh->env_tex] + 4 * (size_t)v->texel[0];
+                for (int k = 0; k < 3; ++k) e[k] += thr[k] * (double)sh->env_strength * (double)c[k];
+            } else {
+                for (int k = 0; k < 3; ++k) e[k] += thr[k] * (double)sh->env[k];
+            }
             break;
         }
         if (v->kind == ORC_V_EMIT) {

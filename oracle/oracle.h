@@ -122,7 +122,14 @@ typedef struct {
     const uint32_t* tex_w;
     const uint32_t* tex_h;
     float           env[3];        /* constant environment radiance (hdr_env_strength x white) */
+    int32_t         env_tex;       /* equirect environment texture (sample.hlsl:4-11), -1: the constant env */
+    float           env_strength;  /* hdr_env_strength, with env_tex */
 } orc_shading;
+
+/* Texel index of the environment texture in direction d: uv = (atan2(z, x),
+ * asin(-y)) * (0.1591, 0.3183) + 0.5 (direction_to_equirect_uv,
+ * sample.hlsl:4-11), point load at uint2(uv * size) (shade.hlsl:36-40). */
+uint32_t orc_env_texel(const orc_shading* sh, const float d[3]);
 
 typedef struct {
     uint32_t max_bounces;

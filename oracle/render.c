@@ -257,6 +257,18 @@ static void primary_ray(const orc_view* v, uint32_t px, uint32_t py, orc_ray* r)
 
 /* One path; rec (max_bounces + 1 entries, may be NULL) receives the vertex
  * records for the FP64 evaluator. */
+uint32_t orc_env_texel(const orc_shading* sh, const float d[3])
+{
+    const uint32_t w = sh->tex_w[sh->env_tex], h = sh->tex_h[sh->env_tex];
+    const float u = atan2f(d[2], d[0]) * 0.1591f + 0.5f;
+    const float v = asinf(-d[1]) * 0.3183f + 0.5f;
+    const float fx = u * (float)w, fy = v * (float)h;
+    uint32_t x = fx >= 1.0f ? (uint32_t)fx : 0u, y = fy >= 1.0f ? (uint32_t)fy : 0u;
+    if (x >= w) x = w - 1;
+    if (y >= h) y = h - 1;
+    return y * w + x;
+}
+
 static int trace_path(render_job* J, uint32_t path, float energy[3], orc_vertex* rec)
 {
     const orc_render_params* P = J->p;
@@ -277,7 +289,15 @@ static int trace_path(render_job* J, uint32_t path, float energy[3], orc_vertex*
         float u[6];
         orc_path_rng(path, d, P->frame_seed, u);
         if (hit.instance_idx == ~0u) {
-            for (int k = 0; k < 3; ++k) energy[k] = energy[k] + thr[k] * sh->env[k];
+            if (sh->env_tex >= 0) {
+                /* throughput * hdr_env_strength * env (shade.hlsl:62-63) */
+                const uint32_t ti = orc_env_texel(sh, ray.dir);
+                const float* c = sh->tex_rgba + sh->tex_off[sh->env_tex] + 4 * (size_t)ti;
+                for (int k = 0; k < 3; ++k) energy[k] = energy[k] + (thr[k] * sh->env_strength) * c[k];
+                if (vr) vr->texel[0] = ti;
+            } else {
+                for (int k = 0; k < 3; ++k) energy[k] = energy[k] + thr[k] * sh->env[k];
+            }
             if (vr) vr->kind = ORC_V_MISS;
             break;
         }

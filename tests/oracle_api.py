@@ -42,7 +42,7 @@ class OrcMaterialInstance(C.Structure):
 class OrcShading(C.Structure):
     _fields_ = [("shade_tris", C.c_void_p), ("shade_off", C.c_void_p), ("tri_count", C.c_void_p), ("mi", C.c_void_p),
                 ("n_inst", C.c_uint32), ("tex_rgba", C.c_void_p), ("tex_off", C.c_void_p), ("tex_w", C.c_void_p),
-                ("tex_h", C.c_void_p), ("env", C.c_float * 3)]
+                ("tex_h", C.c_void_p), ("env", C.c_float * 3), ("env_tex", C.c_int32), ("env_strength", C.c_float)]
 
 
 class OrcRenderParams(C.Structure):
@@ -85,6 +85,7 @@ def olib():
                                            C.POINTER(OrcRenderParams), vp, C.c_int]),
             "orc_eval_f64": (C.c_int, [C.POINTER(OrcShading), C.POINTER(OrcRenderParams), u32, u32, vp, C.c_int32,
                                        vp, vp, vp, vp]),
+            "orc_env_texel": (u32, [C.POINTER(OrcShading), vp]),
             "orc_adjoint_f64": (C.c_int, [C.POINTER(OrcShading), C.POINTER(OrcRenderParams), u32, u32, vp, C.c_int32,
                                           vp, vp, vp, vp, vp, vp]),
         }
@@ -252,6 +253,8 @@ class OracleScene:
         sh = OrcShading(p(self._sh_tris), p(self._sh_off), p(self._sh_cnt), C.cast(mi, C.c_void_p), len(sc.instances),
                         p(self._tex), p(self._toff), p(self._tw), p(self._th))
         sh.env[:] = [float(np.float32(env))] * 3
+        sh.env_tex = getattr(sc, "env_texture", -1)
+        sh.env_strength = float(np.float32(env))
         self._sh = sh
         return sh
 
@@ -325,6 +328,12 @@ class Differentiable:
                                     p(self.factor3), p(strength), p(dl), p(d_tex), p(d_str))
         assert rc == 0
         return d_tex, d_str
+
+
+def env_texel(sh, d) -> int:
+    """orc_env_texel for one direction (the equirect lookup of a miss)."""
+    dd = np.ascontiguousarray(d, dtype=np.float32)
+    return int(olib().orc_env_texel(C.byref(sh), p(dd)))
 
 
 def hits_equal(a: np.ndarray, b: np.ndarray):

@@ -99,3 +99,62 @@ def test_render_is_deterministic():
     a = orc.render(max_bounces=5, spp=2, nee=1, env=0.3, nthreads=1)
     b = orc.render(max_bounces=5, spp=2, nee=1, env=0.3, nthreads=8)
     assert np.array_equal(a, b)
+
+
+def _env_scene(w_env=64, h_env=32, const=None):
+    """furnace_scene with an equirect environment texture: either a constant
+    value or seeded noise."""
+    sc = furnace_scene()
+    if const is not None:
+        rgb = np.full((h_env, w_env, 3), const, np.float32)
+    else:
+        rgb = scenes.value_noise_texture(3, w_env, h_env, cells=4, lo=0.2, hi=2.0)
+    sc.textures = [(w_env, h_env, rgb)]
+    sc.env_texture = 0
+    return sc
+
+
+def test_env_texel_of_axis_directions():
+    """direction_to_equirect_uv (sample.hlsl:4-11): +x looks at the centre of
+    the map, -x at its last column (atan2 = pi, and 0.1591 < 1/(2 pi) keeps
+    u < 1), straight up at row 0 and straight down at the last row
+    (asin(-y) = -+pi/2); +z and -z are mirror columns about the centre."""
+    w, h = 64, 32
+    orc = O.OracleScene(_env_scene(w, h))
+    sh = orc.shading(1.0)
+    col = lambda d: O.env_texel(sh, d) % w
+    row = lambda d: O.env_texel(sh, d) // w
+    assert (col([1, 0, 0]), row([1, 0, 0])) == (w // 2, h // 2)
+    assert col([-1, 0, 0]) == w - 1
+    assert row([0, 1, 0]) == 0 and row([0, -1, 0]) == h - 1
+    assert col([0, 0, 1]) == 47 and col([0, 0, -1]) == 16          # u = 0.5 +- pi/2 * 0.1591
+    assert col([0.6, 0, 0.8]) + col([0.6, 0, -0.8]) in (w - 1, w)   # mirror about the centre column
+    assert row([0, 0.5, 0.8660254]) < h // 2 < row([0, -0.5, 0.8660254])
+
+
+def test_constant_env_texture_equals_constant_env():
+    """A constant environment texture E and hdr_env_strength s give the image
+    of the constant environment s*E bit for bit ((thr * s) * 1 = thr * s)."""
+    a = O.OracleScene(_env_scene(const=1.0)).render(max_bounces=3, spp=2, nee=1, env=0.75)
+    sc = furnace_scene()
+    b = O.OracleScene(sc).render(max_bounces=3, spp=2, nee=1, env=0.75)
+    assert np.array_equal(a, b)
+
+
+def test_env_texture_lights_the_furnace_by_direction():
+    """With a noise environment, camera rays that miss return the texel the
+    equirect map assigns to their direction times the strength."""
+    sc = _env_scene()
+    orc = O.OracleScene(sc)
+    img = orc.render(max_bounces=2, spp=1, nee=0, env=2.0)
+    view = O.camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
+    rays = O.primary_rays(view, sc.width, sc.height)
+    hits = orc.trace_closest(rays)
+    sh = orc.shading(2.0)
+    rgb = sc.textures[0][2].reshape(-1, 3)
+    miss = np.nonzero(hits["instance_idx"] == L.MISS)[0]
+    assert len(miss) > 100
+    for k in miss[::17]:
+        t = O.env_texel(sh, rays["direction"][k])
+        expect = np.float32(2.0) * rgb[t]
+        assert np.array_equal(img.reshape(-1, 4)[k, :3], expect), k

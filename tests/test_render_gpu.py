@@ -219,3 +219,51 @@ def test_fused_connect_equals_separate_connect(batch):
     for a, b in zip(out[0][0], out[1][0]):
         assert np.array_equal(a, b)
     assert out[0][1:] == out[1][1:]
+
+
+def _env_tlas_scene(textured=False):
+    """The small TLAS scene under a seeded-noise equirect environment (and,
+    if textured, a 32x32 albedo texture on every non-emissive instance)."""
+    sc = scenes.small_tlas_scene(width=96, height=64)
+    sc.eye = (0.0, 6.0, -16.0)
+    env = scenes.value_noise_texture(9, 64, 32, cells=4, lo=0.1, hi=3.0)
+    if textured:
+        sc.textures = [(32, 32, scenes.value_noise_texture(5, 32, 32, cells=8, lo=0.2, hi=0.8)), (64, 32, env)]
+        for ins in sc.instances:
+            if not any(ins.emissive):
+                ins.texture = 0
+        sc.env_texture = 1
+    else:
+        sc.textures = [(64, 32, env)]
+        sc.env_texture = 0
+    return sc
+
+
+@pytest.mark.parametrize("nee", [0, 1])
+def test_env_texture_frame_matches_oracle(nee):
+    """Misses read the equirect environment texture (sample_hdr_env,
+    shade.hlsl:36-40, 62-63): wavefront and megakernel frames match the
+    oracle within 1e-3 relative L1 (atan2/asin may round to a neighbouring
+    texel at texel boundaries), and each other bit for bit."""
+    sc = _env_tlas_scene()
+    orc = O.OracleScene(sc)
+    ref = orc.render(max_bounces=4, spp=2, nee=nee, env=0.8, frame_seed=3, nthreads=8)[..., :3].astype(np.float64)
+    imgs = []
+    for wavefront in (1, 0):
+        r = Renderer(sc.width, sc.height)
+        try:
+            scenes.load(r, sc)
+            r.update_settings(max_bounces=4, spp=2, next_event_estimation=nee, hdr_env_strength=0.8, frame_seed=3,
+                              accumulate=0, use_wavefront_pathtracing=wavefront)
+            r.render()
+            imgs.append(r.read_energy())
+        finally:
+            r.exit()
+    img = imgs[0][..., :3].astype(np.float64)
+    err = np.abs(img - ref).sum() / np.abs(ref).sum()
+    assert err <= 1e-3, err
+    assert np.array_equal(imgs[0], imgs[1])
+    # the texture is really sampled: the constant environment gives another image
+    flat = scenes.Scene(**{**sc.__dict__, "env_texture": -1})
+    assert np.abs(ref - O.OracleScene(flat).render(max_bounces=4, spp=2, nee=nee, env=0.8, frame_seed=3,
+                                                   nthreads=8)[..., :3]).max() > 0.05

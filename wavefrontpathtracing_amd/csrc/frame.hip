@@ -321,8 +321,6 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
     if (slots.empty()) return fail(WF_ERROR_INVALID_STATE, "render called before begin_scene");
     if (count == 0 || first >= slots.size() || count > slots.size() - first)
         return fail(WF_ERROR_INVALID_ARGUMENT, "scene range [%u, %u) outside [0, %zu)", first, first + count, slots.size());
-    if (env_texture != WF_INVALID_HANDLE && find_texture(env_texture))
-        return fail(WF_ERROR_INVALID_ARGUMENT, "environment textures are not supported (only the constant default)");
     wf_status s = upload_scene();
     if (s != WF_OK) return s;
 
@@ -385,6 +383,10 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
     A.nee = settings.next_event_estimation;
     A.rr_survival = settings.rr_survival;
     for (int k = 0; k < 3; ++k) A.env[k] = settings.hdr_env_strength;   // default 1x1 white environment
+    // begin_scene's environment texture; a missing one falls back to the
+    // white default as the reference does (renderer.cpp:605-610)
+    A.env_tex = (env_texture != WF_INVALID_HANDLE && find_texture(env_texture)) ? (int32_t)(env_texture & 0xFFFFFFFFu) : -1;
+    A.env_strength = settings.hdr_env_strength;
     A.accumulate = settings.accumulate;
     A.dyn = (const FrameDyn*)d_dyn.p;
     A.thr = (float4*)d_thr.p;
@@ -423,7 +425,7 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
             settings.accumulate, ext_cfg.block, ext_cfg.lds_stack, ext_cfg.skip_popped, ext_grid,
             (uint64_t)(uintptr_t)d_tnodes.p, (uint64_t)(uintptr_t)d_tpairs.p, ext_cfg.refill_min, ext_cfg.tlas_lds, ext_cfg.visits, ext_grid_tlds, n_tpairs, (uint64_t)(uintptr_t)d_inst.p, (uint64_t)(uintptr_t)d_shade_inst.p,
             (uint64_t)(uintptr_t)d_emitters.p, (uint64_t)(uintptr_t)d_textures.p, (uint64_t)(uintptr_t)d_spill.p,
-            (uint64_t)(uintptr_t)d_rays[0].p, (uint64_t)(uintptr_t)d_image.p, slots.size(), ext_cfg.layout, settings.separate_connect,
+            (uint64_t)(uintptr_t)d_rays[0].p, (uint64_t)(uintptr_t)d_image.p, slots.size(), ext_cfg.layout, settings.separate_connect, (uint64_t)(uint32_t)A.env_tex,
             (uint64_t)(uintptr_t)d_tquads.p};
         if (!graph_exec || key != graph_key) {
             drop_graph();

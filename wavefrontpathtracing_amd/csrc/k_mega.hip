@@ -39,6 +39,7 @@ __device__ __forceinline__ void mega_loop(const FrameArgs& A, const MegaLaunch& 
     float4 contrib = make_float4(0, 0, 0, 0);
     bool has_next = false;
     V3 next_o{0, 0, 0}, next_d{0, 0, 0};
+    V3 cur_d{0, 0, 0};                // direction of the lane's current closest-hit ray
     uint32_t n_closest = 0, n_shadow = 0;
     bool drained = false;
 
@@ -47,7 +48,7 @@ __device__ __forceinline__ void mega_loop(const FrameArgs& A, const MegaLaunch& 
         r.origin[0] = o.x; r.origin[1] = o.y; r.origin[2] = o.z; r.tmin = kRayMinT;
         r.direction[0] = dir.x; r.direction[1] = dir.y; r.direction[2] = dir.z; r.tmax = tmax;
         any = shadow;
-        if (shadow) ++n_shadow; else ++n_closest;
+        if (shadow) ++n_shadow; else { ++n_closest; cur_d = dir; }
         return trace_begin<false, L4>(s, sc, r, root, cnt_dummy);
     };
 
@@ -97,7 +98,7 @@ __device__ __forceinline__ void mega_loop(const FrameArgs& A, const MegaLaunch& 
                 hit.t = s.h_inst == ~0u ? kRayMaxT : s.t;
                 hit.bary[0] = s.h_u; hit.bary[1] = s.h_v;
                 VertexOut o;
-                shade_vertex<false>(A, p, scene, d, seed, hit, thr, e4, make_float4(0, 0, 0, 0), o);
+                shade_vertex<false>(A, p, scene, d, seed, hit, cur_d, thr, e4, make_float4(0, 0, 0, 0), o);
                 has_next = o.spawn;
                 next_o = o.new_o;
                 next_d = o.new_d;

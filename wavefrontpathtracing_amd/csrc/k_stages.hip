@@ -86,8 +86,13 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
             float4 e4 = ADJ ? A.lrem[p] : A.energy[p];
             const float4 dlt = ADJ ? A.delta[p] : make_float4(0, 0, 0, 0);
             V3 thr{t4.x, t4.y, t4.z};
+            V3 dir{0.0f, 0.0f, 0.0f};
+            if (hit.instance_idx == 0xFFFFFFFFu && A.env_tex >= 0) {
+                const wf_ray& r = A.rays[cur][i];
+                dir = V3{r.direction[0], r.direction[1], r.direction[2]};
+            }
             VertexOut o;
-            shade_vertex<ADJ>(A, p, A.path_scene[p], d, seed, hit, thr, e4, dlt, o);
+            shade_vertex<ADJ>(A, p, A.path_scene[p], d, seed, hit, dir, thr, e4, dlt, o);
             spawn = o.spawn; shadow = o.shadow;
             new_o = o.new_o; new_d = o.new_d; sh_o = o.sh_o; sh_d = o.sh_d; sh_tmax = o.sh_tmax;
             contrib = o.contrib; unit = o.unit; light = o.light;

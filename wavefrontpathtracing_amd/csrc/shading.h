@@ -259,22 +259,53 @@ struct VertexOut {
 // dEstimate/dalbedo_d = lrem / albedo_d at vertex d, scattered to the
 // texture's bilinear corners (dlt = dLoss/dEstimate); emission gradients are
 // the collected contributions per unit emission strength.
+// Texel of an equirect environment in direction d (sample_hdr_env,
+// shade.hlsl:36-40, with direction_to_equirect_uv, sample.hlsl:4-11):
+// uv = (atan2(z, x), asin(-y)) * (0.1591, 0.3183) + 0.5, a point load at
+// uint2(uv * size).  0.1591 < 1/(2 pi) and 0.3183 < 1/pi keep uv below 1 for
+// every finite direction; other inputs are clamped into the texture.
+__device__ __forceinline__ uint32_t env_texel(const GpuTexture& T, const V3& d)
+{
+    const float u = atan2f(d.z, d.x) * 0.1591f + 0.5f;
+    const float v = asinf(-d.y) * 0.3183f + 0.5f;
+    const float fx = u * (float)T.width, fy = v * (float)T.height;
+    uint32_t x = fx >= 1.0f ? (uint32_t)fx : 0u, y = fy >= 1.0f ? (uint32_t)fy : 0u;
+    x = x < T.width ? x : T.width - 1u;
+    y = y < T.height ? y : T.height - 1u;
+    return y * T.width + x;
+}
+
+// dir: the direction of the ray that reached this vertex (read for misses only)
 template <bool ADJ>
 __device__ __forceinline__ void shade_vertex(const FrameArgs& A, uint32_t p, uint32_t scene, uint32_t d, uint32_t seed,
-                                             const wf_hit& hit, V3& thr, float4& e4, const float4& dlt, VertexOut& o)
+                                             const wf_hit& hit, const V3& dir, V3& thr, float4& e4, const float4& dlt,
+                                             VertexOut& o)
 {
     float u[6];
     path_rng(p, d, seed, u);
     bool done = false;
     if (hit.instance_idx == 0xFFFFFFFFu) {
-        if (ADJ) {
-            e4.x = e4.x - thr.x * A.env[0];
-            e4.y = e4.y - thr.y * A.env[1];
-            e4.z = e4.z - thr.z * A.env[2];
+        // environment radiance: throughput * hdr_env_strength * env (shade.hlsl:62-63)
+        float ex, ey, ez;
+        if (A.env_tex >= 0) {
+            const GpuTexture& T = A.textures[A.env_tex];
+            const float* c = T.texels + 4ull * env_texel(T, dir);
+            ex = (thr.x * A.env_strength) * c[0];
+            ey = (thr.y * A.env_strength) * c[1];
+            ez = (thr.z * A.env_strength) * c[2];
         } else {
-            e4.x = e4.x + thr.x * A.env[0];
-            e4.y = e4.y + thr.y * A.env[1];
-            e4.z = e4.z + thr.z * A.env[2];
+            ex = thr.x * A.env[0];
+            ey = thr.y * A.env[1];
+            ez = thr.z * A.env[2];
+        }
+        if (ADJ) {
+            e4.x = e4.x - ex;
+            e4.y = e4.y - ey;
+            e4.z = e4.z - ez;
+        } else {
+            e4.x = e4.x + ex;
+            e4.y = e4.y + ey;
+            e4.z = e4.z + ez;
         }
         done = true;
     }

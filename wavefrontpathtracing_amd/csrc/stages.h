@@ -42,7 +42,9 @@ struct FrameArgs {
     uint32_t pix_base, n_pix;      // batch pixels [pix_base, pix_base + n_pix)
     uint32_t spp, max_bounces, rr_start, nee;
     float rr_survival;
-    float env[3];
+    float env[3];                // constant environment (hdr_env_strength x the 1x1 white fallback)
+    int32_t env_tex;             // equirect environment texture slot, -1 = the constant
+    float env_strength;          // hdr_env_strength
     uint32_t accumulate;
     const FrameDyn* dyn;
 

@@ -47,6 +47,7 @@ class Scene:
     height: int
     textures: list = field(default_factory=list)   # [(w, h, rgb float32 (h, w, 3))]
     name: str = ""
+    env_texture: int = -1            # equirect environment (index into textures), -1 = the constant default
 
     @property
     def triangle_counts(self):
@@ -261,7 +262,8 @@ def load(renderer, scene: Scene, begin: bool = True):
     mesh_h = [renderer.create_render_mesh(v, i) for v, i in scene.meshes]
     tex_h = [renderer.create_render_texture(w, h, rgb) for (w, h, rgb) in scene.textures]
     if begin:
-        renderer.begin_scene(camera_create(scene.eye, scene.target, scene.vfov_deg))
+        env = tex_h[scene.env_texture] if scene.env_texture >= 0 else L.INVALID_HANDLE
+        renderer.begin_scene(camera_create(scene.eye, scene.target, scene.vfov_deg), env)
     for ins in scene.instances:
         mat = Material(base_color_factor=(*ins.base_color, 1.0), emissive_factor=ins.emissive,
                        emissive_strength=ins.emissive_strength,
