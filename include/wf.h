@@ -174,7 +174,9 @@ WF_API wf_status wf_renderer_read_scene_energy(wf_renderer* r, uint32_t scene, f
  * concatenated; alpha ignored):
  *   grad_texels   dLoss/d(texel) of base-color texture `texture` (RGBA floats,
  *                 width*height*4; pass WF_INVALID_HANDLE to skip), summed
- *                 over every instance and scene that samples it;
+ *                 over every instance and scene that samples it; a texture
+ *                 that is also an emissive or the environment texture is
+ *                 rejected (WF_ERROR_INVALID_ARGUMENT);
  *   grad_emission dLoss/d(emissive_strength) per instance (all scenes'
  *                 instances in submission order).
  * Russian roulette and sampling do not depend on albedo or emission, so the

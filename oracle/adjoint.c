@@ -78,7 +78,7 @@ static void eval_path(const orc_shading* sh, const orc_render_params* p, const o
             if (v->emit_counted) {
                 double le[3];
                 emission_f64(factor3, strength, v->inst, le);
-                for (int k = 0; k < 3; ++k) e[k] += thr[k] * le[k];
+                for (int k = 0; k < 3; ++k) e[k] += thr[k] * le[k] * (double)v->em_tex[k];
             }
             break;
         }
@@ -87,7 +87,7 @@ static void eval_path(const orc_shading* sh, const orc_render_params* p, const o
         if (v->nee) {
             double le[3];
             emission_f64(factor3, strength, v->light_inst, le);
-            for (int k = 0; k < 3; ++k) e[k] += ((thr[k] * a[k]) * inv_pi) * le[k] * (double)v->nee_wgt;
+            for (int k = 0; k < 3; ++k) e[k] += ((thr[k] * a[k]) * inv_pi) * le[k] * (double)v->nee_tex[k] * (double)v->nee_wgt;
         }
         if (d + 1 > p->max_bounces || rec[d + 1].kind == ORC_V_END) break;
         for (int k = 0; k < 3; ++k) {
@@ -97,11 +97,24 @@ static void eval_path(const orc_shading* sh, const orc_render_params* p, const o
     }
 }
 
+/* Only the albedo use of `tex` is a variable here: a texture that also gives
+ * emission or the environment is refused (its other terms would be held
+ * at the recorded values). */
+static int tex_is_albedo_only(const orc_shading* sh, int32_t tex)
+{
+    if (tex < 0) return 1;
+    if (sh->env_tex == tex) return 0;
+    for (uint32_t i = 0; i < sh->n_inst; ++i)
+        if (sh->mi[i].emissive_tex == tex) return 0;
+    return 1;
+}
+
 int orc_eval_f64(const orc_shading* sh, const orc_render_params* p, uint32_t width, uint32_t height,
                  const orc_vertex* rec, int32_t tex, const double* texels, const double* factor3,
                  const double* strength, double* image)
 {
     if (!sh || !p || !rec || !factor3 || !strength || !image) return ORC_EINVAL;
+    if (!tex_is_albedo_only(sh, tex)) return ORC_EINVAL;
     const uint32_t npix = width * height, stride = p->max_bounces + 1;
     memset(image, 0, sizeof(double) * 3 * (size_t)npix);
     for (uint32_t s = 0; s < p->spp; ++s)
@@ -119,6 +132,7 @@ int orc_adjoint_f64(const orc_shading* sh, const orc_render_params* p, uint32_t 
                     const double* strength, const double* dl_dimage, double* d_texels, double* d_strength)
 {
     if (!sh || !p || !rec || !factor3 || !strength || !dl_dimage || !d_strength) return ORC_EINVAL;
+    if (!tex_is_albedo_only(sh, tex)) return ORC_EINVAL;
     const uint32_t npix = width * height, stride = p->max_bounces + 1;
     const double inv_pi = (double)INV_PI_F;
     for (uint32_t s = 0; s < p->spp; ++s)
@@ -133,7 +147,7 @@ int orc_adjoint_f64(const orc_shading* sh, const orc_render_params* p, uint32_t 
                 if (v->kind == ORC_V_EMIT) {
                     if (v->emit_counted) {
                         double g = 0.0;
-                        for (int k = 0; k < 3; ++k) g += delta[k] * thr[k] * factor3[3 * (size_t)v->inst + k];
+                        for (int k = 0; k < 3; ++k) g += delta[k] * thr[k] * factor3[3 * (size_t)v->inst + k] * (double)v->em_tex[k];
                         d_strength[v->inst] += g;
                     }
                     break;
@@ -154,7 +168,7 @@ int orc_adjoint_f64(const orc_shading* sh, const orc_render_params* p, uint32_t 
                     double le[3], g = 0.0;
                     emission_f64(factor3, strength, v->light_inst, le);
                     for (int k = 0; k < 3; ++k) {
-                        const double unit = ((thr[k] * a[k]) * inv_pi) * (double)v->nee_wgt;
+                        const double unit = ((thr[k] * a[k]) * inv_pi) * (double)v->nee_wgt * (double)v->nee_tex[k];
                         lrem[k] -= unit * le[k];
                         g += delta[k] * unit * factor3[3 * (size_t)v->light_inst + k];
                     }

@@ -109,6 +109,7 @@ typedef struct {
     float    emissive[3];          /* emissive_factor * emissive_strength */
     int32_t  base_color_tex;       /* -1: none (white) */
     uint32_t mesh;
+    int32_t  emissive_tex;         /* -1: none; emission = emissive * texel (sample_material, material.hlsl:35-37) */
 } orc_material_instance;
 
 typedef struct {
@@ -170,6 +171,8 @@ typedef struct {
     uint32_t nee;           /* 1: a visible shadow-ray connection to light_inst */
     uint32_t light_inst;
     float    nee_wgt;       /* G * total emitter area */
+    float    em_tex[3];     /* ORC_V_EMIT: emissive texture value at the hit (1 without a texture) */
+    float    nee_tex[3];    /* nee: emissive texture value at the light sample (1 without a texture) */
 } orc_vertex;
 
 /* Walk every path once (FP32 geometry, as orc_render); rec holds
@@ -187,7 +190,8 @@ int orc_eval_f64(const orc_shading* sh, const orc_render_params* p, uint32_t wid
 
 /* FP64 path-replay adjoint: given dLoss/dimage (width*height*3), the
  * gradients with respect to the texels of `tex` (RGBA doubles) and the
- * per-instance emission strengths. */
+ * per-instance emission strengths.  Both functions return ORC_EINVAL when
+ * `tex` is also an emissive or the environment texture. */
 int orc_adjoint_f64(const orc_shading* sh, const orc_render_params* p, uint32_t width, uint32_t height,
                     const orc_vertex* rec, int32_t tex, const double* texels, const double* factor3,
                     const double* strength, const double* dl_dimage, double* d_texels, double* d_strength);

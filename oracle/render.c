@@ -185,11 +185,6 @@ static void tex_sample(const orc_shading* sh, int32_t tex, float u, float v, flo
 }
 
 /* ---- emitter table --------------------------------------------------------------- */
-static int is_emissive(const orc_material_instance* m)
-{
-    return m->emissive[0] != 0.0f || m->emissive[1] != 0.0f || m->emissive[2] != 0.0f;
-}
-
 static void world_tri(const orc_shading* sh, uint32_t inst, uint32_t prim, float w[3][3])
 {
     const orc_material_instance* m = &sh->mi[inst];
@@ -322,10 +317,18 @@ static int trace_path(render_job* J, uint32_t path, float energy[3], orc_vertex*
             for (int k = 0; k < 3; ++k) albedo[k] = albedo[k] * tc[k];
             if (vr) vr->tex = m->base_color_tex;
         }
-        if (is_emissive(m)) {
+        /* emission = strength * factor * emissive texel (sample_material, material.hlsl:35-37) */
+        float etc[3] = { 1.0f, 1.0f, 1.0f }, em[3];
+        if (m->emissive_tex >= 0) tex_sample(sh, m->emissive_tex, uv[0], uv[1], etc, NULL);
+        for (int k = 0; k < 3; ++k) em[k] = m->emissive[k] * etc[k];
+        if (em[0] != 0.0f || em[1] != 0.0f || em[2] != 0.0f) {
             if (!P->nee || d == 0)
-                for (int k = 0; k < 3; ++k) energy[k] = energy[k] + thr[k] * m->emissive[k];
-            if (vr) { vr->kind = ORC_V_EMIT; vr->emit_counted = (!P->nee || d == 0) ? 1u : 0u; }
+                for (int k = 0; k < 3; ++k) energy[k] = energy[k] + thr[k] * em[k];
+            if (vr) {
+                vr->kind = ORC_V_EMIT;
+                vr->emit_counted = (!P->nee || d == 0) ? 1u : 0u;
+                for (int k = 0; k < 3; ++k) vr->em_tex[k] = etc[k];
+            }
             break;
         }
         if (vr) vr->kind = ORC_V_DIFFUSE;
@@ -359,7 +362,16 @@ static int trace_path(render_job* J, uint32_t path, float energy[3], orc_vertex*
             if (cos_s > 0.0f && cos_l > 0.0f && dist2 > 0.0f) {
                 float G = (cos_s * cos_l) / dist2;
                 float wgt = G * J->em_area;
-                const float* Le = sh->mi[li].emissive;
+                /* the light's emission at the sampled point, with its emissive texture */
+                float ltc[3] = { 1.0f, 1.0f, 1.0f }, Le[3];
+                const orc_material_instance* lm = &sh->mi[li];
+                if (lm->emissive_tex >= 0) {
+                    const float* lt6 = sh->shade_tris + 24 * ((size_t)sh->shade_off[lm->mesh] + lt);
+                    const float lu = interp(lt6[18], lt6[20], lt6[22], bv, bw);
+                    const float lv = interp(lt6[19], lt6[21], lt6[23], bv, bw);
+                    tex_sample(sh, lm->emissive_tex, lu, lv, ltc, NULL);
+                }
+                for (int k = 0; k < 3; ++k) Le[k] = lm->emissive[k] * ltc[k];
                 float c[3];
                 for (int k = 0; k < 3; ++k) c[k] = (((thr[k] * albedo[k]) * INV_PI) * Le[k]) * wgt;
                 orc_ray sr;
@@ -371,7 +383,10 @@ static int trace_path(render_job* J, uint32_t path, float energy[3], orc_vertex*
                 if (rc) return rc;
                 if (sh_hit.instance_idx == ~0u) {
                     for (int k = 0; k < 3; ++k) energy[k] = energy[k] + c[k];
-                    if (vr) { vr->nee = 1; vr->light_inst = li; vr->nee_wgt = wgt; }
+                    if (vr) {
+                        vr->nee = 1; vr->light_inst = li; vr->nee_wgt = wgt;
+                        for (int k = 0; k < 3; ++k) vr->nee_tex[k] = ltc[k];
+                    }
                 }
             }
         }

@@ -30,13 +30,15 @@ def png_bytes(rgb8: np.ndarray) -> bytes:
     return buf.getvalue()
 
 
-def build(interleave: bool = True, glb: bool = False, texture: np.ndarray = None):
+def build(interleave: bool = True, glb: bool = False, texture: np.ndarray = None, emissive_texture: np.ndarray = None):
     """Two meshes: the room of scenes.box_room() (u32 indices) and a quad
     (u16 indices, interleaved position/normal/uv when `interleave`).  Node 0
     (scale 2) holds the room; node 1 rotates 90 degrees about +y, translates
     by (0, 1, 0) and has child node 2 (translation (0, 0, 0.5), the quad).  The
     quad's material is emissive (factor (1, 0.5, 0.25), strength 4) or, with
-    `texture`, textured.  Returns the file bytes (JSON text or GLB)."""
+    `texture`, textured, and then a third node holds a downward emissive quad
+    (factor 1, strength 6, with `emissive_texture` as its emissiveTexture).
+    Returns the file bytes (JSON text or GLB)."""
     rv, ri = scenes.box_room()
     qp, qn, quv, qi = quad_mesh()
     views, accessors, blob = [], [], b""
@@ -114,8 +116,13 @@ def build(interleave: bool = True, glb: bool = False, texture: np.ndarray = None
                                               "indices": a_qi, "material": 2}]})
         doc["nodes"][-1]["mesh"] = 2
         doc["scenes"][0]["nodes"].append(3)
-        doc["materials"].append({"emissiveFactor": [1.0, 1.0, 1.0],
-                                 "extensions": {"KHR_materials_emissive_strength": {"emissiveStrength": 6.0}}})
+        light = {"pbrMetallicRoughness": {"baseColorFactor": [1.0, 1.0, 1.0, 1.0]}, "emissiveFactor": [1.0, 1.0, 1.0],
+                 "extensions": {"KHR_materials_emissive_strength": {"emissiveStrength": 6.0}}}
+        if emissive_texture is not None:
+            doc["images"].append({"bufferView": add_view(png_bytes(emissive_texture)), "mimeType": "image/png"})
+            doc["textures"].append({"source": 1})
+            light["emissiveTexture"] = {"index": 1}
+        doc["materials"].append(light)
     if glb:
         doc["buffers"] = [{"byteLength": len(blob)}]
         js = json.dumps(doc).encode()

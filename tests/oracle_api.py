@@ -19,8 +19,9 @@ ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 COUNTERS = np.dtype([("tlas_nodes", "<u4"), ("blas_nodes", "<u4"), ("instances", "<u4"), ("tri_tests", "<u4")])
 VERTEX_REC = np.dtype([("kind", "<u4"), ("inst", "<u4"), ("emit_counted", "<u4"), ("rr", "<u4"), ("noL", "<f4"),
                        ("inv_pdf", "<f4"), ("tex", "<i4"), ("texel", "<u4", 4), ("fx", "<f4"), ("fy", "<f4"),
-                       ("nee", "<u4"), ("light_inst", "<u4"), ("nee_wgt", "<f4")])
-assert VERTEX_REC.itemsize == 64
+                       ("nee", "<u4"), ("light_inst", "<u4"), ("nee_wgt", "<f4"), ("em_tex", "<f4", 3),
+                       ("nee_tex", "<f4", 3)])
+assert VERTEX_REC.itemsize == 88
 
 
 class OrcScene(C.Structure):
@@ -36,7 +37,7 @@ class OrcView(C.Structure):
 
 class OrcMaterialInstance(C.Structure):
     _fields_ = [("local_to_world", C.c_float * 16), ("base_color", C.c_float * 3), ("emissive", C.c_float * 3),
-                ("base_color_tex", C.c_int32), ("mesh", C.c_uint32)]
+                ("base_color_tex", C.c_int32), ("mesh", C.c_uint32), ("emissive_tex", C.c_int32)]
 
 
 class OrcShading(C.Structure):
@@ -236,6 +237,7 @@ class OracleScene:
             mi[k].emissive[:] = [float(np.float32(np.float32(e) * np.float32(ins.emissive_strength))) for e in ins.emissive]
             mi[k].base_color_tex = ins.texture
             mi[k].mesh = ins.mesh
+            mi[k].emissive_tex = getattr(ins, "emissive_texture", -1)
         self._mi = mi
         texs, toff, tw, th = [], [], [], []
         at = 0
@@ -315,7 +317,8 @@ class Differentiable:
         out = np.zeros((self.h, self.w, 3), dtype=np.float64)
         rc = olib().orc_eval_f64(C.byref(self.sh), C.byref(self.prm), self.w, self.h, p(self.rec), tex, p(texels),
                                  p(self.factor3), p(strength), p(out))
-        assert rc == 0
+        if rc != 0:
+            raise ValueError(f"orc_eval_f64 rc={rc}")
         return out
 
     def adjoint(self, dl_dimage, texels=None, strength=None, tex: int = 0):
@@ -326,7 +329,8 @@ class Differentiable:
         d_str = np.zeros_like(strength)
         rc = olib().orc_adjoint_f64(C.byref(self.sh), C.byref(self.prm), self.w, self.h, p(self.rec), tex, p(texels),
                                     p(self.factor3), p(strength), p(dl), p(d_tex), p(d_str))
-        assert rc == 0
+        if rc != 0:
+            raise ValueError(f"orc_adjoint_f64 rc={rc}")
         return d_tex, d_str
 
 

@@ -208,3 +208,43 @@ def test_adjoint_with_env_texture_matches_fp64_reference():
     want = ref_tex[:, :3]
     err = np.abs(got - want).sum() / np.abs(want).sum()
     assert err <= REL_TOL, err
+
+
+@pytest.mark.parametrize("nee", [0, 1], ids=["bsdf", "nee"])
+def test_adjoint_with_emissive_texture_matches_fp64_reference(nee):
+    """With an emissive texture on the light, emission at hits and at light
+    samples is strength * factor * texel: the GPU adjoint's albedo-texture
+    and emission-strength gradients match the FP64 reference."""
+    sc = diff_scene()
+    sc.textures.append((8, 8, scenes.value_noise_texture(13, 8, 8, cells=4, lo=0.0, hi=2.0)))
+    sc.instances[3].emissive_texture = 1
+    ref = O.OracleScene(sc).record(max_bounces=SETTINGS["max_bounces"], spp=SETTINGS["spp"], nee=nee,
+                                   env=SETTINGS["env"], frame_seed=SETTINGS["seed"])
+    rng = np.random.default_rng(12)
+    t0 = ref.texels0()
+    target = ref.image(texels=np.clip(t0 + rng.normal(0, 0.1, t0.shape), 0.05, 0.95), strength=ref.strength0 * 0.9)
+    r, tex = open_renderer(sc, nee)
+    try:
+        r.render()
+        img = r.read_energy()
+        dl = rgba(2.0 * (img[..., :3].astype(np.float64) - target))
+        g_tex, g_em = r.render_adjoint(dl, tex, (32, 32), len(sc.instances))
+    finally:
+        r.exit()
+    ref_tex, ref_em = ref.adjoint(dl[..., :3].astype(np.float64))
+    got = g_tex.reshape(-1, 4)[:, :3].astype(np.float64)
+    err = np.abs(got - ref_tex[:, :3]).sum() / np.abs(ref_tex[:, :3]).sum()
+    assert err <= REL_TOL, err
+    assert abs(g_em[3] - ref_em[3]) <= REL_TOL * abs(ref_em[3]), (g_em[3], ref_em[3])
+
+
+def test_adjoint_refuses_a_texture_that_also_emits():
+    sc = diff_scene()
+    sc.instances[3].emissive_texture = 0
+    r, tex = open_renderer(sc, 1)
+    try:
+        r.render()
+        with pytest.raises(L.WfError):
+            r.render_adjoint(np.zeros((H, W, 4), np.float32), tex, (32, 32), len(sc.instances))
+    finally:
+        r.exit()

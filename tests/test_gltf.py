@@ -55,6 +55,38 @@ def test_texture_decodes_to_linear(tmp_path):
     assert sc.instances[2].emissive_strength == 6.0
 
 
+def test_emissive_texture_decodes_to_linear(tmp_path):
+    """emissiveTexture is sRGB like baseColorTexture (asset_loader.cpp:404-408,
+    460-467) and lands in the light instance's emissive_texture slot."""
+    rgb8 = np.array([[[0, 128, 255], [10, 200, 40]]], np.uint8)
+    em8 = np.array([[[255, 64, 0]], [[30, 30, 220]]], np.uint8)
+    path = tmp_path / "e.gltf"
+    path.write_bytes(F.build(texture=rgb8, emissive_texture=em8))
+    sc = gltf.load(str(path))
+    assert len(sc.textures) == 2
+    w, h, rgb = sc.textures[1]
+    assert (w, h) == (1, 2)
+    c = em8.astype(np.float64) / 255.0
+    want = np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4).astype(np.float32)
+    assert np.array_equal(rgb, want)
+    assert [ins.emissive_texture for ins in sc.instances] == [-1, -1, 1]
+    assert [ins.texture for ins in sc.instances] == [-1, 0, -1]
+
+
+def test_emission_needs_a_pbr_block(tmp_path):
+    """The reference reads emissiveFactor, its strength and emissiveTexture only
+    inside `has_pbr_metallic_roughness` (asset_loader.cpp:426-468): a material
+    without that block emits nothing."""
+    import json
+    doc = json.loads(F.build())
+    del doc["materials"][1]["pbrMetallicRoughness"]
+    path = tmp_path / "nopbr.gltf"
+    path.write_text(json.dumps(doc))
+    sc = gltf.load(str(path))
+    assert sc.instances[1].emissive_strength == 0.0 and sc.instances[1].emissive == (0.0, 0.0, 0.0)
+    assert sc.instances[1].base_color == (1.0, 1.0, 1.0)
+
+
 def test_missing_normals_rejected(tmp_path):
     import json
     doc = json.loads(F.build())

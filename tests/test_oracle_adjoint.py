@@ -11,15 +11,23 @@ H = 1e-4
 REL_TOL = 1e-3
 
 
-def small_diff_scene():
-    return scenes.differentiable_scene(width=24, height=20, ico_subdiv=2, tex_size=16)
+def small_diff_scene(emissive_texture=False):
+    """The differentiable room; with `emissive_texture`, the light also
+    carries a seeded-noise emissive texture (texture 1)."""
+    sc = scenes.differentiable_scene(width=24, height=20, ico_subdiv=2, tex_size=16)
+    if emissive_texture:
+        sc.textures.append((8, 8, scenes.value_noise_texture(13, 8, 8, cells=4, lo=0.0, hi=2.0)))
+        sc.instances[3].emissive_texture = 1
+    return sc
 
 
-@pytest.fixture(scope="module", params=[0, 1], ids=["bsdf", "nee"])
+@pytest.fixture(scope="module", params=[(0, False), (1, False), (0, True), (1, True)],
+                ids=["bsdf", "nee", "bsdf-emtex", "nee-emtex"])
 def diff(request):
-    sc = small_diff_scene()
+    nee, emtex = request.param
+    sc = small_diff_scene(emtex)
     orc = O.OracleScene(sc)
-    return orc.record(max_bounces=4, spp=8, nee=request.param, env=0.0, frame_seed=7)
+    return orc.record(max_bounces=4, spp=8, nee=nee, env=0.0, frame_seed=7)
 
 
 def _loss_and_grad(img, target):
@@ -68,3 +76,16 @@ def test_emission_gradient_matches_central_differences(diff):
     fd = (_loss_and_grad(diff.image(strength=sp), target)[0] - _loss_and_grad(diff.image(strength=sm), target)[0]) / (2 * H)
     assert abs(fd) > 0 and abs(fd - g_s[k]) <= REL_TOL * abs(fd), (fd, g_s[k])
     assert np.all(g_s[:3] == 0.0)   # non-emissive instances get no gradient
+
+
+def test_texture_that_also_emits_is_refused():
+    """Differentiating a texture that is also an emissive texture is refused:
+    only its albedo use would be a variable."""
+    sc = small_diff_scene()
+    sc.instances[3].emissive_texture = 0
+    d = O.OracleScene(sc).record(max_bounces=2, spp=1, nee=1, env=0.0, frame_seed=7)
+    with pytest.raises(ValueError):
+        d.image()
+    with pytest.raises(ValueError):
+        d.adjoint(np.zeros((sc.height, sc.width, 3)))
+    d.image(tex=-1)

@@ -158,3 +158,80 @@ def test_env_texture_lights_the_furnace_by_direction():
         t = O.env_texel(sh, rays["direction"][k])
         expect = np.float32(2.0) * rgb[t]
         assert np.array_equal(img.reshape(-1, 4)[k, :3], expect), k
+
+
+def _emissive_tex_scene(tex=None, w=24, h=18, ico_subdiv=2):
+    """The differentiable room at a small size; with `tex` (w, h, rgb) added as
+    texture 1 and made the ceiling light's (instance 3) emissive texture."""
+    sc = scenes.differentiable_scene(width=w, height=h, ico_subdiv=ico_subdiv, tex_size=16)
+    if tex is not None:
+        sc.textures.append(tex)
+        sc.instances[3].emissive_texture = 1
+    return sc
+
+
+def _ramp(w=16, h=8, su=0.25, sv=0.5):
+    """Texel (i, j) = su * j + sv * i: bilinear filtering reproduces the
+    linear function su * tx + sv * ty exactly away from the wrapped border."""
+    j, i = np.meshgrid(np.arange(w), np.arange(h))
+    val = (su * j + sv * i).astype(np.float32)
+    return w, h, np.repeat(val[..., None], 3, axis=2)
+
+
+@pytest.mark.parametrize("nee", [0, 1])
+def test_constant_emissive_texture_scales_emission(nee):
+    """emission = strength * factor * texel (material.hlsl:35-37): a constant
+    texture of 1 gives the untextured image and one of 0.5 the image at half
+    the strength, bit for bit (scaling by 1 and by 0.5 is exact)."""
+    kw = dict(max_bounces=3, spp=2, nee=nee, env=0.0, nthreads=8)
+    base = O.OracleScene(_emissive_tex_scene()).render(**kw)
+    assert base[..., :3].sum() > 0
+    one = O.OracleScene(_emissive_tex_scene((4, 4, np.ones((4, 4, 3), np.float32)))).render(**kw)
+    assert np.array_equal(base, one)
+    half = O.OracleScene(_emissive_tex_scene((4, 4, np.full((4, 4, 3), 0.5, np.float32)))).render(**kw)
+    sc = _emissive_tex_scene()
+    sc.instances[3].emissive_strength = 4.0
+    assert np.array_equal(half, O.OracleScene(sc).render(**kw))
+
+
+def test_emissive_texture_seen_directly_follows_the_hit_uv():
+    """Camera rays that hit the light (centre (0, 1.95, 0), size 1.2 in x and
+    z, uv = (x / 1.2 + 0.5, z / 1.2 + 0.5)) return strength * texel(uv); with
+    the linear ramp texture that is 8 * (0.25 * (16 u - 0.5) + 0.5 * (8 v -
+    0.5)), computed here from the hit point, not from the barycentrics."""
+    sc = _emissive_tex_scene(_ramp(), w=48, h=40)
+    sc.eye, sc.target, sc.vfov_deg = (0.0, -0.5, -1.2), (0.0, 1.95, 0.1), 50.0
+    orc = O.OracleScene(sc)
+    img = orc.render(max_bounces=2, spp=1, nee=1, env=0.0).reshape(-1, 4)
+    view = O.camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
+    rays = O.primary_rays(view, sc.width, sc.height)
+    hits = orc.trace_closest(rays)
+    on = np.nonzero(hits["instance_idx"] == 3)[0]
+    assert len(on) > 200
+    o = rays["origin"][on].astype(np.float64)
+    d = rays["direction"][on].astype(np.float64)
+    P = o + hits["t"][on, None].astype(np.float64) * d
+    u, v = P[:, 0] / 1.2 + 0.5, P[:, 2] / 1.2 + 0.5
+    inner = (u > 0.05) & (u < 0.95) & (v > 0.1) & (v < 0.9)
+    assert inner.sum() > 100
+    expect = 8.0 * (0.25 * (16.0 * u - 0.5) + 0.5 * (8.0 * v - 0.5))
+    got = img[on, 0].astype(np.float64)
+    assert np.allclose(got[inner], expect[inner], rtol=1e-4, atol=1e-4)
+    assert np.array_equal(img[on, 0], img[on, 1]) and np.array_equal(img[on, 0], img[on, 2])
+
+
+def test_nee_and_bsdf_agree_under_a_textured_light():
+    """Light sampling reads the emissive texture at the sampled point's uv
+    (the light triangle's uvs at the sampled barycentrics): with a light
+    that is dark on its -x side and bright on its +x side, the NEE and the
+    BSDF estimators agree on each half of the image, and the halves differ."""
+    rgb = np.zeros((8, 16, 3), np.float32)
+    rgb[:, 8:] = 4.0
+    sc = _emissive_tex_scene((16, 8, rgb), w=40, h=30)
+    orc = O.OracleScene(sc)
+    a = orc.render(max_bounces=3, spp=256, nee=0, env=0.0)[..., :3]
+    b = orc.render(max_bounces=3, spp=256, nee=1, env=0.0)[..., :3]
+    halves = [(a[:, :20].mean(), b[:, :20].mean()), (a[:, 20:].mean(), b[:, 20:].mean())]
+    for x, y in halves:
+        assert x > 0 and abs(x - y) / x < 0.04, halves
+    assert max(halves[0][1], halves[1][1]) > 1.15 * min(halves[0][1], halves[1][1]), halves

@@ -100,14 +100,17 @@ def test_render_without_scene_is_an_error():
         r.exit()
 
 
-def test_gltf_scene_matches_oracle(tmp_path):
-    """A glTF file (room, textured quad under a node hierarchy, emissive quad)
-    loaded by the asset loader renders like the oracle."""
+@pytest.mark.parametrize("emissive_texture", [False, True])
+def test_gltf_scene_matches_oracle(tmp_path, emissive_texture):
+    """A glTF file (room, textured quad under a node hierarchy, emissive quad,
+    optionally with an emissive texture) loaded by the asset loader renders
+    like the oracle."""
     from wavefrontpathtracing_amd import gltf
     import gltf_fixtures as F
     rgb8 = np.random.default_rng(4).integers(0, 256, (8, 8, 3), dtype=np.uint8)
+    em8 = np.random.default_rng(5).integers(0, 256, (4, 4, 3), dtype=np.uint8) if emissive_texture else None
     path = tmp_path / "scene.gltf"
-    path.write_bytes(F.build(texture=rgb8))
+    path.write_bytes(F.build(texture=rgb8, emissive_texture=em8))
     sc = gltf.load(str(path), width=96, height=64)
     gpu, _, _, _ = render_gpu(sc, max_bounces=4, spp=2, nee=1, env=0.1, seed=6)
     ref = O.OracleScene(sc).render(max_bounces=4, spp=2, nee=1, env=0.1, frame_seed=6, nthreads=8)
@@ -267,3 +270,45 @@ def test_env_texture_frame_matches_oracle(nee):
     flat = scenes.Scene(**{**sc.__dict__, "env_texture": -1})
     assert np.abs(ref - O.OracleScene(flat).render(max_bounces=4, spp=2, nee=nee, env=0.8, frame_seed=3,
                                                    nthreads=8)[..., :3]).max() > 0.05
+
+
+def emissive_texture_room(width=96, height=64):
+    """The differentiable room whose ceiling light (instance 3) carries a
+    seeded-noise emissive texture in [0, 2] (texture 1); the camera sees the
+    light and the room."""
+    sc = scenes.differentiable_scene(width=width, height=height, ico_subdiv=3, tex_size=32)
+    sc.textures.append((16, 16, scenes.value_noise_texture(13, 16, 16, cells=4, lo=0.0, hi=2.0)))
+    sc.instances[3].emissive_texture = 1
+    sc.eye, sc.target = (0.0, -0.4, -1.7), (0.0, 1.2, 0.6)
+    return sc
+
+
+@pytest.mark.parametrize("nee", [0, 1])
+def test_emissive_texture_frame_matches_oracle(nee):
+    """Emission is strength * factor * emissive texel (sample_material,
+    material.hlsl:35-37) at camera and BSDF hits and at light samples:
+    wavefront and megakernel frames match the oracle (relative L1 <= 1e-3,
+    > 99% of pixels bit-identical) and each other bit for bit."""
+    sc = emissive_texture_room()
+    orc = O.OracleScene(sc)
+    ref = orc.render(max_bounces=4, spp=2, nee=nee, env=0.0, frame_seed=8, nthreads=8)
+    imgs = []
+    for wavefront in (1, 0):
+        r = Renderer(sc.width, sc.height)
+        try:
+            scenes.load(r, sc)
+            r.update_settings(max_bounces=4, spp=2, next_event_estimation=nee, hdr_env_strength=0.0, frame_seed=8,
+                              accumulate=0, use_wavefront_pathtracing=wavefront)
+            r.render()
+            imgs.append(r.read_energy())
+        finally:
+            r.exit()
+    err = rel_l1(imgs[0], ref)
+    same = np.mean(np.all(imgs[0][..., :3] == ref[..., :3], axis=-1))
+    assert err <= REL_L1_TOL and same > 0.99, (err, same)
+    assert np.array_equal(imgs[0], imgs[1])
+    # the texture is really sampled: without it the image changes
+    plain = scenes.Scene(**{**sc.__dict__})
+    plain.instances = [scenes.Instance(**{**ins.__dict__, "emissive_texture": -1}) for ins in sc.instances]
+    other = O.OracleScene(plain).render(max_bounces=4, spp=2, nee=nee, env=0.0, frame_seed=8, nthreads=8)
+    assert np.abs(other[..., :3] - ref[..., :3]).max() > 0.05

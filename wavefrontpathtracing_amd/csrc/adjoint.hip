@@ -74,6 +74,14 @@ static wf_status adjoint_shape(wf_renderer* r, wf_texture_handle tex, int32_t* s
         if (!t) return r->fail(WF_ERROR_NOT_FOUND, "texture handle is not valid");
         *slot = (int32_t)(tex & 0xFFFFFFFFu);
         *texels = (size_t)t->width * t->height;
+        // the replay differentiates the albedo use of the texture only: one
+        // that also supplies emission or the environment would miss terms
+        for (const auto& sl : r->slots)
+            for (const auto& rec : sl.instances)
+                if (rec.material.emissive_texture == tex)
+                    return r->fail(WF_ERROR_INVALID_ARGUMENT, "the differentiated texture is also an emissive texture");
+        if (r->env_texture == tex)
+            return r->fail(WF_ERROR_INVALID_ARGUMENT, "the differentiated texture is also the environment texture");
     }
     *n_inst = 0;
     for (const auto& sl : r->slots) *n_inst += sl.instances.size();

@@ -329,15 +329,20 @@ __device__ __forceinline__ void shade_vertex(const FrameArgs& A, uint32_t p, uin
             const V3 tc = tex_sample(A.textures[si->base_color_tex], tu, tv);
             albedo = V3{albedo.x * tc.x, albedo.y * tc.y, albedo.z * tc.z};
         }
+        // emission = strength * factor * emissive texel (sample_material, material.hlsl:35-37)
         const float es = si->emissive_strength;
-        const V3 ef{si->emissive_factor[0], si->emissive_factor[1], si->emissive_factor[2]};
-        const V3 em{ef.x * es, ef.y * es, ef.z * es};
+        V3 etc{1.0f, 1.0f, 1.0f};
+        if (si->emissive_tex >= 0) etc = tex_sample(A.textures[si->emissive_tex], tu, tv);
+        const V3 ef{si->emissive_factor[0] * etc.x, si->emissive_factor[1] * etc.y, si->emissive_factor[2] * etc.z};
+        const V3 em{(si->emissive_factor[0] * es) * etc.x, (si->emissive_factor[1] * es) * etc.y,
+                    (si->emissive_factor[2] * es) * etc.z};
         if (em.x != 0.0f || em.y != 0.0f || em.z != 0.0f) {
             if (!A.nee || d == 0) {
                 if (ADJ) {
                     e4.x = e4.x - thr.x * em.x;
                     e4.y = e4.y - thr.y * em.y;
                     e4.z = e4.z - thr.z * em.z;
+                    // d emission / d strength = factor * texel
                     o.em_g = (dlt.x * (thr.x * ef.x) + dlt.y * (thr.y * ef.y)) + dlt.z * (thr.z * ef.z);
                     o.em_key = hit.instance_idx;
                     o.em_has = o.em_g != 0.0f;
@@ -399,13 +404,18 @@ __device__ __forceinline__ void shade_vertex(const FrameArgs& A, uint32_t p, uin
                     const float G = (cos_s * cos_l) / dist2;
                     const float wgt = G * sc.em_area;
                     const float ls = li->emissive_strength;
-                    o.contrib.x = (((thr.x * albedo.x) * kInvPi) * (li->emissive_factor[0] * ls)) * wgt;
-                    o.contrib.y = (((thr.y * albedo.y) * kInvPi) * (li->emissive_factor[1] * ls)) * wgt;
-                    o.contrib.z = (((thr.z * albedo.z) * kInvPi) * (li->emissive_factor[2] * ls)) * wgt;
+                    // the light's emissive texel at the sampled point
+                    V3 ltc{1.0f, 1.0f, 1.0f};
+                    if (li->emissive_tex >= 0)
+                        ltc = tex_sample(A.textures[li->emissive_tex], interp(lt->uv[0][0], lt->uv[1][0], lt->uv[2][0], bv, bw),
+                                         interp(lt->uv[0][1], lt->uv[1][1], lt->uv[2][1], bv, bw));
+                    o.contrib.x = (((thr.x * albedo.x) * kInvPi) * ((li->emissive_factor[0] * ls) * ltc.x)) * wgt;
+                    o.contrib.y = (((thr.y * albedo.y) * kInvPi) * ((li->emissive_factor[1] * ls) * ltc.y)) * wgt;
+                    o.contrib.z = (((thr.z * albedo.z) * kInvPi) * ((li->emissive_factor[2] * ls) * ltc.z)) * wgt;
                     if (ADJ) {
-                        o.unit.x = (((thr.x * albedo.x) * kInvPi) * li->emissive_factor[0]) * wgt;
-                        o.unit.y = (((thr.y * albedo.y) * kInvPi) * li->emissive_factor[1]) * wgt;
-                        o.unit.z = (((thr.z * albedo.z) * kInvPi) * li->emissive_factor[2]) * wgt;
+                        o.unit.x = (((thr.x * albedo.x) * kInvPi) * (li->emissive_factor[0] * ltc.x)) * wgt;
+                        o.unit.y = (((thr.y * albedo.y) * kInvPi) * (li->emissive_factor[1] * ltc.y)) * wgt;
+                        o.unit.z = (((thr.z * albedo.z) * kInvPi) * (li->emissive_factor[2] * ltc.z)) * wgt;
                         o.light = er.x;
                     }
                     o.sh_o = V3{P.x + wi.x * kRayMinT, P.y + wi.y * kRayMinT, P.z + wi.z * kRayMinT};

@@ -7,12 +7,14 @@ What is read, following the reference loader:
   POSITION and NORMAL are required, TEXCOORD_0 is read (zeros when absent),
   indices as u8/u16/u32 (a non-indexed primitive gets 0..n-1);
 - materials: pbrMetallicRoughness.baseColorFactor and baseColorTexture,
-  emissiveFactor and KHR_materials_emissive_strength (default 1, as cgltf);
+  emissiveFactor, emissiveTexture and KHR_materials_emissive_strength
+  (default 1, as cgltf); as in the reference, the emissive fields are read
+  only for materials that carry a pbrMetallicRoughness block;
 - every node with a mesh submits each of its primitives with the node's
   world transform (matrix, or T * R * S, composed down the hierarchy).
 
-Base-color images are decoded with Pillow and converted from sRGB to linear
-RGB floats here, once, so that the library and any checker receive the same
+Base-color and emissive images (both sRGB in glTF 2.0) are decoded with
+Pillow and converted from sRGB to linear RGB floats here, once, so that the library and any checker receive the same
 texel values.  Unlike the reference, a node's primitive is found through its
 own mesh (the reference indexes its global primitive list by the primitive's
 index within the mesh, which is correct only for files with one mesh).
@@ -148,24 +150,34 @@ def load(path: str, width: int = 1280, height: int = 720, vfov_deg: float = 60.0
         doc = json.loads(blob.decode("utf-8"))
     buffers = _read_buffers(doc, base_dir, glb_bin)
 
-    # materials -> (base color, texture image, emissive, strength)
+    # materials -> (base color, base-color texture, emissive, strength, emissive texture)
+    # (asset_loader.cpp:409-470)
     tex_of_image, scene_textures = {}, []
+
+    def texture_of(ref):
+        if not textures or ref is None:
+            return -1
+        src = doc["textures"][ref["index"]].get("source")
+        if src is None:
+            return -1
+        if src not in tex_of_image:
+            tex_of_image[src] = len(scene_textures)
+            scene_textures.append(_decode_image(doc, buffers, base_dir, src))
+        return tex_of_image[src]
+
     mats = []
     for m in doc.get("materials", []):
-        pbr = m.get("pbrMetallicRoughness", {})
+        pbr = m.get("pbrMetallicRoughness")
+        if pbr is None:
+            mats.append(((1.0, 1.0, 1.0), -1, (0.0, 0.0, 0.0), 0.0, -1))
+            continue
         bc = pbr.get("baseColorFactor", [1.0, 1.0, 1.0, 1.0])
-        tex = -1
-        if textures and "baseColorTexture" in pbr:
-            src = doc["textures"][pbr["baseColorTexture"]["index"]].get("source")
-            if src is not None:
-                if src not in tex_of_image:
-                    tex_of_image[src] = len(scene_textures)
-                    scene_textures.append(_decode_image(doc, buffers, base_dir, src))
-                tex = tex_of_image[src]
+        tex = texture_of(pbr.get("baseColorTexture"))
         strength = m.get("extensions", {}).get("KHR_materials_emissive_strength", {}).get("emissiveStrength", 1.0)
-        mats.append((tuple(float(x) for x in bc[:3]), tex, tuple(float(x) for x in m.get("emissiveFactor", [0, 0, 0])),
-                     float(strength)))
-    default_mat = ((1.0, 1.0, 1.0), -1, (0.0, 0.0, 0.0), 1.0)
+        em = tuple(float(x) for x in m.get("emissiveFactor", [0, 0, 0]))
+        etex = texture_of(m.get("emissiveTexture")) if any(em) else -1
+        mats.append((tuple(float(x) for x in bc[:3]), tex, em, float(strength), etex))
+    default_mat = ((1.0, 1.0, 1.0), -1, (0.0, 0.0, 0.0), 1.0, -1)
 
     # primitives -> meshes
     meshes, prim_index, prim_material = [], {}, {}
@@ -208,10 +220,11 @@ def load(path: str, width: int = 1280, height: int = 720, vfov_deg: float = 60.0
                 if key not in prim_index:
                     continue
                 mat_i = prim_material[key]
-                bc, tex, em, strength = mats[mat_i] if mat_i is not None else default_mat
+                bc, tex, em, strength, etex = mats[mat_i] if mat_i is not None else default_mat
                 instances.append(scenes.Instance(prim_index[key], world.T.astype(np.float32).reshape(16).copy(),
                                                  base_color=bc, emissive=em,
-                                                 emissive_strength=strength if any(em) else 0.0, texture=tex))
+                                                 emissive_strength=strength if any(em) else 0.0, texture=tex,
+                                                 emissive_texture=etex))
         for c in reversed(node.get("children", [])):
             stack.append((c, world))
 

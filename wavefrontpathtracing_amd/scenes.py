@@ -34,6 +34,7 @@ class Instance:
     emissive: tuple = (0.0, 0.0, 0.0)
     emissive_strength: float = 0.0
     texture: int = -1                # index into Scene.textures, -1 = none
+    emissive_texture: int = -1       # index into Scene.textures, -1 = none (emission = factor * strength * texel)
 
 
 @dataclass
@@ -267,6 +268,7 @@ def load(renderer, scene: Scene, begin: bool = True):
     for ins in scene.instances:
         mat = Material(base_color_factor=(*ins.base_color, 1.0), emissive_factor=ins.emissive,
                        emissive_strength=ins.emissive_strength,
-                       base_color_texture=tex_h[ins.texture] if ins.texture >= 0 else L.INVALID_HANDLE)
+                       base_color_texture=tex_h[ins.texture] if ins.texture >= 0 else L.INVALID_HANDLE,
+                       emissive_texture=tex_h[ins.emissive_texture] if ins.emissive_texture >= 0 else L.INVALID_HANDLE)
         renderer.submit_render_mesh(mesh_h[ins.mesh], ins.transform, mat)
     return mesh_h, tex_h
