@@ -59,64 +59,82 @@ __global__ void k_generate(FrameArgs A)
 // Shade stage (shade.hlsl:42-174): one queue entry per thread, shade_vertex
 // for the path, then the continuation and shadow rays appended to the next
 // queues (ADJ: the path-replay adjoint of the same vertex).
+//
+// Each workgroup claims the next kShadeChunk entries from a cursor, so the
+// queue is shaded in order (give or take the resident workgroups' chunks)
+// and the appended queues keep the rays of one scene and of neighbouring
+// pixels together.  A grid-stride loop lets workgroups drift apart over a
+// long queue, which interleaves far apart entries (other scenes of a batch)
+// in the next queue and costs the next extend launch its cache locality.
+constexpr uint32_t kShadeChunk = 1024;
+
 template <bool ADJ>
 __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
 {
+    __shared__ uint32_t s_base;
     const uint32_t n = A.counts[d];
     const uint32_t cur = d & 1u, nxt = cur ^ 1u;
-    const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t seed = A.dyn->frame_seed;
-    // iterate whole waves so that the ballot-based appends see converged lanes
-    const uint32_t n_round = (n + 63u) & ~63u;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
-        const bool active = i < n;
-        bool spawn = false, shadow = false;
-        V3 new_o{0, 0, 0}, new_d{0, 0, 0}, sh_o{0, 0, 0}, sh_d{0, 0, 0};
-        float sh_tmax = 0.0f;
-        float4 contrib = make_float4(0, 0, 0, 0), unit = make_float4(0, 0, 0, 0);
-        uint32_t p = 0, root = 0, light = 0;
-        uint32_t em_key = 0;
-        float em_g = 0.0f;
-        bool em_has = false;
-        if (active) {
-            p = A.path_of[cur][i];
-            root = A.root_of[cur][i];
-            const wf_hit hit = A.hits[i];
-            const float4 t4 = A.thr[p];
-            float4 e4 = ADJ ? A.lrem[p] : A.energy[p];
-            const float4 dlt = ADJ ? A.delta[p] : make_float4(0, 0, 0, 0);
-            V3 thr{t4.x, t4.y, t4.z};
-            V3 dir{0.0f, 0.0f, 0.0f};
-            if (hit.instance_idx == 0xFFFFFFFFu && A.env_tex >= 0) {
-                const wf_ray& r = A.rays[cur][i];
-                dir = V3{r.direction[0], r.direction[1], r.direction[2]};
+    uint32_t* cursor = A.counts + kShadeCursorBase + d;
+    // workgroup-uniform control flow: the ballot-based appends below see converged lanes
+    for (;;) {
+        if (threadIdx.x == 0) s_base = atomicAdd(cursor, kShadeChunk);
+        __syncthreads();
+        const uint32_t base = s_base;
+        __syncthreads();
+        if (base >= n) break;
+        const uint32_t stop = n - base < kShadeChunk ? n : base + kShadeChunk;
+        for (uint32_t j = base; j < stop; j += 256u) {
+            const uint32_t i = j + threadIdx.x;
+            const bool active = i < stop;
+            bool spawn = false, shadow = false;
+            V3 new_o{0, 0, 0}, new_d{0, 0, 0}, sh_o{0, 0, 0}, sh_d{0, 0, 0};
+            float sh_tmax = 0.0f;
+            float4 contrib = make_float4(0, 0, 0, 0), unit = make_float4(0, 0, 0, 0);
+            uint32_t p = 0, root = 0, light = 0;
+            uint32_t em_key = 0;
+            float em_g = 0.0f;
+            bool em_has = false;
+            if (active) {
+                p = A.path_of[cur][i];
+                root = A.root_of[cur][i];
+                const wf_hit hit = A.hits[i];
+                const float4 t4 = A.thr[p];
+                float4 e4 = ADJ ? A.lrem[p] : A.energy[p];
+                const float4 dlt = ADJ ? A.delta[p] : make_float4(0, 0, 0, 0);
+                V3 thr{t4.x, t4.y, t4.z};
+                V3 dir{0.0f, 0.0f, 0.0f};
+                if (hit.instance_idx == 0xFFFFFFFFu && A.env_tex >= 0) {
+                    const wf_ray& r = A.rays[cur][i];
+                    dir = V3{r.direction[0], r.direction[1], r.direction[2]};
+                }
+                VertexOut o;
+                shade_vertex<ADJ>(A, p, A.path_scene[p], d, seed, hit, dir, thr, e4, dlt, o);
+                spawn = o.spawn; shadow = o.shadow;
+                new_o = o.new_o; new_d = o.new_d; sh_o = o.sh_o; sh_d = o.sh_d; sh_tmax = o.sh_tmax;
+                contrib = o.contrib; unit = o.unit; light = o.light;
+                em_key = o.em_key; em_g = o.em_g; em_has = o.em_has;
+                A.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                if (ADJ) A.lrem[p] = e4;
+                else A.energy[p] = e4;
             }
-            VertexOut o;
-            shade_vertex<ADJ>(A, p, A.path_scene[p], d, seed, hit, dir, thr, e4, dlt, o);
-            spawn = o.spawn; shadow = o.shadow;
-            new_o = o.new_o; new_d = o.new_d; sh_o = o.sh_o; sh_d = o.sh_d; sh_tmax = o.sh_tmax;
-            contrib = o.contrib; unit = o.unit; light = o.light;
-            em_key = o.em_key; em_g = o.em_g; em_has = o.em_has;
-            A.thr[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
-            if (ADJ) A.lrem[p] = e4;
-            else A.energy[p] = e4;
-        }
-        if (ADJ) wave_add_keyed(A.grad_em, em_key, em_g, em_has);
-        const uint32_t slot = wave_append(&A.counts[d + 1], spawn);
-        if (spawn) {
-            store_ray(A.rays[nxt], slot, new_o, new_d, kRayMinT, FLT_MAX);
-            A.path_of[nxt][slot] = p;
-            A.root_of[nxt][slot] = root;
-        }
-        const uint32_t sslot = wave_append(&A.counts[kShadowCountBase + d], shadow);
-        if (shadow) {
-            store_ray(A.shadow_rays, sslot, sh_o, sh_d, kRayMinT, sh_tmax);
-            A.shadow_path[sslot] = p;
-            A.shadow_root[sslot] = root;
-            A.shadow_contrib[sslot] = contrib;
-            if (ADJ) {
-                A.shadow_unit[sslot] = unit;
-                A.shadow_light[sslot] = light;
+            if (ADJ) wave_add_keyed(A.grad_em, em_key, em_g, em_has);
+            const uint32_t slot = wave_append(&A.counts[d + 1], spawn);
+            if (spawn) {
+                store_ray(A.rays[nxt], slot, new_o, new_d, kRayMinT, FLT_MAX);
+                A.path_of[nxt][slot] = p;
+                A.root_of[nxt][slot] = root;
+            }
+            const uint32_t sslot = wave_append(&A.counts[kShadowCountBase + d], shadow);
+            if (shadow) {
+                store_ray(A.shadow_rays, sslot, sh_o, sh_d, kRayMinT, sh_tmax);
+                A.shadow_path[sslot] = p;
+                A.shadow_root[sslot] = root;
+                A.shadow_contrib[sslot] = contrib;
+                if (ADJ) {
+                    A.shadow_unit[sslot] = unit;
+                    A.shadow_light[sslot] = light;
+                }
             }
         }
     }

@@ -13,9 +13,11 @@
 namespace wf {
 
 // counts[d] = length of the ray queue of bounce d (d <= kMaxBounces);
-// counts[kShadowCountBase + d] = length of the shadow-ray queue of bounce d.
-constexpr uint32_t kCountSlots = 32;
+// counts[kShadowCountBase + d] = length of the shadow-ray queue of bounce d;
+// counts[kShadeCursorBase + d] = the shade stage's claim cursor over queue d.
+constexpr uint32_t kCountSlots = 48;
 constexpr uint32_t kShadowCountBase = 16;
+constexpr uint32_t kShadeCursorBase = 32;
 
 // one scene slot (device table)
 struct alignas(16) GpuScene {
