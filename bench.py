@@ -273,12 +273,31 @@ def bench_frame(frames: int):
     for _ in range(frames):
         r.render()
         mt += r.frame_stats().ms_total
+    # both modes with the union-metric TLAS; one frame at a fixed seed with
+    # each TLAS checks that the image is the same
+    r.update_settings(frame_seed=7)
+    r.render()
+    ref_img = r.read_energy()
+    r.set_tlas_metric("union")
+    r.build_tlas()
+    r.render()
+    um = {"image_identical": bool(np.array_equal(r.read_energy(), ref_img))}
+    r.update_settings(frame_seed=0)
+    for wavefront in (1, 0):
+        r.update_settings(use_wavefront_pathtracing=wavefront)
+        r.render()
+        t = 0.0
+        for _ in range(frames):
+            r.render()
+            t += r.frame_stats().ms_total
+        um["wavefront_ms_total" if wavefront else "megakernel_ms_total"] = round(t / frames, 3)
     r.exit()
     out = {k: round(v / frames, 3) for k, v in acc.items()}
     out.update({"workload": "256-instance scene, 1920x1080, 1 spp, depth 5, RR from 3, NEE, black environment",
                 "rays_per_frame": rays // frames, "mrays_per_s": round(rays / (acc["ms_total"] * 1e-3) / 1e6, 2),
                 "graph_ms_total": round(gt / frames, 3), "megakernel_ms_total": round(mt / frames, 3),
-                "separate_connect": {k: round(v / frames, 3) for k, v in sep.items()}})
+                "separate_connect": {k: round(v / frames, 3) for k, v in sep.items()},
+                "union_tlas": um})
     return out
 
 
@@ -543,6 +562,27 @@ def main():
     cfg = tuning.get_extend_config(r)
     traffic, traffic_src = measured_traffic(cfg)
 
+    # the same rays with the union-metric TLAS (opt-in): same hits, fewer node visits
+    r.set_tlas_metric("union")
+    r.build_tlas()
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    u_ms = ev0.elapsed_time(ev1) / args.steps
+    u_hits = d_hits.cpu().numpy().view(L.HIT)
+    _, u_counters = r.trace_closest_counters(rays)
+    union = {"mrays": round(len(rays) / (u_ms * 1e-3) / 1e6, 2), "ms_per_step": round(u_ms, 4),
+             "hits_identical": u_hits.tobytes() == hits.tobytes(),
+             "mean_counters": {k: round(float(v), 3) for k, v in
+                               zip(("tlas_nodes", "blas_nodes", "instances", "tri_tests"), u_counters.mean(axis=0))}}
+    r.set_tlas_metric("reference")
+    r.build_tlas()
+
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -566,6 +606,7 @@ def main():
             "scene_build_s": round(t_build, 2),
             "host_arrays_mrays": round(host_mrays, 2),
             "wall_s_timed": round(t_wall, 4),
+            "union_tlas": union,
         },
         "roofline": {
             "bound": "hbm",

@@ -215,6 +215,9 @@ WF_API wf_status wf_renderer_set_blas_builder(wf_renderer* r, uint32_t builder);
  * reference rebuilds it on the CPU every frame, renderer.cpp:637-645); AUTO
  * uses the GPU from 4,096 instances.  Same nodes either way. */
 WF_API wf_status wf_renderer_set_tlas_builder(wf_renderer* r, uint32_t builder);
+/* The match metric of the renderer's TLAS builds (wf_tlas_metric; default
+ * REFERENCE, the reference's tree).  Takes effect at the next TLAS build. */
+WF_API wf_status wf_renderer_set_tlas_metric(wf_renderer* r, uint32_t metric);
 
 /* Traversal kernel configuration (no reference counterpart: the autotuner's
  * axes).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
@@ -310,12 +313,25 @@ WF_API const char* wf_bvh_builder_last_error(const wf_bvh_builder* b);
 /* Agglomerative TLAS blob (tlas_header_t ‖ nodes ‖ instances). */
 WF_API wf_status wf_tlas_build(const wf_bvh_instance* instances, uint32_t m,
                                void* out_blob, uint64_t capacity, uint64_t* out_size, uint32_t* out_node_count);
+/* The agglomerative walk's match metric (no reference counterpart for
+ * UNION).  REFERENCE is tlas_builder.cpp:86-110's "max of the mins, max of
+ * the maxes" (wf_tlas_build); UNION is the half area of the two boxes' union
+ * (Walter et al. 2008), which gives a tree that rays cross in far fewer node
+ * visits.  Closest hits are the same with either tree except where two
+ * instances are hit at exactly the same distance, where the visit order
+ * picks the one reported. */
+typedef enum wf_tlas_metric { WF_TLAS_METRIC_REFERENCE = 0, WF_TLAS_METRIC_UNION = 1 } wf_tlas_metric;
+WF_API wf_status wf_tlas_build_metric(const wf_bvh_instance* instances, uint32_t m, uint32_t metric,
+                                      void* out_blob, uint64_t capacity, uint64_t* out_size, uint32_t* out_node_count);
 /* The same TLAS built on the GPU (tlas_builder_t::build, tlas_builder.cpp:6-110):
  * d_instances holds m records in device memory; d_nodes (capacity >= 2m
  * nodes) receives the node array of the blob above, the same bytes
  * wf_tlas_build writes.  Returns after the build; *out_node_count = node count. */
 WF_API wf_status wf_tlas_build_device(const wf_bvh_instance* d_instances, uint32_t m, wf_tlas_node* d_nodes,
                                       uint64_t node_capacity, uint32_t* out_node_count, void* stream);
+/* wf_tlas_build_device with a choice of match metric (wf_tlas_metric). */
+WF_API wf_status wf_tlas_build_device_metric(const wf_bvh_instance* d_instances, uint32_t m, uint32_t metric, wf_tlas_node* d_nodes,
+                                             uint64_t node_capacity, uint32_t* out_node_count, void* stream);
 /* The software instance record submit_render_mesh derives (renderer.cpp:1318-1336). */
 WF_API wf_status wf_make_bvh_instance(const float transform[16], const float blas_min[3], const float blas_max[3],
                                       uint32_t bvh_index, wf_bvh_instance* out);

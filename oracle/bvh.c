@@ -241,7 +241,7 @@ int orc_blas_build(const float* pos9, uint32_t n, uint32_t bins, int single_prim
 /* TLAS builder: tlas_builder.cpp:6-68, 86-110                              */
 /* ------------------------------------------------------------------------ */
 
-static uint32_t find_best_match(const orc_tnode* nodes, uint32_t A, const uint32_t* ids, uint32_t count)
+static uint32_t find_best_match(const orc_tnode* nodes, uint32_t A, const uint32_t* ids, uint32_t count, uint32_t metric)
 {
     float smallest = FLT_MAX;
     uint32_t best = ~0u;
@@ -249,13 +249,24 @@ static uint32_t find_best_match(const orc_tnode* nodes, uint32_t A, const uint32
         if (B == A) continue;
         const orc_tnode* na = &nodes[ids[A]];
         const orc_tnode* nb = &nodes[ids[B]];
-        /* the reference's metric: max of the mins and max of the maxes */
-        float bmax[3], bmin[3];
-        for (int k = 0; k < 3; ++k) {
-            bmax[k] = gmax(na->aabb_min[k], nb->aabb_min[k]);
-            bmin[k] = gmax(na->aabb_max[k], nb->aabb_max[k]);
+        float area;
+        if (metric == ORC_TLAS_UNION) {
+            /* half area of the union box (Walter et al. 2008) */
+            float mn[3], mx[3];
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = gmin(na->aabb_min[k], nb->aabb_min[k]);
+                mx[k] = gmax(na->aabb_max[k], nb->aabb_max[k]);
+            }
+            area = half_area(mn, mx);
+        } else {
+            /* the reference's metric: max of the mins and max of the maxes */
+            float bmax[3], bmin[3];
+            for (int k = 0; k < 3; ++k) {
+                bmax[k] = gmax(na->aabb_min[k], nb->aabb_min[k]);
+                bmin[k] = gmax(na->aabb_max[k], nb->aabb_max[k]);
+            }
+            area = half_area(bmin, bmax);
         }
-        float area = half_area(bmin, bmax);
         if (area < smallest) { smallest = area; best = B; }
     }
     return best;
@@ -263,7 +274,15 @@ static uint32_t find_best_match(const orc_tnode* nodes, uint32_t A, const uint32
 
 int orc_tlas_build(const orc_instance* inst, uint32_t m, orc_tnode* nodes, uint32_t* node_count)
 {
-    if (!inst || !nodes || !node_count || m == 0) return ORC_EINVAL;
+    return orc_tlas_build_metric(inst, m, ORC_TLAS_REFERENCE, nodes, node_count);
+}
+
+/* The same walk with either metric.  The reference's walk keeps A's index
+ * when A was the last list entry and moved into B's slot (its list then
+ * holds a stale A past the end); the union metric follows the moved entry. */
+int orc_tlas_build_metric(const orc_instance* inst, uint32_t m, uint32_t metric, orc_tnode* nodes, uint32_t* node_count)
+{
+    if (!inst || !nodes || !node_count || m == 0 || metric > ORC_TLAS_UNION) return ORC_EINVAL;
     if (m > 32767) return ORC_ECAP;   /* 16-bit packed child indices */
     memset(nodes, 0, sizeof(orc_tnode) * 2 * (size_t)m);
     uint32_t* ids = (uint32_t*)calloc(m, sizeof(uint32_t));
@@ -279,11 +298,11 @@ int orc_tlas_build(const orc_instance* inst, uint32_t m, orc_tnode* nodes, uint3
         at++;
     }
     uint32_t A = 0;
-    uint32_t B = find_best_match(nodes, A, ids, count);
+    uint32_t B = find_best_match(nodes, A, ids, count, metric);
     uint64_t guard = 0, guard_max = (uint64_t)m * m * 4 + 16;
     while (count > 1) {
         if (B == ~0u || ++guard > guard_max) { free(ids); return ORC_EINVAL; }
-        uint32_t C = find_best_match(nodes, B, ids, count);
+        uint32_t C = find_best_match(nodes, B, ids, count, metric);
         if (A == C) {
             uint32_t ia = ids[A], ib = ids[B];
             orc_tnode* nn = &nodes[at];
@@ -294,7 +313,8 @@ int orc_tlas_build(const orc_instance* inst, uint32_t m, orc_tnode* nodes, uint3
             }
             ids[A] = at++;
             ids[B] = ids[count - 1];
-            B = find_best_match(nodes, A, ids, --count);
+            if (metric == ORC_TLAS_UNION && A == count - 1) A = B;
+            B = find_best_match(nodes, A, ids, --count, metric);
         } else {
             A = B;
             B = C;

@@ -54,6 +54,11 @@ int orc_blas_build(const float* pos9, uint32_t n, uint32_t bins, int single_prim
 
 /* Agglomerative TLAS over M instance records; nodes_out holds 2M entries. */
 int orc_tlas_build(const orc_instance* inst, uint32_t m, orc_tnode* nodes_out, uint32_t* node_count);
+/* The same agglomerative walk with a choice of match metric: the reference's
+ * (max of mins / max of maxes, as orc_tlas_build) or the half area of the
+ * union box.  The union metric has no counterpart in the reference. */
+enum { ORC_TLAS_REFERENCE = 0, ORC_TLAS_UNION = 1 };
+int orc_tlas_build_metric(const orc_instance* inst, uint32_t m, uint32_t metric, orc_tnode* nodes_out, uint32_t* node_count);
 
 /* glm restatements (column-major m[col*4+row]) */
 void orc_mat4_inverse(const float m[16], float out[16]);

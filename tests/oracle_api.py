@@ -64,6 +64,7 @@ def olib():
         sig = {
             "orc_blas_build": (C.c_int, [vp, u32, u32, C.c_int, vp, vp, C.POINTER(u32)]),
             "orc_tlas_build": (C.c_int, [vp, u32, vp, C.POINTER(u32)]),
+            "orc_tlas_build_metric": (C.c_int, [vp, u32, u32, vp, C.POINTER(u32)]),
             "orc_mat4_inverse": (None, [vp, vp]),
             "orc_mat4_mul_vec4": (None, [vp, vp, vp]),
             "orc_instance_record": (None, [vp, vp, vp, u32, vp]),
@@ -139,18 +140,23 @@ def instance_record(transform, bmin, bmax, bvh_index: int) -> np.ndarray:
     return out[0]
 
 
-def tlas_build(instances: np.ndarray):
+TLAS_REFERENCE, TLAS_UNION = 0, 1
+
+
+def tlas_build(instances: np.ndarray, metric: int = TLAS_REFERENCE):
+    """orc_tlas_build_metric: the reference's walk and metric (default) or
+    the union-area metric."""
     inst = np.ascontiguousarray(instances, dtype=L.BVH_INSTANCE)
     nodes = np.zeros(2 * len(inst), dtype=L.TLAS_NODE)
     cnt = C.c_uint32()
-    rc = olib().orc_tlas_build(p(inst), len(inst), p(nodes), C.byref(cnt))
+    rc = olib().orc_tlas_build_metric(p(inst), len(inst), metric, p(nodes), C.byref(cnt))
     if rc != 0:
-        raise RuntimeError(f"orc_tlas_build rc={rc}")
+        raise RuntimeError(f"orc_tlas_build_metric rc={rc}")
     return nodes[: cnt.value].copy()
 
 
-def tlas_blob(instances: np.ndarray) -> bytes:
-    nodes = tlas_build(instances)
+def tlas_blob(instances: np.ndarray, metric: int = TLAS_REFERENCE) -> bytes:
+    nodes = tlas_build(instances, metric)
     inst = np.ascontiguousarray(instances, dtype=L.BVH_INSTANCE)
     hdr = np.array([8, 8 + 32 * len(nodes)], dtype=np.uint32)
     return hdr.tobytes() + nodes.tobytes() + inst.tobytes()
@@ -172,7 +178,7 @@ def primary_rays(view: OrcView, w: int, h: int) -> np.ndarray:
 class OracleScene:
     """The oracle's own acceleration structures for a scenes.Scene."""
 
-    def __init__(self, scene, bins: int = 8):
+    def __init__(self, scene, bins: int = 8, tlas_metric: int = TLAS_REFERENCE):
         self.scene = scene
         node_list, tri_list, idx_list = [], [], []
         node_off, tri_off = [0], [0]
@@ -193,7 +199,7 @@ class OracleScene:
         self.tri_off = np.array(tri_off, dtype=np.uint32)
         recs = [instance_record(ins.transform, *self.blas_bounds[ins.mesh], ins.mesh) for ins in scene.instances]
         self.instances = np.ascontiguousarray(np.array(recs, dtype=L.BVH_INSTANCE))
-        self.tnodes = tlas_build(self.instances)
+        self.tnodes = tlas_build(self.instances, tlas_metric)
         self.s = OrcScene(p(self.tnodes), len(self.tnodes), p(self.instances), len(self.instances), p(self.nodes),
                           p(self.node_off), p(self.tris), p(self.idx), p(self.tri_off), len(scene.meshes))
 

@@ -170,3 +170,74 @@ def test_tlas_reproduces_reference_match_metric():
     first_merge = nodes[4]   # leaves are nodes 1..3 for instances 0..2
     assert {first_merge["left_right"] & 0xFFFF, first_merge["left_right"] >> 16} == {1, 3}
     assert nodes.tobytes() == O.tlas_build(inst).tobytes()
+
+
+def _union_area(nodes, a, b):
+    mn = np.minimum(nodes[a]["aabb_min"], nodes[b]["aabb_min"])
+    mx = np.maximum(nodes[a]["aabb_max"], nodes[b]["aabb_max"])
+    ex, ey, ez = (mx - mn).astype(np.float32)
+    return np.float32(ex * ey + ey * ez + ez * ex)
+
+
+def _check_mutual_nearest(nodes, m):
+    """Replay the merges in creation order: each joins two clusters alive at
+    that moment, and each is the other's least-union-area partner among them
+    (the fixed point of the locally-ordered walk, Walter et al. 2008)."""
+    alive = set(range(1, m + 1))
+    for at in range(m + 1, len(nodes)):
+        lr = int(nodes[at]["left_right"])
+        l, r = lr & 0xFFFF, lr >> 16
+        assert l in alive and r in alive and l != r
+        d = _union_area(nodes, l, r)
+        for x in alive - {l, r}:
+            assert d <= _union_area(nodes, l, x) and d <= _union_area(nodes, r, x), (at, l, r, x)
+        alive -= {l, r}
+        alive.add(at)
+    assert len(alive) == 1
+    assert nodes[0].tobytes() == nodes[alive.pop()].tobytes()
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 17, 256, 300])
+def test_union_tlas_equals_oracle_and_merges_mutual_nearest(m):
+    inst = _instances(m, seed=m)
+    a = R.tlas_build(inst, "union")
+    assert a == O.tlas_blob(inst, O.TLAS_UNION)
+    nodes, _ = R.parse_tlas_blob(a)
+    leaves = nodes[1:][nodes[1:]["left_right"] == 0]
+    assert sorted(leaves["instance_idx"].tolist()) == list(range(m))
+    for nd in nodes[m + 1:]:
+        l, r = nodes[nd["left_right"] >> 16], nodes[nd["left_right"] & 0xFFFF]
+        assert np.array_equal(nd["aabb_min"], np.minimum(l["aabb_min"], r["aabb_min"]))
+        assert np.array_equal(nd["aabb_max"], np.maximum(l["aabb_max"], r["aabb_max"]))
+    if m > 1:
+        _check_mutual_nearest(nodes, m)
+
+
+def test_union_metric_three_boxes():
+    """The three boxes of test_tlas_reproduces_reference_match_metric: by the
+    union's half area A=[-1.8,1.1] and B=[-3.9,-3.7] are mutual best matches
+    (union extent 5.0: 5 + 1 + 5 = 11 against A with C: 5.1 -> 11.2), so the
+    first merged node joins leaves 1 and 2, where the reference metric joins
+    1 and 3."""
+    inst = np.zeros(3, dtype=L.BVH_INSTANCE)
+    inst["aabb_min"] = [(-1.8, 0, 0), (-3.9, 0, 0), (2.5, 0, 0)]
+    inst["aabb_max"] = [(1.1, 1, 1), (-3.7, 1, 1), (3.3, 1, 1)]
+    nodes, _ = R.parse_tlas_blob(R.tlas_build(inst, "union"))
+    assert {nodes[4]["left_right"] & 0xFFFF, nodes[4]["left_right"] >> 16} == {1, 2}
+    assert {nodes[5]["left_right"] & 0xFFFF, nodes[5]["left_right"] >> 16} == {3, 4}
+
+
+def test_tlas_metric_entry_points():
+    """wf_tlas_build is wf_tlas_build_metric with the reference metric, and
+    orc_tlas_build is orc_tlas_build_metric with it; unknown metrics fail."""
+    import ctypes as C
+    inst = _instances(40, seed=5)
+    blob = R.tlas_build(inst)
+    size, cnt = C.c_uint64(), C.c_uint32()
+    out = np.empty(len(blob), np.uint8)
+    assert L.lib().wf_tlas_build_metric(R.ptr(inst), 40, 0, R.ptr(out), len(blob), C.byref(size), C.byref(cnt)) == 0
+    assert out.tobytes() == blob
+    assert L.lib().wf_tlas_build_metric(R.ptr(inst), 40, 2, R.ptr(out), len(blob), C.byref(size), C.byref(cnt)) != 0
+    nodes = np.zeros(80, dtype=L.TLAS_NODE)
+    assert O.olib().orc_tlas_build(O.p(inst), 40, O.p(nodes), C.byref(cnt)) == 0
+    assert nodes[:cnt.value].tobytes() == O.tlas_build(inst, O.TLAS_REFERENCE).tobytes()

@@ -224,3 +224,24 @@ def test_traversal_counters_are_consistent():
     assert np.all(cnt["tlas_nodes"] == 1) and np.all(cnt["instances"] <= 1)
     entered = cnt["instances"] == 1
     assert np.all(cnt["blas_nodes"][entered] % 2 == 1)
+
+
+def test_union_tlas_gives_the_same_hits_in_fewer_visits():
+    """The union-metric TLAS changes only the order in which instances are
+    visited: closest hits (ids, t, barycentrics) equal the reference TLAS's
+    and brute force, and rays visit far fewer TLAS nodes (the reference's
+    "max of mins / max of maxes" metric pairs boxes that are far apart)."""
+    sc = scenes.tlas_scene(seed=9, width=48, height=40, n_instances=64, hf_quads=10, ico_subdiv=2, soup_tris=400)
+    ref = O.OracleScene(sc)
+    uni = O.OracleScene(sc, tlas_metric=O.TLAS_UNION)
+    assert ref.tnodes.tobytes() != uni.tnodes.tobytes()
+    bmin, bmax = ref.tnodes[0]["aabb_min"], ref.tnodes[0]["aabb_max"]
+    rays = scenes.sphere_rays(3, bmin, bmax, 1 << 13)
+    h_ref, c_ref = ref.trace_closest(rays, nthreads=8, counters=True)
+    h_uni, c_uni = uni.trace_closest(rays, nthreads=8, counters=True)
+    assert (h_ref["instance_idx"] != L.MISS).sum() > 1000
+    assert len(O.hits_equal(h_ref, h_uni)) == 0
+    brute = uni.brute_closest(rays[:2000])
+    assert np.array_equal(h_uni["t"][:2000].view(np.uint32), brute["t"].view(np.uint32))
+    assert c_uni["tlas_nodes"].mean() < 0.7 * c_ref["tlas_nodes"].mean(), (c_uni["tlas_nodes"].mean(), c_ref["tlas_nodes"].mean())
+    assert np.array_equal(uni.trace_any(rays) == 1, h_uni["instance_idx"] == L.MISS)

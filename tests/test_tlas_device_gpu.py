@@ -76,3 +76,34 @@ def test_renderer_device_tlas_traces_like_host_tlas(gpu_renderer_factory):
         outs.append((hits, r.read_energy()))
     assert outs[0][0].tobytes() == outs[1][0].tobytes()
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 17, 256, 2048, 2049, 3000])
+def test_device_union_tlas_equals_oracle(m):
+    inst = _instances(m, seed=m, spread=10.0 * max(1.0, (m / 256) ** (1 / 3)))
+    assert R.tlas_build_device(inst, "union") == O.tlas_blob(inst, O.TLAS_UNION)
+
+
+@pytest.mark.parametrize("builder", ["host", "device"])
+def test_renderer_union_tlas_traces_like_the_reference(gpu_renderer_factory, builder):
+    """The 256-instance scene with the union-metric TLAS (built on the host or
+    the GPU): closest hits bit-equal to the oracle's with the reference's
+    TLAS, and the frame equals the reference-TLAS frame bit for bit."""
+    sc = scenes.tlas_scene(width=128, height=72, n_instances=256, hf_quads=48, ico_subdiv=4, soup_tris=20000)
+    orc = O.OracleScene(sc)
+    bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
+    rays = np.concatenate([scenes.primary_rays(sc), scenes.sphere_rays(5, bmin, bmax, 1 << 15)])
+    want = orc.trace_closest(rays, nthreads=8)
+    imgs = []
+    for metric in ("union", "reference"):
+        r = gpu_renderer_factory(sc.width, sc.height)
+        r.set_tlas_builder(builder)
+        r.set_tlas_metric(metric)
+        scenes.load(r, sc)
+        r.build_tlas()
+        if metric == "union":
+            assert len(O.hits_equal(r.trace_closest(rays), want)) == 0
+        r.update_settings(max_bounces=3, spp=1, next_event_estimation=1, hdr_env_strength=0.5, frame_seed=4)
+        r.render()
+        imgs.append(r.read_energy())
+    assert np.array_equal(imgs[0], imgs[1])

@@ -158,9 +158,12 @@ SIGNATURES = {
     "wf_bvh_builder_last_build": (_st, [_vp, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
     "wf_bvh_builder_last_error": (C.c_char_p, [_vp]),
     "wf_tlas_build_device": (_st, [_vp, _u32, _vp, _u64, C.POINTER(_u32), _vp]),
+    "wf_tlas_build_device_metric": (_st, [_vp, _u32, _u32, _vp, _u64, C.POINTER(_u32), _vp]),
     "wf_renderer_set_tlas_builder": (_st, [_vp, _u32]),
+    "wf_renderer_set_tlas_metric": (_st, [_vp, _u32]),
     "wf_bvh_build_device": (_st, [_vp, _u32, C.POINTER(BuildOptions), _vp, _u64, _vp, C.POINTER(_u32), _vp]),
     "wf_tlas_build": (_st, [_vp, _u32, _vp, _u64, C.POINTER(_u64), C.POINTER(_u32)]),
+    "wf_tlas_build_metric": (_st, [_vp, _u32, _u32, _vp, _u64, C.POINTER(_u64), C.POINTER(_u32)]),
     "wf_make_bvh_instance": (_st, [_vp, _vp, _vp, _u32, _vp]),
     # wf_scene.h
     "wfs_pcg32_seed": (None, [C.POINTER(Pcg32), _u64, _u64]),

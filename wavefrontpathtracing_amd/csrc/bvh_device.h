@@ -37,7 +37,8 @@ wf_status device_build_blas(const float* d_pos, uint32_t n, const wf_build_optio
 // the reference's agglomerative build, node for node.  list/ids hold the
 // candidate list when m exceeds what one workgroup keeps in LDS.
 wf_status device_build_tlas(const wf_bvh_instance* d_inst, uint32_t m, wf_tlas_node* d_nodes, DevBuf& list, DevBuf& ids,
-                            DevBuf& status, hipStream_t stream, uint32_t& node_count, std::string& err);
+                            DevBuf& status, hipStream_t stream, uint32_t& node_count, std::string& err,
+                            uint32_t metric = WF_TLAS_METRIC_REFERENCE);
 
 // The 4-wide records (kQuadBytes each) of a BLAS node array, node_count / 2 of them.
 hipError_t launch_make_quads(const wf_bvh_node* d_nodes, uint32_t node_count, void* d_quads, hipStream_t stream);

@@ -245,9 +245,10 @@ void blas_blob(const BlasData& d, std::vector<uint8_t>& blob)
     std::memcpy(blob.data() + h.indices_offset, d.idx.data(), 4ull * n);
 }
 
-// find_best_match (tlas_builder.cpp:86-110), including the reference's
-// "max of mins / max of maxes" metric
-static uint32_t best_match(const std::vector<wf_tlas_node>& nodes, uint32_t a, const uint32_t* ids, uint32_t count)
+// find_best_match (tlas_builder.cpp:86-110): the first candidate of least
+// metric.  WF_TLAS_METRIC_REFERENCE is the reference's "max of mins / max of
+// maxes"; WF_TLAS_METRIC_UNION the half area of the union box.
+static uint32_t best_match(const std::vector<wf_tlas_node>& nodes, uint32_t a, const uint32_t* ids, uint32_t count, uint32_t metric)
 {
     float smallest = FLT_MAX;
     uint32_t best = ~0u;
@@ -255,20 +256,32 @@ static uint32_t best_match(const std::vector<wf_tlas_node>& nodes, uint32_t a, c
     for (uint32_t b = 0; b < count; ++b) {
         if (b == a) continue;
         const wf_tlas_node& Bn = nodes[ids[b]];
-        float hi[3], lo[3];
-        for (int k = 0; k < 3; ++k) {
-            hi[k] = gmax(A.aabb_min[k], Bn.aabb_min[k]);
-            lo[k] = gmax(A.aabb_max[k], Bn.aabb_max[k]);
+        float area;
+        if (metric == WF_TLAS_METRIC_UNION) {
+            float lo[3], hi[3];
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = gmin(A.aabb_min[k], Bn.aabb_min[k]);
+                hi[k] = gmax(A.aabb_max[k], Bn.aabb_max[k]);
+            }
+            area = half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
+        } else {
+            float hi[3], lo[3];
+            for (int k = 0; k < 3; ++k) {
+                hi[k] = gmax(A.aabb_min[k], Bn.aabb_min[k]);
+                lo[k] = gmax(A.aabb_max[k], Bn.aabb_max[k]);
+            }
+            area = half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
         }
-        const float area = half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
         if (area < smallest) { smallest = area; best = b; }
     }
     return best;
 }
 
-wf_status build_tlas(const wf_bvh_instance* inst, uint32_t m, TlasData& out)
+// tlas_builder.cpp:6-68.  With the union metric, an A that was the last list
+// entry follows its move into B's slot (the reference keeps the stale index).
+wf_status build_tlas(const wf_bvh_instance* inst, uint32_t m, TlasData& out, uint32_t metric)
 {
-    if (!inst || m == 0) return WF_ERROR_INVALID_ARGUMENT;
+    if (!inst || m == 0 || metric > WF_TLAS_METRIC_UNION) return WF_ERROR_INVALID_ARGUMENT;
     if (m > kMaxTlasInstances) return WF_ERROR_CAPACITY;
     out.nodes.assign(2ull * m, wf_tlas_node{});
     std::vector<uint32_t> ids(m);
@@ -283,12 +296,12 @@ wf_status build_tlas(const wf_bvh_instance* inst, uint32_t m, TlasData& out)
     }
     uint32_t count = m;
     uint32_t A = 0;
-    uint32_t B = best_match(out.nodes, A, ids.data(), count);
+    uint32_t B = best_match(out.nodes, A, ids.data(), count, metric);
     uint64_t guard = 0;
     const uint64_t guard_max = 4ull * m * m + 16;
     while (count > 1) {
         if (B == ~0u || ++guard > guard_max) return WF_ERROR_INVALID_ARGUMENT;
-        const uint32_t C = best_match(out.nodes, B, ids.data(), count);
+        const uint32_t C = best_match(out.nodes, B, ids.data(), count, metric);
         if (C != A) { A = B; B = C; continue; }
         const uint32_t ia = ids[A], ib = ids[B];
         wf_tlas_node& nn = out.nodes[at];
@@ -299,8 +312,9 @@ wf_status build_tlas(const wf_bvh_instance* inst, uint32_t m, TlasData& out)
         }
         ids[A] = at++;
         ids[B] = ids[count - 1];
+        if (metric == WF_TLAS_METRIC_UNION && A == count - 1) A = B;
         --count;
-        B = best_match(out.nodes, A, ids.data(), count);
+        B = best_match(out.nodes, A, ids.data(), count, metric);
     }
     out.nodes[0] = out.nodes[ids[A]];
     out.node_count = at;
@@ -361,9 +375,15 @@ extern "C" WF_API wf_status wf_bvh_build(const float* tri_positions, uint32_t n,
 extern "C" WF_API wf_status wf_tlas_build(const wf_bvh_instance* instances, uint32_t m,
                                           void* out_blob, uint64_t capacity, uint64_t* out_size, uint32_t* out_node_count)
 {
+    return wf_tlas_build_metric(instances, m, WF_TLAS_METRIC_REFERENCE, out_blob, capacity, out_size, out_node_count);
+}
+
+extern "C" WF_API wf_status wf_tlas_build_metric(const wf_bvh_instance* instances, uint32_t m, uint32_t metric,
+                                                 void* out_blob, uint64_t capacity, uint64_t* out_size, uint32_t* out_node_count)
+{
     if (!instances || !out_size) return WF_ERROR_INVALID_ARGUMENT;
     wf::TlasData d;
-    wf_status s = wf::build_tlas(instances, m, d);
+    wf_status s = wf::build_tlas(instances, m, d, metric);
     if (s != WF_OK) return s;
     std::vector<uint8_t> blob;
     wf::tlas_blob(d, blob);

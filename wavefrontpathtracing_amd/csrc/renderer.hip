@@ -286,6 +286,15 @@ extern "C" WF_API wf_status wf_renderer_set_tlas_builder(wf_renderer* r, uint32_
     return WF_OK;
 }
 
+extern "C" WF_API wf_status wf_renderer_set_tlas_metric(wf_renderer* r, uint32_t metric)
+{
+    if (!r) return WF_ERROR_INVALID_ARGUMENT;
+    if (metric > WF_TLAS_METRIC_UNION) return r->fail(WF_ERROR_INVALID_ARGUMENT, "unknown TLAS metric %u", metric);
+    r->tlas_metric = metric;
+    r->tlas_valid = false;
+    return WF_OK;
+}
+
 extern "C" WF_API wf_status wf_renderer_set_build_options(wf_renderer* r, const wf_build_options* o)
 {
     if (!r || !o) return WF_ERROR_INVALID_ARGUMENT;
@@ -500,7 +509,7 @@ wf_status wf_renderer::upload_scene()
             uint32_t count = 0;
             std::string berr;
             const wf_status s = device_build_tlas((const wf_bvh_instance*)tbuild_inst.p, m, (wf_tlas_node*)tbuild_nodes.p, tbuild_list,
-                                                  tbuild_ids, tbuild_status, stream, count, berr);
+                                                  tbuild_ids, tbuild_status, stream, count, berr, tlas_metric);
             if (s != WF_OK) return fail(s, "device TLAS build: %s", berr.c_str());
             t.nodes.resize(count);
             e = hipMemcpyAsync(t.nodes.data(), tbuild_nodes.p, sizeof(wf_tlas_node) * count, hipMemcpyDeviceToHost, stream);
@@ -509,7 +518,7 @@ wf_status wf_renderer::upload_scene()
             t.node_count = count;
             t.instances = sw;
         } else {
-            wf_status s = build_tlas(sw.data(), m, t);
+            wf_status s = build_tlas(sw.data(), m, t, tlas_metric);
             if (s != WF_OK) return fail(s, "TLAS build failed");
         }
         const uint32_t node_base = (uint32_t)all_nodes.size();

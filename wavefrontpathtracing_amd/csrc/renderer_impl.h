@@ -71,6 +71,7 @@ struct wf_renderer {
     wf::DevBuildScratch build_scratch;   // device BLAS builder scratch, kept across meshes
     wf::DevBuf build_pos, build_idx;
     uint32_t tlas_builder = WF_BLAS_BUILDER_AUTO;
+    uint32_t tlas_metric = WF_TLAS_METRIC_REFERENCE;
     wf::DevBuf tbuild_inst, tbuild_nodes, tbuild_list, tbuild_ids, tbuild_status;
     std::string err;
 

@@ -1,6 +1,7 @@
 // Device TLAS builder: the reference's agglomerative build
 // (tlas_builder.cpp:6-68 with find_best_match :86-110, including its
-// "max of mins / max of maxes" match metric) in one workgroup.
+// "max of mins / max of maxes" match metric) in one workgroup; UNION swaps
+// in the half area of the union box as the metric.
 //
 // The merge sequence is the reference's and stays sequential: A's best match
 // B, B's best match C, a merge when C == A, otherwise the walk moves on, with
@@ -27,7 +28,7 @@ __device__ __forceinline__ float tmax_(float x, float y) { return (x < y) ? y : 
 __device__ __forceinline__ float half_area3(float ex, float ey, float ez) { return ex * ey + ey * ez + ez * ex; }
 
 // status[0] = node count, status[1] = 1 if the walk found no match or did not end
-template <bool LDS>
+template <bool LDS, bool UNION>
 __global__ __launch_bounds__(kTWG) void k_tlas_build(const wf_bvh_instance* __restrict__ inst, uint32_t m, wf_tlas_node* nodes,
                                                      float* g_list, uint32_t* g_ids, uint32_t* status)
 {
@@ -66,9 +67,16 @@ __global__ __launch_bounds__(kTWG) void k_tlas_build(const wf_bvh_instance* __re
         uint32_t bi = 0xFFFFFFFFu;
         for (uint32_t b = tid; b < count; b += kTWG) {
             if (b == a) continue;
-            const float h0 = tmax_(amn0, lst[0][b]), h1 = tmax_(amn1, lst[1][b]), h2 = tmax_(amn2, lst[2][b]);
-            const float l0 = tmax_(amx0, lst[3][b]), l1 = tmax_(amx1, lst[4][b]), l2 = tmax_(amx2, lst[5][b]);
-            const float area = half_area3(h0 - l0, h1 - l1, h2 - l2);
+            float area;
+            if constexpr (UNION) {
+                const float l0 = tmin_(amn0, lst[0][b]), l1 = tmin_(amn1, lst[1][b]), l2 = tmin_(amn2, lst[2][b]);
+                const float h0 = tmax_(amx0, lst[3][b]), h1 = tmax_(amx1, lst[4][b]), h2 = tmax_(amx2, lst[5][b]);
+                area = half_area3(h0 - l0, h1 - l1, h2 - l2);
+            } else {
+                const float h0 = tmax_(amn0, lst[0][b]), h1 = tmax_(amn1, lst[1][b]), h2 = tmax_(amn2, lst[2][b]);
+                const float l0 = tmax_(amx0, lst[3][b]), l1 = tmax_(amx1, lst[4][b]), l2 = tmax_(amx2, lst[5][b]);
+                area = half_area3(h0 - l0, h1 - l1, h2 - l2);
+            }
             if (area < best) { best = area; bi = b; }
         }
         for (int o = 32; o > 0; o >>= 1) {
@@ -116,6 +124,7 @@ __global__ __launch_bounds__(kTWG) void k_tlas_build(const wf_bvh_instance* __re
         }
         __syncthreads();
         ++at;
+        if (UNION && A == count - 1) A = B;   // A was the last entry and moved into B's slot
         --count;
         B = best_match(A, count);
     }
@@ -129,18 +138,23 @@ __global__ __launch_bounds__(kTWG) void k_tlas_build(const wf_bvh_instance* __re
 } // namespace
 
 wf_status device_build_tlas(const wf_bvh_instance* d_inst, uint32_t m, wf_tlas_node* d_nodes, DevBuf& list, DevBuf& ids,
-                            DevBuf& status, hipStream_t st, uint32_t& node_count, std::string& err)
+                            DevBuf& status, hipStream_t st, uint32_t& node_count, std::string& err, uint32_t metric)
 {
     if (!d_inst || !d_nodes || m == 0) { err = "null pointer or no instances"; return WF_ERROR_INVALID_ARGUMENT; }
+    if (metric > WF_TLAS_METRIC_UNION) { err = "unknown TLAS metric"; return WF_ERROR_INVALID_ARGUMENT; }
+    const bool uni = metric == WF_TLAS_METRIC_UNION;
     if (m > kMaxTlasInstances) { err = "too many instances for 16-bit TLAS child indices"; return WF_ERROR_CAPACITY; }
     hipError_t e = status.ensure(8);
     if (e == hipSuccess && m > kTlasLdsMax) e = list.ensure(24ull * m);
     if (e == hipSuccess && m > kTlasLdsMax) e = ids.ensure(4ull * m);
     if (e != hipSuccess) { err = "TLAS builder scratch allocation failed"; return WF_ERROR_OUT_OF_MEMORY; }
-    if (m <= kTlasLdsMax)
-        k_tlas_build<true><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, nullptr, nullptr, (uint32_t*)status.p);
-    else
-        k_tlas_build<false><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, (float*)list.p, (uint32_t*)ids.p, (uint32_t*)status.p);
+    if (m <= kTlasLdsMax) {
+        if (uni) k_tlas_build<true, true><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, nullptr, nullptr, (uint32_t*)status.p);
+        else k_tlas_build<true, false><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, nullptr, nullptr, (uint32_t*)status.p);
+    } else {
+        if (uni) k_tlas_build<false, true><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, (float*)list.p, (uint32_t*)ids.p, (uint32_t*)status.p);
+        else k_tlas_build<false, false><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, (float*)list.p, (uint32_t*)ids.p, (uint32_t*)status.p);
+    }
     uint32_t h[2] = {0, 0};
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(h, status.p, 8, hipMemcpyDeviceToHost, st);
@@ -156,12 +170,19 @@ wf_status device_build_tlas(const wf_bvh_instance* d_inst, uint32_t m, wf_tlas_n
 extern "C" WF_API wf_status wf_tlas_build_device(const wf_bvh_instance* d_instances, uint32_t m, wf_tlas_node* d_nodes,
                                                  uint64_t node_capacity, uint32_t* out_node_count, void* stream)
 {
+    return wf_tlas_build_device_metric(d_instances, m, WF_TLAS_METRIC_REFERENCE, d_nodes, node_capacity, out_node_count, stream);
+}
+
+extern "C" WF_API wf_status wf_tlas_build_device_metric(const wf_bvh_instance* d_instances, uint32_t m, uint32_t metric,
+                                                        wf_tlas_node* d_nodes, uint64_t node_capacity, uint32_t* out_node_count,
+                                                        void* stream)
+{
     if (!d_instances || !d_nodes || !out_node_count) return WF_ERROR_INVALID_ARGUMENT;
     if (node_capacity < 2ull * m) return WF_ERROR_CAPACITY;
     wf::DevBuf list, ids, status;
     std::string err;
     uint32_t count = 0;
-    const wf_status s = wf::device_build_tlas(d_instances, m, d_nodes, list, ids, status, (hipStream_t)stream, count, err);
+    const wf_status s = wf::device_build_tlas(d_instances, m, d_nodes, list, ids, status, (hipStream_t)stream, count, err, metric);
     list.release();
     ids.release();
     status.release();

@@ -42,7 +42,7 @@ struct TlasData {
     uint32_t node_count = 0;
 };
 wf_status build_blas(const float* pos9, uint32_t n, const wf_build_options& opt, BlasData& out);
-wf_status build_tlas(const wf_bvh_instance* inst, uint32_t m, TlasData& out);
+wf_status build_tlas(const wf_bvh_instance* inst, uint32_t m, TlasData& out, uint32_t metric = WF_TLAS_METRIC_REFERENCE);
 void blas_blob(const BlasData& d, std::vector<uint8_t>& blob);
 void tlas_blob(const TlasData& d, std::vector<uint8_t>& blob);
 void make_bvh_instance(const float* transform, const float* bmin, const float* bmax, uint32_t bvh_index, wf_bvh_instance& out);

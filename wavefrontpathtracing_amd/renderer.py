@@ -247,6 +247,15 @@ class Renderer:
         "host" or "device"; all give the same nodes."""
         self._check(lib().wf_renderer_set_tlas_builder(self.handle, self.BLAS_BUILDERS[builder]), "set_tlas_builder")
 
+    TLAS_METRICS = {"reference": 0, "union": 1}
+
+    def set_tlas_metric(self, metric: str) -> None:
+        """The match metric of the agglomerative TLAS builds: "reference" (the
+        reference's tree, default) or "union" (half area of the union box; far
+        fewer node visits, the same closest hits except exact ties between
+        instances)."""
+        self._check(lib().wf_renderer_set_tlas_metric(self.handle, self.TLAS_METRICS[metric]), "set_tlas_metric")
+
     def view(self) -> L.View:
         v = L.View()
         self._check(lib().wf_renderer_get_view(self.handle, C.byref(v)), "get_view")
@@ -372,18 +381,27 @@ class DeviceBvhBuilder:
             self.handle = None
 
 
-def tlas_build(instances: np.ndarray) -> bytes:
+def tlas_build(instances: np.ndarray, metric: str = "reference") -> bytes:
+    """TLAS blob (wf_tlas_build; wf_tlas_build_metric for the "union" metric)."""
     inst = np.ascontiguousarray(instances, dtype=L.BVH_INSTANCE)
     size = C.c_uint64()
     nodes = C.c_uint32()
-    check(lib().wf_tlas_build(ptr(inst), len(inst), None, 0, C.byref(size), C.byref(nodes)), what="tlas_build")
+    mt = Renderer.TLAS_METRICS[metric]
+
+    def call(out, cap):
+        if metric == "reference":
+            return lib().wf_tlas_build(ptr(inst), len(inst), out, cap, C.byref(size), C.byref(nodes))
+        return lib().wf_tlas_build_metric(ptr(inst), len(inst), mt, out, cap, C.byref(size), C.byref(nodes))
+
+    check(call(None, 0), what="tlas_build")
     out = np.empty(size.value, dtype=np.uint8)
-    check(lib().wf_tlas_build(ptr(inst), len(inst), ptr(out), size.value, C.byref(size), C.byref(nodes)), what="tlas_build")
+    check(call(ptr(out), size.value), what="tlas_build")
     return out.tobytes()
 
 
-def tlas_build_device(instances: np.ndarray) -> bytes:
-    """The blob of tlas_build with the TLAS built on the GPU (wf_tlas_build_device)."""
+def tlas_build_device(instances: np.ndarray, metric: str = "reference") -> bytes:
+    """The blob of tlas_build with the TLAS built on the GPU (wf_tlas_build_device,
+    wf_tlas_build_device_metric for the "union" metric)."""
     import struct
     import torch
     inst = np.ascontiguousarray(instances, dtype=L.BVH_INSTANCE)
@@ -392,8 +410,13 @@ def tlas_build_device(instances: np.ndarray) -> bytes:
     d_nodes = torch.empty((max(2 * m, 2), 8), dtype=torch.int32, device="cuda")
     count = C.c_uint32()
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    check(lib().wf_tlas_build_device(C.c_void_p(d_inst.data_ptr()), m, C.c_void_p(d_nodes.data_ptr()), 2 * m,
-                                     C.byref(count), stream), what="tlas_build_device")
+    if metric == "reference":
+        check(lib().wf_tlas_build_device(C.c_void_p(d_inst.data_ptr()), m, C.c_void_p(d_nodes.data_ptr()), 2 * m,
+                                         C.byref(count), stream), what="tlas_build_device")
+    else:
+        check(lib().wf_tlas_build_device_metric(C.c_void_p(d_inst.data_ptr()), m, Renderer.TLAS_METRICS[metric],
+                                                C.c_void_p(d_nodes.data_ptr()), 2 * m, C.byref(count), stream),
+              what="tlas_build_device")
     nodes = d_nodes[:count.value].cpu().numpy().tobytes()
     return struct.pack("<II", 8, 8 + 32 * count.value) + nodes + inst.tobytes()
 
