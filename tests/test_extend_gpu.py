@@ -234,3 +234,27 @@ def test_full_tlas_scene_subset():
         _assert_bitexact(r.trace_closest(sr), orc.trace_closest(sr, nthreads=16), "full tlas sphere subset")
     finally:
         r.exit()
+
+
+def test_benchmark_workload_bitexact_at_full_size():
+    """The extend benchmark's own workload at full size: the 256-instance
+    scene and all 2^21 incoherent rays (the same seed and bounds as the
+    benchmark), plus all 1920x1080 camera rays, bit-exact against the oracle,
+    with the reference's TLAS and with the union-metric TLAS."""
+    sc = scenes.tlas_scene()
+    r = _setup(sc)
+    try:
+        orc = O.OracleScene(sc)
+        bmin, bmax = r.scene_bounds()
+        rays = scenes.sphere_rays(3, bmin, bmax, 1 << 21)
+        cam = scenes.primary_rays(sc)
+        want_i = orc.trace_closest(rays, nthreads=16)
+        want_c = orc.trace_closest(cam, nthreads=16)
+        assert (want_i["instance_idx"] != L.MISS).mean() > 0.4
+        for metric in ("reference", "union"):
+            r.set_tlas_metric(metric)
+            r.build_tlas()
+            _assert_bitexact(r.trace_closest(rays), want_i, f"2^21 incoherent rays, {metric} TLAS")
+            _assert_bitexact(r.trace_closest(cam), want_c, f"1080p camera rays, {metric} TLAS")
+    finally:
+        r.exit()

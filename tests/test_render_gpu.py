@@ -312,3 +312,16 @@ def test_emissive_texture_frame_matches_oracle(nee):
     plain.instances = [scenes.Instance(**{**ins.__dict__, "emissive_texture": -1}) for ins in sc.instances]
     other = O.OracleScene(plain).render(max_bounces=4, spp=2, nee=nee, env=0.0, frame_seed=8, nthreads=8)
     assert np.abs(other[..., :3] - ref[..., :3]).max() > 0.05
+
+
+def test_full_size_frame_matches_oracle():
+    """The frame benchmark's workload at full size (the 256-instance scene,
+    1920x1080, 1 spp, depth 5, Russian roulette from bounce 3, NEE, black
+    environment) against the oracle's render of the same frame."""
+    sc = scenes.tlas_scene()
+    gpu, stats, _, _ = render_gpu(sc, max_bounces=5, spp=1, nee=1, env=0.0, seed=11)
+    ref = O.OracleScene(sc).render(max_bounces=5, spp=1, nee=1, env=0.0, frame_seed=11, nthreads=16)
+    err = rel_l1(gpu, ref)
+    same = np.mean(np.all(gpu[..., :3] == ref[..., :3], axis=-1))
+    assert err <= REL_L1_TOL and same > 0.99, (err, same)
+    assert stats.rays_extended > sc.width * sc.height
