@@ -312,8 +312,8 @@ def bench_inverse(steps: int):
     total = time.perf_counter() - t0
     timed = hist[1:]
     s = float(np.mean([h["s"] for h in timed]))
-    return {"workload": "64 views x 512x512, 8 spp, depth 5, NEE, 256-instance scene, 1024x1024 texture + 4 emission "
-                        "scalars, forward + adjoint + Adam per step",
+    return {"workload": "64 views x 512x512, 8 spp, depth 5, NEE, 256-instance scene (union-area TLAS), 1024x1024 "
+                        "texture + 4 emission scalars, forward + adjoint + Adam per step",
             "ms_per_step": round(s * 1e3, 1), "steps_per_s": round(1.0 / s, 3),
             "forward_rays_per_step": timed[-1]["forward_rays"],
             "loss": [round(h["loss"], 6) for h in hist], "texel_mae": [round(h["texel_mae"], 5) for h in hist],

@@ -43,15 +43,18 @@ def orbit_views(n: int, radius: float, center, seed: int):
 
 class InverseProblem:
     """All views of `scene` as one batch on one renderer; texture 0 of the
-    scene is the unknown."""
+    scene is the unknown.  The TLASes use the union-area metric by default
+    (inverse rendering has no reference counterpart whose tree it must
+    keep; the union tree gives the same hits in fewer node visits)."""
 
     def __init__(self, scene: scenes.Scene, views, width: int, height: int, spp: int, max_bounces: int,
-                 vfov_deg: float = 60.0):
+                 vfov_deg: float = 60.0, tlas_metric: str = "union"):
         if not scene.textures:
             raise ValueError("the scene needs a base-color texture to optimise")
         self.scene, self.views, self.width, self.height = scene, views, width, height
         self.r = Renderer(width, height)
         r = self.r
+        r.set_tlas_metric(tlas_metric)
         mesh_h = [r.create_render_mesh(v, i) for v, i in scene.meshes]
         tw, th, rgb = scene.textures[0]
         self.tex_shape = (th, tw)
@@ -93,7 +96,7 @@ class InverseProblem:
 
 def run(steps: int = 200, views: int = 64, res: int = 512, spp: int = 8, max_bounces: int = 5, tex_size: int = 1024,
         lr: float = 1e-2, seed: int = 7, log=print, scene: scenes.Scene = None, frame_seed: int = 0,
-        fit_emission: bool = True):
+        fit_emission: bool = True, tlas_metric: str = "union"):
     """Optimise the texture (from a constant 0.5) and, with fit_emission, the
     light strengths (from half their value) towards the scene's own; returns
     per-step records (loss, seconds, rays).  frame_seed 0 draws a new frame seed every render,
@@ -104,7 +107,7 @@ def run(steps: int = 200, views: int = 64, res: int = 512, spp: int = 8, max_bou
     if scene is None:
         sc.textures = [(tex_size, tex_size, scenes.value_noise_texture(seed, tex_size, tex_size))]
     vs = orbit_views(views, 26.0, (0.0, 0.0, 0.0), seed)
-    prob = InverseProblem(sc, vs, res, res, spp, max_bounces)
+    prob = InverseProblem(sc, vs, res, res, spp, max_bounces, tlas_metric=tlas_metric)
     prob.r.update_settings(frame_seed=frame_seed)
     dev = torch.device("cuda:0")
     try:
