@@ -228,13 +228,14 @@ def bench_build(cpu: bool):
     return out
 
 
-def bench_frame(frames: int):
+def bench_frame(frames: int, union: bool = True):
     """One wavefront frame of the 256-instance scene at 1920x1080, 1 spp,
     depth 5, Russian roulette from bounce 3, next-event estimation to the 4
     emissive quads, black environment: stage times from the library's own
     events (direct launches; shadow rays traced in the next bounce's extend
     launch, so ms_extend includes them), then the same frame as a captured
-    graph, with one launch per stage, and as one megakernel."""
+    graph, with one launch per stage, and as one megakernel; with `union`,
+    both modes again with the union-area TLAS."""
     from wavefrontpathtracing_amd import Renderer, scenes
     sc = scenes.tlas_scene()
     r = Renderer(sc.width, sc.height)
@@ -278,19 +279,21 @@ def bench_frame(frames: int):
     r.update_settings(frame_seed=7)
     r.render()
     ref_img = r.read_energy()
-    r.set_tlas_metric("union")
-    r.build_tlas()
-    r.render()
-    um = {"image_identical": bool(np.array_equal(r.read_energy(), ref_img))}
-    r.update_settings(frame_seed=0)
-    for wavefront in (1, 0):
-        r.update_settings(use_wavefront_pathtracing=wavefront)
+    um = None
+    if union:
+        r.set_tlas_metric("union")
+        r.build_tlas()
         r.render()
-        t = 0.0
-        for _ in range(frames):
+        um = {"image_identical": bool(np.array_equal(r.read_energy(), ref_img))}
+        r.update_settings(frame_seed=0)
+        for wavefront in (1, 0):
+            r.update_settings(use_wavefront_pathtracing=wavefront)
             r.render()
-            t += r.frame_stats().ms_total
-        um["wavefront_ms_total" if wavefront else "megakernel_ms_total"] = round(t / frames, 3)
+            t = 0.0
+            for _ in range(frames):
+                r.render()
+                t += r.frame_stats().ms_total
+            um["wavefront_ms_total" if wavefront else "megakernel_ms_total"] = round(t / frames, 3)
     r.exit()
     out = {k: round(v / frames, 3) for k, v in acc.items()}
     out.update({"workload": "256-instance scene, 1920x1080, 1 spp, depth 5, RR from 3, NEE, black environment",
@@ -467,6 +470,8 @@ def main():
     ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--no-inverse", action="store_true")
     ap.add_argument("--no-build", action="store_true")
+    ap.add_argument("--no-union", action="store_true",
+                    help="skip the opt-in union-area TLAS measurements (headline rays and frame)")
     ap.add_argument("--build-only", action="store_true",
                     help="only the BLAS/TLAS build section (for profiling the builders); prints its JSON")
     ap.add_argument("--tune", action="store_true", help="autotune the traversal kernel on the batched frame")
@@ -563,25 +568,27 @@ def main():
     traffic, traffic_src = measured_traffic(cfg)
 
     # the same rays with the union-metric TLAS (opt-in): same hits, fewer node visits
-    r.set_tlas_metric("union")
-    r.build_tlas()
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    u_ms = ev0.elapsed_time(ev1) / args.steps
-    u_hits = d_hits.cpu().numpy().view(L.HIT)
-    _, u_counters = r.trace_closest_counters(rays)
-    union = {"mrays": round(len(rays) / (u_ms * 1e-3) / 1e6, 2), "ms_per_step": round(u_ms, 4),
-             "hits_identical": u_hits.tobytes() == hits.tobytes(),
-             "mean_counters": {k: round(float(v), 3) for k, v in
-                               zip(("tlas_nodes", "blas_nodes", "instances", "tri_tests"), u_counters.mean(axis=0))}}
-    r.set_tlas_metric("reference")
-    r.build_tlas()
+    union = None
+    if not args.no_union:
+        r.set_tlas_metric("union")
+        r.build_tlas()
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            step()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        u_ms = ev0.elapsed_time(ev1) / args.steps
+        u_hits = d_hits.cpu().numpy().view(L.HIT)
+        _, u_counters = r.trace_closest_counters(rays)
+        union = {"mrays": round(len(rays) / (u_ms * 1e-3) / 1e6, 2), "ms_per_step": round(u_ms, 4),
+                 "hits_identical": u_hits.tobytes() == hits.tobytes(),
+                 "mean_counters": {k: round(float(v), 3) for k, v in
+                                   zip(("tlas_nodes", "blas_nodes", "instances", "tri_tests"), u_counters.mean(axis=0))}}
+        r.set_tlas_metric("reference")
+        r.build_tlas()
 
     out = {
         "metric": METRIC,
@@ -626,7 +633,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline(sc, rays)
     r.exit()
     if not args.no_frame:
-        out["frame"] = bench_frame(frames=5)
+        out["frame"] = bench_frame(frames=5, union=not args.no_union)
     if not args.no_soup:
         out["soup_65k"] = bench_soup(65536, steps=10, cpu=not args.no_cpu)
         out["soup_1m"] = bench_soup(1 << 20, steps=10, cpu=False)
