@@ -105,7 +105,7 @@ typedef struct wf_render_settings {
     uint32_t use_software_rt;
     uint32_t render_view_mode;
     uint32_t max_bounces;
-    uint32_t cosine_weighted_diffuse;
+    uint32_t cosine_weighted_diffuse;   /* 1: cosine-weighted hemisphere sampling; 0: uniform (shade.hlsl:114-131) */
     uint32_t accumulate;
     float    hdr_env_strength;
     /* additions (no reference counterpart) */

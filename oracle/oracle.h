@@ -145,6 +145,7 @@ typedef struct {
     uint32_t frame_seed;
     float    rr_survival;
     uint32_t path_base;            /* id of the first path (scenes of a batch number paths consecutively) */
+    uint32_t uniform_diffuse;      /* 1: uniform hemisphere sampling (cosine_weighted_diffuse = 0), else cosine */
 } orc_render_params;
 
 /* energy_rgba: width*height*4 floats (the resolved per-pixel mean over spp). */
@@ -207,6 +208,7 @@ void     orc_path_rng(uint32_t path, uint32_t bounce, uint32_t frame_seed, float
 float    orc_sin_turns(float u);
 float    orc_cos_turns(float u);
 void     orc_cosine_sample(const float n[3], float r0, float r1, float out[3]);
+void     orc_uniform_sample(const float n[3], float r0, float r1, float out[3]);
 
 #ifdef __cplusplus
 }

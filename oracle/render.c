@@ -33,6 +33,7 @@
 #include <string.h>
 
 #define INV_PI 0.31830988f     /* common.hlsl:23 */
+#define INV_TWO_PI 0.15915494f /* common.hlsl:24 */
 #define RAY_MIN_T 1e-8f
 
 int orc_trace_one(const orc_scene* s, const orc_ray* r, orc_hit* h, int any);
@@ -145,6 +146,23 @@ void orc_cosine_sample(const float n[3], float r0, float r1, float out[3])
     float sp, cp;
     sincos_turns(r1, &sp, &cp);
     float v[3] = { sin_theta * cp, sqrtf(1.0f - r0), sin_theta * sp };
+    float vn[3];
+    normalize3(v, vn);
+    float t0[3], t[3], b[3];
+    if (fabsf(n[0]) > fabsf(n[2])) { t0[0] = -n[1]; t0[1] = n[0]; t0[2] = 0.0f; }
+    else { t0[0] = 0.0f; t0[1] = -n[2]; t0[2] = n[1]; }
+    normalize3(t0, t);
+    cross3(n, t, b);
+    for (int k = 0; k < 3; ++k) out[k] = (vn[0] * t[k] + vn[1] * n[k]) + vn[2] * b[k];
+}
+
+/* uniform_hemisphere_sample with create_orthonormal_basis (sample.hlsl:13-43) */
+void orc_uniform_sample(const float n[3], float r0, float r1, float out[3])
+{
+    float sin_theta = sqrtf(1.0f - r0 * r0);
+    float sp, cp;
+    sincos_turns(r1, &sp, &cp);
+    float v[3] = { sin_theta * cp, r0, sin_theta * sp };
     float vn[3];
     normalize3(v, vn);
     float t0[3], t[3], b[3];
@@ -390,12 +408,14 @@ static int trace_path(render_job* J, uint32_t path, float energy[3], orc_vertex*
                 }
             }
         }
+        /* shade.hlsl:108-135: cosine (default) or uniform hemisphere sample */
         float L[3];
-        orc_cosine_sample(N, u[0], u[1], L);
+        if (P->uniform_diffuse) orc_uniform_sample(N, u[0], u[1], L);
+        else orc_cosine_sample(N, u[0], u[1], L);
         float NoL = dot3(N, L);
         if (NoL < 0.0f) NoL = 0.0f;
         if (!(NoL > 0.0f)) break;
-        float pdf = NoL * INV_PI;
+        float pdf = P->uniform_diffuse ? INV_TWO_PI : NoL * INV_PI;
         float inv_pdf = 1.0f / pdf;
         for (int k = 0; k < 3; ++k) thr[k] = thr[k] * ((NoL * (albedo[k] * INV_PI)) * inv_pdf);
         if (vr) { vr->noL = NoL; vr->inv_pdf = inv_pdf; }

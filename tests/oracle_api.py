@@ -48,7 +48,8 @@ class OrcShading(C.Structure):
 
 class OrcRenderParams(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("spp", C.c_uint32), ("rr_start", C.c_uint32), ("nee", C.c_uint32),
-                ("frame_seed", C.c_uint32), ("rr_survival", C.c_float), ("path_base", C.c_uint32)]
+                ("frame_seed", C.c_uint32), ("rr_survival", C.c_float), ("path_base", C.c_uint32),
+                ("uniform_diffuse", C.c_uint32)]
 
 
 _o = None
@@ -83,6 +84,7 @@ def olib():
             "orc_sin_turns": (f32, [f32]),
             "orc_cos_turns": (f32, [f32]),
             "orc_cosine_sample": (None, [vp, f32, f32, vp]),
+            "orc_uniform_sample": (None, [vp, f32, f32, vp]),
             "orc_record_paths": (C.c_int, [C.POINTER(OrcScene), C.POINTER(OrcShading), C.POINTER(OrcView), u32, u32,
                                            C.POINTER(OrcRenderParams), vp, C.c_int]),
             "orc_eval_f64": (C.c_int, [C.POINTER(OrcShading), C.POINTER(OrcRenderParams), u32, u32, vp, C.c_int32,
@@ -267,11 +269,11 @@ class OracleScene:
         return sh
 
     def render(self, max_bounces=3, spp=1, rr_start=3, rr_survival=0.75, nee=0, frame_seed=1, env=1.0, nthreads=8,
-               path_base=0):
+               path_base=0, uniform_diffuse=0):
         sc = self.scene
         sh = self.shading(env)
         view = camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
-        prm = OrcRenderParams(max_bounces, spp, rr_start, nee, frame_seed, rr_survival, path_base)
+        prm = OrcRenderParams(max_bounces, spp, rr_start, nee, frame_seed, rr_survival, path_base, uniform_diffuse)
         out = np.zeros((sc.height, sc.width, 4), dtype=np.float32)
         rc = olib().orc_render(C.byref(self.s), C.byref(sh), C.byref(view), sc.width, sc.height, C.byref(prm), p(out), nthreads)
         if rc != 0:
@@ -280,13 +282,13 @@ class OracleScene:
 
     # ---- FP64 differentiable reference ----
     def record(self, max_bounces=3, spp=1, rr_start=3, rr_survival=0.75, nee=0, frame_seed=1, env=1.0, nthreads=8,
-               path_base=0):
+               path_base=0, uniform_diffuse=0):
         """Walk every path once (FP32 geometry) and keep the vertex records;
         returns a Differentiable over them."""
         sc = self.scene
         sh = self.shading(env)
         view = camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
-        prm = OrcRenderParams(max_bounces, spp, rr_start, nee, frame_seed, rr_survival, path_base)
+        prm = OrcRenderParams(max_bounces, spp, rr_start, nee, frame_seed, rr_survival, path_base, uniform_diffuse)
         rec = np.zeros(sc.width * sc.height * spp * (max_bounces + 1), dtype=VERTEX_REC)
         rc = olib().orc_record_paths(C.byref(self.s), C.byref(sh), C.byref(view), sc.width, sc.height, C.byref(prm),
                                      p(rec), nthreads)

@@ -248,3 +248,29 @@ def test_adjoint_refuses_a_texture_that_also_emits():
             r.render_adjoint(np.zeros((H, W, 4), np.float32), tex, (32, 32), len(sc.instances))
     finally:
         r.exit()
+
+
+def test_adjoint_with_uniform_diffuse_matches_fp64_reference():
+    """Uniform hemisphere sampling (cosine_weighted_diffuse = 0) changes the
+    bounce weight, not its linearity in albedo: the GPU adjoint matches the
+    FP64 reference of the uniform-sampling paths."""
+    sc = diff_scene()
+    ref = O.OracleScene(sc).record(max_bounces=SETTINGS["max_bounces"], spp=SETTINGS["spp"], nee=1, env=SETTINGS["env"],
+                                   frame_seed=SETTINGS["seed"], uniform_diffuse=1)
+    rng = np.random.default_rng(21)
+    t0 = ref.texels0()
+    target = ref.image(texels=np.clip(t0 + rng.normal(0, 0.1, t0.shape), 0.05, 0.95))
+    r, tex = open_renderer(sc, 1)
+    try:
+        r.update_settings(cosine_weighted_diffuse=0)
+        r.render()
+        img = r.read_energy()
+        dl = rgba(2.0 * (img[..., :3].astype(np.float64) - target))
+        g_tex, g_em = r.render_adjoint(dl, tex, (32, 32), len(sc.instances))
+    finally:
+        r.exit()
+    ref_tex, ref_em = ref.adjoint(dl[..., :3].astype(np.float64))
+    got = g_tex.reshape(-1, 4)[:, :3].astype(np.float64)
+    err = np.abs(got - ref_tex[:, :3]).sum() / np.abs(ref_tex[:, :3]).sum()
+    assert err <= REL_TOL, err
+    assert abs(g_em[3] - ref_em[3]) <= REL_TOL * abs(ref_em[3]), (g_em[3], ref_em[3])

@@ -235,3 +235,46 @@ def test_nee_and_bsdf_agree_under_a_textured_light():
     for x, y in halves:
         assert x > 0 and abs(x - y) / x < 0.04, halves
     assert max(halves[0][1], halves[1][1]) > 1.15 * min(halves[0][1], halves[1][1]), halves
+
+
+def test_uniform_sample_distribution():
+    """uniform_hemisphere_sample (sample.hlsl:29-38): unit length, N.L equal
+    to r0 (the sample's cos theta), uniform azimuth, E[N.L] = 1/2."""
+    rng = np.random.default_rng(2)
+    n = np.array([-0.7, 0.2, 0.4], np.float32)
+    n = (n / np.linalg.norm(n)).astype(np.float32)
+    out = np.zeros(3, np.float32)
+    cos = []
+    for _ in range(20000):
+        r = rng.random(2).astype(np.float32)
+        O.olib().orc_uniform_sample(O.p(n), float(r[0]), float(r[1]), O.p(out))
+        assert abs(np.linalg.norm(out) - 1.0) < 1e-5
+        c = float(np.dot(out, n))
+        assert abs(c - float(r[0])) < 2e-5
+        cos.append(c)
+    cos = np.array(cos)
+    assert abs(cos.mean() - 0.5) < 0.01
+    assert abs(np.mean(cos ** 2) - 1.0 / 3.0) < 0.01   # cos theta uniform on [0, 1]
+
+
+def test_uniform_diffuse_furnace_and_estimator_agreement():
+    """With uniform hemisphere sampling (cosine_weighted_diffuse = 0) each
+    bounce weighs by 2 N.L albedo, whose mean is albedo: the furnace returns
+    albedo * E on average, and the lit scene's mean matches the cosine
+    estimator's to Monte Carlo noise."""
+    sc = furnace_scene(w=24, h=20)
+    orc = O.OracleScene(sc)
+    view = O.camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
+    hit = orc.trace_closest(O.primary_rays(view, sc.width, sc.height))["instance_idx"] != L.MISS
+    img = orc.render(max_bounces=4, spp=512, nee=0, env=1.5, uniform_diffuse=1)
+    got = img.reshape(-1, 4)[hit, :3].mean(axis=0)
+    expect = 1.5 * np.array([0.6, 0.5, 0.4])
+    assert np.allclose(got, expect, rtol=0.02), (got, expect)
+    sc2 = scenes.tlas_scene(seed=21, width=40, height=30, n_instances=24, hf_quads=12, ico_subdiv=2, soup_tris=300)
+    sc2.eye, sc2.target = (0.0, 8.0, -3.0), (0.0, 0.0, 0.0)
+    orc2 = O.OracleScene(sc2)
+    a = orc2.render(max_bounces=4, spp=128, nee=1, env=0.3)[..., :3].mean()
+    b = orc2.render(max_bounces=4, spp=128, nee=1, env=0.3, uniform_diffuse=1)[..., :3].mean()
+    assert a > 0 and abs(a - b) / a < 0.05, (a, b)
+    assert not np.array_equal(orc2.render(max_bounces=2, spp=1, env=0.3), orc2.render(max_bounces=2, spp=1, env=0.3,
+                                                                                         uniform_diffuse=1))

@@ -325,3 +325,28 @@ def test_full_size_frame_matches_oracle():
     same = np.mean(np.all(gpu[..., :3] == ref[..., :3], axis=-1))
     assert err <= REL_L1_TOL and same > 0.99, (err, same)
     assert stats.rays_extended > sc.width * sc.height
+
+
+@pytest.mark.parametrize("nee", [0, 1])
+def test_uniform_diffuse_frame_matches_oracle(nee):
+    """cosine_weighted_diffuse = 0 samples the hemisphere uniformly
+    (shade.hlsl:114-131): wavefront and megakernel frames match the oracle's
+    uniform-sampling render and each other bit for bit."""
+    sc = scenes.small_tlas_scene(width=96, height=64)
+    sc.eye = (0.0, 9.0, -14.0)
+    ref = O.OracleScene(sc).render(max_bounces=4, spp=2, nee=nee, env=0.4, frame_seed=6, nthreads=8, uniform_diffuse=1)
+    imgs = []
+    for wavefront in (1, 0):
+        r = Renderer(sc.width, sc.height)
+        try:
+            scenes.load(r, sc)
+            r.update_settings(max_bounces=4, spp=2, next_event_estimation=nee, hdr_env_strength=0.4, frame_seed=6,
+                              accumulate=0, use_wavefront_pathtracing=wavefront, cosine_weighted_diffuse=0)
+            r.render()
+            imgs.append(r.read_energy())
+        finally:
+            r.exit()
+    err = rel_l1(imgs[0], ref)
+    same = np.mean(np.all(imgs[0][..., :3] == ref[..., :3], axis=-1))
+    assert err <= REL_L1_TOL and same > 0.99, (err, same)
+    assert np.array_equal(imgs[0], imgs[1])

@@ -216,7 +216,6 @@ extern "C" WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_re
     if (!(s->rr_survival > 0.0f && s->rr_survival <= 1.0f)) return r->fail(WF_ERROR_INVALID_ARGUMENT, "rr_survival must be in (0, 1]");
     if (s->use_software_rt == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only software ray tracing is available");
     if (s->render_view_mode != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "debug view modes are not supported");
-    if (s->cosine_weighted_diffuse == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only cosine-weighted diffuse sampling is built");
     const bool reset = s->max_bounces != r->settings.max_bounces || s->spp != r->settings.spp ||
                        s->next_event_estimation != r->settings.next_event_estimation;
     if (std::memcmp(s, &r->settings, sizeof(*s)) != 0) r->forward_valid = false;

@@ -16,6 +16,7 @@ namespace wf {
 namespace shading {
 
 constexpr float kInvPi = 0.31830988f;   // common.hlsl:23
+constexpr float kInvTwoPi = 0.15915494f;   // common.hlsl:24
 constexpr float kRayMinT = 1e-8f;
 
 // ---- RNG: PCG32 (O'Neill 2014) seeded per (path, bounce) ------------------
@@ -104,6 +105,19 @@ __device__ __forceinline__ V3 cosine_sample(V3 n, float r0, float r1)
     float sp, cp;
     sincos_turns(r1, sp, cp);
     const V3 v = normalize3(V3{sin_theta * cp, sqrtf(1.0f - r0), sin_theta * sp});
+    const V3 t0 = (fabsf(n.x) > fabsf(n.z)) ? V3{-n.y, n.x, 0.0f} : V3{0.0f, -n.z, n.y};
+    const V3 t = normalize3(t0);
+    const V3 b = cross3(n, t);
+    return V3{(v.x * t.x + v.y * n.x) + v.z * b.x, (v.x * t.y + v.y * n.y) + v.z * b.y, (v.x * t.z + v.y * n.z) + v.z * b.z};
+}
+
+// uniform_hemisphere_sample + create_orthonormal_basis (sample.hlsl:13-43)
+__device__ __forceinline__ V3 uniform_sample(V3 n, float r0, float r1)
+{
+    const float sin_theta = sqrtf(1.0f - r0 * r0);
+    float sp, cp;
+    sincos_turns(r1, sp, cp);
+    const V3 v = normalize3(V3{sin_theta * cp, r0, sin_theta * sp});
     const V3 t0 = (fabsf(n.x) > fabsf(n.z)) ? V3{-n.y, n.x, 0.0f} : V3{0.0f, -n.z, n.y};
     const V3 t = normalize3(t0);
     const V3 b = cross3(n, t);
@@ -426,13 +440,14 @@ __device__ __forceinline__ void shade_vertex(const FrameArgs& A, uint32_t p, uin
             }
         }
         if (!done) {
-            const V3 L = cosine_sample(N, u[0], u[1]);
+            // shade.hlsl:108-135: cosine-weighted (default) or uniform hemisphere sample
+            const V3 L = A.uniform_diffuse ? uniform_sample(N, u[0], u[1]) : cosine_sample(N, u[0], u[1]);
             float NoL = dot3(N, L);
             if (NoL < 0.0f) NoL = 0.0f;
             if (!(NoL > 0.0f)) {
                 done = true;
             } else {
-                const float pdf = NoL * kInvPi;
+                const float pdf = A.uniform_diffuse ? kInvTwoPi : NoL * kInvPi;
                 const float inv_pdf = 1.0f / pdf;
                 thr.x = thr.x * ((NoL * (albedo.x * kInvPi)) * inv_pdf);
                 thr.y = thr.y * ((NoL * (albedo.y * kInvPi)) * inv_pdf);
