@@ -350,3 +350,25 @@ def test_uniform_diffuse_frame_matches_oracle(nee):
     same = np.mean(np.all(imgs[0][..., :3] == ref[..., :3], axis=-1))
     assert err <= REL_L1_TOL and same > 0.99, (err, same)
     assert np.array_equal(imgs[0], imgs[1])
+
+
+@pytest.mark.parametrize("size", [(37, 23), (5, 3), (8, 8), (64, 9)])
+def test_odd_image_sizes_match_oracle(size):
+    """Camera rays enter the first queue in 8x8 pixel tiles, then the right
+    and bottom strips: every pixel's paths are traced exactly once at sizes
+    that are not multiples of the tile, in both frame modes."""
+    w, h = size
+    sc = scenes.small_tlas_scene(width=w, height=h)
+    sc.eye = (0.0, 9.0, -14.0)
+    ref = O.OracleScene(sc).render(max_bounces=3, spp=3, nee=1, env=0.5, frame_seed=2, nthreads=8)
+    for wavefront in (1, 0):
+        r = Renderer(w, h)
+        try:
+            scenes.load(r, sc)
+            r.update_settings(max_bounces=3, spp=3, next_event_estimation=1, hdr_env_strength=0.5, frame_seed=2,
+                              accumulate=0, use_wavefront_pathtracing=wavefront)
+            r.render()
+            img = r.read_energy()
+            assert rel_l1(img, ref) <= REL_L1_TOL and np.all(img[..., :3] > 0) == np.all(ref[..., :3] > 0)
+        finally:
+            r.exit()

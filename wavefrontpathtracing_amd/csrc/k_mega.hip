@@ -67,9 +67,9 @@ __device__ __forceinline__ void mega_loop(const FrameArgs& A, const MegaLaunch& 
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
             const uint32_t i = base + rank;
             if (i >= n) continue;
-            // new path: the camera ray of its pixel (generate.hlsl:12-20)
-            p = A.path_base + i;
-            scene = scene_of_path(A, p);
+            // new path: the camera ray of its pixel (generate.hlsl:12-20),
+            // paths taken in the wavefront frame's first-queue order
+            p = path_of_position(A, i, scene);
             const GpuScene& gs = A.scenes[scene];
             root = gs.tlas_root;
             A.path_scene[p] = scene;

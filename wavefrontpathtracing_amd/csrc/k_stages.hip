@@ -38,14 +38,14 @@ __global__ void k_clear(FrameArgs A)
 }
 
 // generate.hlsl + make_primary_ray (common.hlsl:156-169): the local path q of
-// a scene covers pixel q % (W*H) (sample q / (W*H)); queue slot i holds path
-// path_base + i.
+// a scene covers pixel q % (W*H) (sample q / (W*H)); queue slot i holds the
+// path path_of_position gives (8x8 pixel tiles within each sample's image).
 __global__ void k_generate(FrameArgs A)
 {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n_paths; i += stride) {
-        const uint32_t p = A.path_base + i;
-        const uint32_t s = scene_of_path(A, p);
+        uint32_t s;
+        const uint32_t p = path_of_position(A, i, s);
         const GpuScene& sc = A.scenes[s];
         V3 o, dw;
         camera_ray(sc, (p - sc.path_off) % sc.n_pix, o, dw);

@@ -202,6 +202,38 @@ __device__ __forceinline__ uint32_t scene_of_path(const FrameArgs& A, uint32_t p
     return lo;
 }
 
+// The order in which camera rays enter the first queue: 8x8 pixel tiles
+// (one wave64 each) over the part of the image the tiles cover, then the
+// right strip and the bottom strip row by row.  A bijection on [0, W*H).
+// Only the queue order changes: a path keeps its id (and so its random
+// numbers and its pixel), so the image does not depend on it.
+__device__ __forceinline__ uint32_t tile_pixel(uint32_t k, uint32_t W, uint32_t H)
+{
+    const uint32_t W8 = W & ~7u, H8 = H & ~7u;
+    const uint32_t n_full = W8 * H8;
+    if (k < n_full) {
+        const uint32_t tile = k >> 6, local = k & 63u;
+        const uint32_t tiles_per_row = W8 >> 3;
+        return ((tile / tiles_per_row) * 8u + (local >> 3)) * W + (tile % tiles_per_row) * 8u + (local & 7u);
+    }
+    k -= n_full;
+    const uint32_t rw = W - W8;
+    const uint32_t n_right = rw * H8;
+    if (k < n_right) return (k / rw) * W + W8 + k % rw;
+    return H8 * W + (k - n_right);
+}
+
+// the path of first-queue position i (0-based in the frame) and its scene
+__device__ __forceinline__ uint32_t path_of_position(const FrameArgs& A, uint32_t i, uint32_t& scene)
+{
+    const uint32_t lin = A.path_base + i;
+    scene = scene_of_path(A, lin);
+    const GpuScene& sc = A.scenes[scene];
+    const uint32_t q = lin - sc.path_off;
+    const uint32_t sample = q / sc.n_pix, k = q - sample * sc.n_pix;
+    return sc.path_off + sample * sc.n_pix + tile_pixel(k, sc.width, sc.height);
+}
+
 __device__ __forceinline__ uint32_t scene_of_pixel(const FrameArgs& A, uint32_t q)
 {
     uint32_t lo = A.first_scene, hi = A.first_scene + A.n_scenes - 1u;
