@@ -100,10 +100,21 @@ typedef struct wf_material_asset {
  * cosine-weighted diffuse, accumulate, env strength 1, spp 1, Russian
  * roulette from bounce 3 with survival 0.75, next-event estimation off,
  * frame_seed 0 (0 = advance an internal xorshift per frame, renderer.cpp:739). */
+/* RENDER_VIEW_MODE (shared.hlsl.h:17-32): debug views that replace a camera
+ * ray's radiance with one property of what it hits (shade.hlsl:140-153). */
+typedef enum wf_render_view_mode {
+    WF_RENDER_VIEW_MODE_NONE = 0,
+    WF_RENDER_VIEW_MODE_GEOMETRY_INSTANCE, WF_RENDER_VIEW_MODE_GEOMETRY_PRIMITIVE, WF_RENDER_VIEW_MODE_GEOMETRY_BARYCENTRICS,
+    WF_RENDER_VIEW_MODE_GEOMETRY_NORMAL, WF_RENDER_VIEW_MODE_GEOMETRY_UV, WF_RENDER_VIEW_MODE_MATERIAL_BASE_COLOR,
+    WF_RENDER_VIEW_MODE_MATERIAL_NORMAL, WF_RENDER_VIEW_MODE_MATERIAL_METALLIC_ROUGHNESS, WF_RENDER_VIEW_MODE_MATERIAL_EMISSIVE,
+    WF_RENDER_VIEW_MODE_WORLD_NORMAL, WF_RENDER_VIEW_MODE_RENDER_TARGET_DEPTH,
+    WF_RENDER_VIEW_MODE_COUNT
+} wf_render_view_mode;
+
 typedef struct wf_render_settings {
     uint32_t use_wavefront_pathtracing;
     uint32_t use_software_rt;
-    uint32_t render_view_mode;
+    uint32_t render_view_mode;          /* wf_render_view_mode */
     uint32_t max_bounces;
     uint32_t cosine_weighted_diffuse;   /* 1: cosine-weighted hemisphere sampling; 0: uniform (shade.hlsl:114-131) */
     uint32_t accumulate;

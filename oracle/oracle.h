@@ -115,6 +115,8 @@ typedef struct {
     int32_t  base_color_tex;       /* -1: none (white) */
     uint32_t mesh;
     int32_t  emissive_tex;         /* -1: none; emission = emissive * texel (sample_material, material.hlsl:35-37) */
+    int32_t  normal_tex;           /* -1: none; read by the debug view modes only */
+    int32_t  metal_rough_tex;      /* -1: none; read by the debug view modes only */
 } orc_material_instance;
 
 typedef struct {
@@ -146,6 +148,7 @@ typedef struct {
     float    rr_survival;
     uint32_t path_base;            /* id of the first path (scenes of a batch number paths consecutively) */
     uint32_t uniform_diffuse;      /* 1: uniform hemisphere sampling (cosine_weighted_diffuse = 0), else cosine */
+    uint32_t view_mode;            /* RENDER_VIEW_MODE (shared.hlsl.h:17-32), 0 = the path tracer */
 } orc_render_params;
 
 /* energy_rgba: width*height*4 floats (the resolved per-pixel mean over spp). */
@@ -209,6 +212,7 @@ float    orc_sin_turns(float u);
 float    orc_cos_turns(float u);
 void     orc_cosine_sample(const float n[3], float r0, float r1, float out[3]);
 void     orc_uniform_sample(const float n[3], float r0, float r1, float out[3]);
+uint32_t orc_murmur_mix(uint32_t h);
 
 #ifdef __cplusplus
 }

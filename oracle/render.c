@@ -282,6 +282,89 @@ uint32_t orc_env_texel(const orc_shading* sh, const float d[3])
     return y * w + x;
 }
 
+/* murmur_mix (common.hlsl:276-284): MurmurHash3's 32-bit finaliser */
+uint32_t orc_murmur_mix(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+/* int_to_color (common.hlsl:286-295) */
+static void int_to_color(uint32_t v, float out[3])
+{
+    const uint32_t h = orc_murmur_mix(v);
+    const float s = 1.0f / 255.0f;
+    out[0] = (float)(h & 255u) * s;
+    out[1] = (float)((h >> 8) & 255u) * s;
+    out[2] = (float)((h >> 16) & 255u) * s;
+}
+
+/* The debug view modes (shade.hlsl:140-153) for a camera ray's hit.  A miss
+ * sees the reference's zeroed records: its ids, bary (0, 0) and t, zero
+ * geometry and material values, normalize(0) as its world normal; the
+ * material normal and metallic-roughness views of a miss are 0. */
+static void view_color(const render_job* J, const orc_hit* hit, float out[3])
+{
+    const orc_shading* sh = J->sh;
+    const uint32_t vm = J->p->view_mode;
+    out[0] = out[1] = out[2] = 0.0f;
+    if (vm == 1) { int_to_color(hit->instance_idx, out); return; }
+    if (vm == 2) { int_to_color(hit->primitive_idx, out); return; }
+    if (vm == 3) { out[0] = hit->bary[0]; out[1] = hit->bary[1]; return; }
+    if (vm == 11) { out[0] = out[1] = out[2] = hit->t / J->v->far_plane; return; }
+    if (hit->instance_idx == ~0u) {
+        if (vm == 10) {
+            const float z[3] = { 0.0f, 0.0f, 0.0f };
+            float n[3];
+            normalize3(z, n);
+            for (int k = 0; k < 3; ++k) out[k] = fabsf(n[k]);
+        }
+        return;
+    }
+    const orc_material_instance* m = &sh->mi[hit->instance_idx];
+    const float* t = sh->shade_tris + 24 * ((size_t)sh->shade_off[m->mesh] + hit->primitive_idx);
+    const float bx = hit->bary[0], by = hit->bary[1];
+    const float u = interp(t[18], t[20], t[22], bx, by), v = interp(t[19], t[21], t[23], bx, by);
+    float ln[3], n0[3], n[3], c[3];
+    for (int k = 0; k < 3; ++k) ln[k] = interp(t[9 + k], t[12 + k], t[15 + k], bx, by);
+    switch (vm) {
+    case 4: for (int k = 0; k < 3; ++k) out[k] = fabsf(ln[k]); break;
+    case 5: out[0] = u; out[1] = v; break;
+    case 6:
+        for (int k = 0; k < 3; ++k) out[k] = m->base_color[k];
+        if (m->base_color_tex >= 0) {
+            tex_sample(sh, m->base_color_tex, u, v, c, NULL);
+            for (int k = 0; k < 3; ++k) out[k] = out[k] * c[k];
+        }
+        break;
+    case 7:   /* default normal texture: RGBA8 (127, 127, 255, 255), renderer.cpp:256-262 */
+        c[0] = 127.0f / 255.0f; c[1] = 127.0f / 255.0f; c[2] = 1.0f;
+        if (m->normal_tex >= 0) tex_sample(sh, m->normal_tex, u, v, c, NULL);
+        for (int k = 0; k < 3; ++k) out[k] = fabsf(c[k]);
+        break;
+    case 8:   /* (0, g, b); default texture RG8 (255, 255), renderer.cpp:264-269 */
+        c[0] = 1.0f; c[1] = 1.0f; c[2] = 0.0f;
+        if (m->metal_rough_tex >= 0) tex_sample(sh, m->metal_rough_tex, u, v, c, NULL);
+        out[1] = c[1]; out[2] = c[2];
+        break;
+    case 9:
+        c[0] = c[1] = c[2] = 1.0f;
+        if (m->emissive_tex >= 0) tex_sample(sh, m->emissive_tex, u, v, c, NULL);
+        for (int k = 0; k < 3; ++k) out[k] = m->emissive[k] * c[k];
+        break;
+    case 10:
+        xform(m->local_to_world, ln, 0.0f, n0);
+        normalize3(n0, n);
+        for (int k = 0; k < 3; ++k) out[k] = fabsf(n[k]);
+        break;
+    default: break;
+    }
+}
+
 static int trace_path(render_job* J, uint32_t path, float energy[3], orc_vertex* rec)
 {
     const orc_render_params* P = J->p;
@@ -299,6 +382,10 @@ static int trace_path(render_job* J, uint32_t path, float energy[3], orc_vertex*
         orc_hit hit;
         int rc = orc_trace_one(J->s, &ray, &hit, 0);
         if (rc) return rc;
+        if (P->view_mode != 0) {   /* a debug view replaces the radiance and ends the path */
+            view_color(J, &hit, energy);
+            break;
+        }
         float u[6];
         orc_path_rng(path, d, P->frame_seed, u);
         if (hit.instance_idx == ~0u) {

@@ -37,7 +37,8 @@ class OrcView(C.Structure):
 
 class OrcMaterialInstance(C.Structure):
     _fields_ = [("local_to_world", C.c_float * 16), ("base_color", C.c_float * 3), ("emissive", C.c_float * 3),
-                ("base_color_tex", C.c_int32), ("mesh", C.c_uint32), ("emissive_tex", C.c_int32)]
+                ("base_color_tex", C.c_int32), ("mesh", C.c_uint32), ("emissive_tex", C.c_int32),
+                ("normal_tex", C.c_int32), ("metal_rough_tex", C.c_int32)]
 
 
 class OrcShading(C.Structure):
@@ -49,7 +50,7 @@ class OrcShading(C.Structure):
 class OrcRenderParams(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("spp", C.c_uint32), ("rr_start", C.c_uint32), ("nee", C.c_uint32),
                 ("frame_seed", C.c_uint32), ("rr_survival", C.c_float), ("path_base", C.c_uint32),
-                ("uniform_diffuse", C.c_uint32)]
+                ("uniform_diffuse", C.c_uint32), ("view_mode", C.c_uint32)]
 
 
 _o = None
@@ -85,6 +86,7 @@ def olib():
             "orc_cos_turns": (f32, [f32]),
             "orc_cosine_sample": (None, [vp, f32, f32, vp]),
             "orc_uniform_sample": (None, [vp, f32, f32, vp]),
+            "orc_murmur_mix": (u32, [u32]),
             "orc_record_paths": (C.c_int, [C.POINTER(OrcScene), C.POINTER(OrcShading), C.POINTER(OrcView), u32, u32,
                                            C.POINTER(OrcRenderParams), vp, C.c_int]),
             "orc_eval_f64": (C.c_int, [C.POINTER(OrcShading), C.POINTER(OrcRenderParams), u32, u32, vp, C.c_int32,
@@ -246,6 +248,8 @@ class OracleScene:
             mi[k].base_color_tex = ins.texture
             mi[k].mesh = ins.mesh
             mi[k].emissive_tex = getattr(ins, "emissive_texture", -1)
+            mi[k].normal_tex = getattr(ins, "normal_texture", -1)
+            mi[k].metal_rough_tex = getattr(ins, "metallic_roughness_texture", -1)
         self._mi = mi
         texs, toff, tw, th = [], [], [], []
         at = 0
@@ -269,11 +273,12 @@ class OracleScene:
         return sh
 
     def render(self, max_bounces=3, spp=1, rr_start=3, rr_survival=0.75, nee=0, frame_seed=1, env=1.0, nthreads=8,
-               path_base=0, uniform_diffuse=0):
+               path_base=0, uniform_diffuse=0, view_mode=0):
         sc = self.scene
         sh = self.shading(env)
         view = camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
-        prm = OrcRenderParams(max_bounces, spp, rr_start, nee, frame_seed, rr_survival, path_base, uniform_diffuse)
+        prm = OrcRenderParams(max_bounces, spp, rr_start, nee, frame_seed, rr_survival, path_base, uniform_diffuse,
+                              view_mode)
         out = np.zeros((sc.height, sc.width, 4), dtype=np.float32)
         rc = olib().orc_render(C.byref(self.s), C.byref(sh), C.byref(view), sc.width, sc.height, C.byref(prm), p(out), nthreads)
         if rc != 0:

@@ -278,3 +278,50 @@ def test_uniform_diffuse_furnace_and_estimator_agreement():
     assert a > 0 and abs(a - b) / a < 0.05, (a, b)
     assert not np.array_equal(orc2.render(max_bounces=2, spp=1, env=0.3), orc2.render(max_bounces=2, spp=1, env=0.3,
                                                                                          uniform_diffuse=1))
+
+
+def test_murmur_mix_known_answers():
+    """murmur_mix (common.hlsl:276-284) is MurmurHash3's fmix32: fmix32(0) = 0
+    and the published fmix32(1) = 0x514E28B7; it is a bijection (no
+    collisions over 2^16 inputs)."""
+    mm = O.olib().orc_murmur_mix
+    assert mm(0) == 0 and mm(1) == 0x514E28B7
+    assert len({mm(i) for i in range(1 << 16)}) == 1 << 16
+
+
+def _int_to_color(v):
+    h = O.olib().orc_murmur_mix(int(v))
+    s = np.float32(1.0) / np.float32(255.0)
+    return np.array([np.float32(h & 255) * s, np.float32((h >> 8) & 255) * s, np.float32((h >> 16) & 255) * s], np.float32)
+
+
+def test_debug_view_modes_follow_the_hits():
+    """Each debug view (shade.hlsl:140-153) against values derived from the
+    closest hits of the camera rays: instance / primitive colours through
+    int_to_color, barycentrics and depth t / far_plane exactly, base colour and
+    emission from the instance records, the default normal (127/255,
+    127/255, 1) and metallic-roughness (0, 1, 0) textures, unit world normals;
+    misses give their ids' colours, zeros and NaN world normals."""
+    sc = scenes.small_tlas_scene(width=32, height=24)
+    sc.eye, sc.target = (0.0, 3.0, -22.0), (0.0, 9.0, 0.0)   # sees instances, two lights and the sky
+    orc = O.OracleScene(sc)
+    view = O.camera_view(sc.eye, sc.target, sc.vfov_deg, sc.width, sc.height)
+    hits = orc.trace_closest(O.primary_rays(view, sc.width, sc.height))
+    hit = hits["instance_idx"] != L.MISS
+    assert 0.2 < hit.mean() < 0.98
+    img = {m: orc.render(max_bounces=3, spp=2, nee=1, view_mode=m).reshape(-1, 4)[:, :3] for m in range(1, 12)}
+    for k in range(len(hits)):
+        assert np.array_equal(img[1][k], _int_to_color(hits["instance_idx"][k]))
+        assert np.array_equal(img[2][k], _int_to_color(hits["primitive_idx"][k]))
+    assert np.array_equal(img[3][:, :2], hits["bary"]) and np.all(img[3][:, 2] == 0)
+    assert np.array_equal(img[11][:, 0], hits["t"] / np.float32(1000.0))
+    ins = sc.instances
+    bc = np.array([ins[i].base_color if h else (0, 0, 0) for i, h in zip(hits["instance_idx"], hit)], np.float32)
+    assert np.array_equal(img[6], bc)
+    em = np.array([np.float32(ins[i].emissive) * np.float32(ins[i].emissive_strength) if h else (0, 0, 0)
+                   for i, h in zip(hits["instance_idx"], hit)], np.float32)
+    assert np.array_equal(img[9], em) and em.any()
+    assert np.all(img[7][hit] == np.float32([127 / 255, 127 / 255, 1.0])) and np.all(img[7][~hit] == 0)
+    assert np.all(img[8][hit] == np.float32([0, 1, 0])) and np.all(img[8][~hit] == 0)
+    assert np.allclose(np.linalg.norm(img[10][hit], axis=1), 1.0, atol=1e-6) and np.all(np.isnan(img[10][~hit]))
+    assert np.all(img[4][~hit] == 0) and np.all(img[5][~hit] == 0) and np.all(img[4][hit].max(axis=1) > 0)

@@ -372,3 +372,36 @@ def test_odd_image_sizes_match_oracle(size):
             assert rel_l1(img, ref) <= REL_L1_TOL and np.all(img[..., :3] > 0) == np.all(ref[..., :3] > 0)
         finally:
             r.exit()
+
+
+def test_debug_view_modes_match_oracle():
+    """render_view_mode 1..11 (shade.hlsl:140-153) in both frame modes equal
+    the oracle's views bit for bit, with base-colour, normal,
+    metallic-roughness and emissive textures on some instances; the adjoint
+    refuses a view mode and unknown modes are rejected."""
+    sc = scenes.small_tlas_scene(width=48, height=32)
+    sc.eye, sc.target = (0.0, 3.0, -22.0), (0.0, 9.0, 0.0)
+    sc.textures = [(16, 16, scenes.value_noise_texture(k + 3, 16, 16, cells=4, lo=0.0, hi=1.0)) for k in range(4)]
+    for j, ins in enumerate(sc.instances):
+        if j % 3 == 0:
+            ins.texture, ins.normal_texture, ins.metallic_roughness_texture = 0, 1, 2
+        if any(ins.emissive) and j % 2 == 0:
+            ins.emissive_texture = 3
+    orc = O.OracleScene(sc)
+    for wavefront in (1, 0):
+        r = Renderer(sc.width, sc.height)
+        try:
+            scenes.load(r, sc)
+            for m in range(1, 12):
+                r.update_settings(max_bounces=3, spp=2, next_event_estimation=1, hdr_env_strength=0.5, frame_seed=4,
+                                  accumulate=0, use_wavefront_pathtracing=wavefront, render_view_mode=m)
+                r.render()
+                got = r.read_energy()[..., :3]
+                want = orc.render(max_bounces=3, spp=2, nee=1, env=0.5, frame_seed=4, view_mode=m)[..., :3]
+                assert np.array_equal(got, want, equal_nan=True), (wavefront, m, np.nanmax(np.abs(got - want)))
+            with pytest.raises(L.WfError):
+                r.render_adjoint(np.zeros((sc.height, sc.width, 4), np.float32), L.INVALID_HANDLE, None, len(sc.instances))
+            with pytest.raises(L.WfError):
+                r.update_settings(render_view_mode=12)
+        finally:
+            r.exit()

@@ -15,6 +15,8 @@ static wf_status adjoint_run(wf_renderer* r, const float* d_dl, int32_t tex_slot
 {
     if (!r->forward_valid)
         return r->fail(WF_ERROR_INVALID_STATE, "render_adjoint needs a render() of the same scene and settings just before it");
+    if (r->settings.render_view_mode != 0)
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "debug view modes have no adjoint");
     FrameArgs A = r->last_args;
     const size_t n_paths = r->frame_capacity;
     hipError_t e = hipSuccess;

@@ -382,6 +382,7 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
     A.rr_start = settings.rr_start_bounce;
     A.nee = settings.next_event_estimation;
     A.uniform_diffuse = settings.cosine_weighted_diffuse ? 0u : 1u;
+    A.view_mode = settings.render_view_mode;
     A.rr_survival = settings.rr_survival;
     for (int k = 0; k < 3; ++k) A.env[k] = settings.hdr_env_strength;   // default 1x1 white environment
     // begin_scene's environment texture; a missing one falls back to the
@@ -427,7 +428,7 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
             (uint64_t)(uintptr_t)d_tnodes.p, (uint64_t)(uintptr_t)d_tpairs.p, ext_cfg.refill_min, ext_cfg.tlas_lds, ext_cfg.visits, ext_grid_tlds, n_tpairs, (uint64_t)(uintptr_t)d_inst.p, (uint64_t)(uintptr_t)d_shade_inst.p,
             (uint64_t)(uintptr_t)d_emitters.p, (uint64_t)(uintptr_t)d_textures.p, (uint64_t)(uintptr_t)d_spill.p,
             (uint64_t)(uintptr_t)d_rays[0].p, (uint64_t)(uintptr_t)d_image.p, slots.size(), ext_cfg.layout, settings.separate_connect, (uint64_t)(uint32_t)A.env_tex,
-            (uint64_t)(uintptr_t)d_tquads.p, A.uniform_diffuse};
+            (uint64_t)(uintptr_t)d_tquads.p, A.uniform_diffuse, A.view_mode};
         if (!graph_exec || key != graph_key) {
             drop_graph();
             hipGraph_t g = nullptr;

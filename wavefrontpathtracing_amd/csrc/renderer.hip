@@ -215,7 +215,8 @@ extern "C" WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_re
     if (s->spp == 0 || s->spp > 4096) return r->fail(WF_ERROR_INVALID_ARGUMENT, "spp must be in [1, 4096]");
     if (!(s->rr_survival > 0.0f && s->rr_survival <= 1.0f)) return r->fail(WF_ERROR_INVALID_ARGUMENT, "rr_survival must be in (0, 1]");
     if (s->use_software_rt == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only software ray tracing is available");
-    if (s->render_view_mode != 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "debug view modes are not supported");
+    if (s->render_view_mode >= WF_RENDER_VIEW_MODE_COUNT)
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "render_view_mode %u is not a view mode", s->render_view_mode);
     const bool reset = s->max_bounces != r->settings.max_bounces || s->spp != r->settings.spp ||
                        s->next_event_estimation != r->settings.next_event_estimation;
     if (std::memcmp(s, &r->settings, sizeof(*s)) != 0) r->forward_valid = false;
@@ -556,6 +557,9 @@ wf_status wf_renderer::upload_scene()
             const Texture* et = find_texture(rec.material.emissive_texture);
             sh.base_color_tex = bt ? (int32_t)handle_index(rec.material.base_color_texture) : -1;
             sh.emissive_tex = et ? (int32_t)handle_index(rec.material.emissive_texture) : -1;
+            sh.normal_tex = find_texture(rec.material.normal_texture) ? (int32_t)handle_index(rec.material.normal_texture) : -1;
+            sh.metal_rough_tex = find_texture(rec.material.metallic_roughness_texture)
+                                     ? (int32_t)handle_index(rec.material.metallic_roughness_texture) : -1;
             sh.tris = (const GpuShadeTri*)m.shade.p;
             si.push_back(sh);
         }

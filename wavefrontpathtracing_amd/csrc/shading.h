@@ -321,12 +321,108 @@ __device__ __forceinline__ uint32_t env_texel(const GpuTexture& T, const V3& d)
     return y * T.width + x;
 }
 
+// murmur_mix (common.hlsl:276-284): MurmurHash3's 32-bit finaliser
+__device__ __forceinline__ uint32_t murmur_mix(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// int_to_color (common.hlsl:286-295)
+__device__ __forceinline__ V3 int_to_color(uint32_t v)
+{
+    const uint32_t h = murmur_mix(v);
+    const float s = 1.0f / 255.0f;
+    return V3{(float)(h & 255u) * s, (float)((h >> 8) & 255u) * s, (float)((h >> 16) & 255u) * s};
+}
+
+// The debug view modes (shade.hlsl:140-153): one property of the camera
+// ray's hit replaces its radiance.  A miss sees what the reference reads for
+// an invalid instance (zeroed records): its ids, barycentrics (0, 0) and t;
+// zero geometry and material values; a world normal of normalize(0) (NaN,
+// which the post-process paints magenta).  The material normal and
+// metallic-roughness views of a miss, which the reference takes from
+// whatever texture sits in descriptor slot 0, are 0 here.
+__device__ __forceinline__ V3 view_color(const FrameArgs& A, uint32_t scene, const wf_hit& hit)
+{
+    const uint32_t m = A.view_mode;
+    if (m == WF_RENDER_VIEW_MODE_GEOMETRY_INSTANCE) return int_to_color(hit.instance_idx);
+    if (m == WF_RENDER_VIEW_MODE_GEOMETRY_PRIMITIVE) return int_to_color(hit.primitive_idx);
+    if (m == WF_RENDER_VIEW_MODE_GEOMETRY_BARYCENTRICS) return V3{hit.bary[0], hit.bary[1], 0.0f};
+    if (m == WF_RENDER_VIEW_MODE_RENDER_TARGET_DEPTH) {
+        const float z = hit.t / A.scenes[scene].view.far_plane;
+        return V3{z, z, z};
+    }
+    if (hit.instance_idx == 0xFFFFFFFFu) {
+        if (m == WF_RENDER_VIEW_MODE_WORLD_NORMAL) {
+            const V3 n = normalize3(V3{0.0f, 0.0f, 0.0f});
+            return V3{fabsf(n.x), fabsf(n.y), fabsf(n.z)};
+        }
+        return V3{0.0f, 0.0f, 0.0f};
+    }
+    const GpuShadeInstance* si = A.shade_inst + hit.instance_idx;
+    const GpuShadeTri* tri = si->tris + hit.primitive_idx;
+    const float bx = hit.bary[0], by = hit.bary[1];
+    const float tu = interp(tri->uv[0][0], tri->uv[1][0], tri->uv[2][0], bx, by);
+    const float tv = interp(tri->uv[0][1], tri->uv[1][1], tri->uv[2][1], bx, by);
+    switch (m) {
+    case WF_RENDER_VIEW_MODE_GEOMETRY_NORMAL:
+    case WF_RENDER_VIEW_MODE_WORLD_NORMAL: {
+        const V3 ln{interp(tri->n[0][0], tri->n[1][0], tri->n[2][0], bx, by),
+                    interp(tri->n[0][1], tri->n[1][1], tri->n[2][1], bx, by),
+                    interp(tri->n[0][2], tri->n[1][2], tri->n[2][2], bx, by)};
+        const V3 n = m == WF_RENDER_VIEW_MODE_WORLD_NORMAL ? normalize3(xform(si->local_to_world, ln, 0.0f)) : ln;
+        return V3{fabsf(n.x), fabsf(n.y), fabsf(n.z)};
+    }
+    case WF_RENDER_VIEW_MODE_GEOMETRY_UV: return V3{tu, tv, 0.0f};
+    case WF_RENDER_VIEW_MODE_MATERIAL_BASE_COLOR: {
+        V3 c{si->base_color[0], si->base_color[1], si->base_color[2]};
+        if (si->base_color_tex >= 0) {
+            const V3 tc = tex_sample(A.textures[si->base_color_tex], tu, tv);
+            c = V3{c.x * tc.x, c.y * tc.y, c.z * tc.z};
+        }
+        return c;
+    }
+    case WF_RENDER_VIEW_MODE_MATERIAL_NORMAL: {
+        // the default normal texture is RGBA8 (127, 127, 255, 255) (renderer.cpp:256-262)
+        const V3 n = si->normal_tex >= 0 ? tex_sample(A.textures[si->normal_tex], tu, tv)
+                                         : V3{127.0f / 255.0f, 127.0f / 255.0f, 1.0f};
+        return V3{fabsf(n.x), fabsf(n.y), fabsf(n.z)};
+    }
+    case WF_RENDER_VIEW_MODE_MATERIAL_METALLIC_ROUGHNESS: {
+        // (0, roughness = g, metallic = b); the default texture is RG8 (255, 255) (renderer.cpp:264-269)
+        const V3 t = si->metal_rough_tex >= 0 ? tex_sample(A.textures[si->metal_rough_tex], tu, tv) : V3{1.0f, 1.0f, 0.0f};
+        return V3{0.0f, t.y, t.z};
+    }
+    case WF_RENDER_VIEW_MODE_MATERIAL_EMISSIVE: {
+        const float es = si->emissive_strength;
+        V3 etc{1.0f, 1.0f, 1.0f};
+        if (si->emissive_tex >= 0) etc = tex_sample(A.textures[si->emissive_tex], tu, tv);
+        return V3{(si->emissive_factor[0] * es) * etc.x, (si->emissive_factor[1] * es) * etc.y,
+                  (si->emissive_factor[2] * es) * etc.z};
+    }
+    default: return V3{0.0f, 0.0f, 0.0f};
+    }
+}
+
 // dir: the direction of the ray that reached this vertex (read for misses only)
 template <bool ADJ>
 __device__ __forceinline__ void shade_vertex(const FrameArgs& A, uint32_t p, uint32_t scene, uint32_t d, uint32_t seed,
                                              const wf_hit& hit, const V3& dir, V3& thr, float4& e4, const float4& dlt,
                                              VertexOut& o)
 {
+    if (!ADJ && A.view_mode != 0u) {
+        // a debug view replaces the camera ray's radiance and ends the path
+        const V3 c = view_color(A, scene, hit);
+        e4.x = c.x;
+        e4.y = c.y;
+        e4.z = c.z;
+        return;
+    }
     float u[6];
     path_rng(p, d, seed, u);
     bool done = false;

@@ -44,6 +44,7 @@ struct FrameArgs {
     uint32_t pix_base, n_pix;      // batch pixels [pix_base, pix_base + n_pix)
     uint32_t spp, max_bounces, rr_start, nee;
     uint32_t uniform_diffuse;    // 1: uniform hemisphere sampling (cosine_weighted_diffuse = 0)
+    uint32_t view_mode;          // render_view_mode (0 = the path tracer)
     float rr_survival;
     float env[3];                // constant environment (hdr_env_strength x the 1x1 white fallback)
     int32_t env_tex;             // equirect environment texture slot, -1 = the constant

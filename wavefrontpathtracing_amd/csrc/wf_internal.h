@@ -67,8 +67,10 @@ struct alignas(16) GpuShadeInstance {
     float emissive_factor[3]; int32_t emissive_tex;     // emission = emissive_factor * emissive_strength
     float emissive_strength; uint32_t pad;
     const GpuShadeTri* tris;
+    int32_t normal_tex, metal_rough_tex;                 // read by the debug view modes only, -1 = none
+    uint32_t pad2[2];
 };
-static_assert(sizeof(GpuShadeInstance) == 112, "GpuShadeInstance is 112 B");
+static_assert(sizeof(GpuShadeInstance) == 128, "GpuShadeInstance is 128 B");
 
 struct GpuTexture {
     const float* texels;   // RGBA float, row-major

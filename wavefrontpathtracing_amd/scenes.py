@@ -35,6 +35,8 @@ class Instance:
     emissive_strength: float = 0.0
     texture: int = -1                # index into Scene.textures, -1 = none
     emissive_texture: int = -1       # index into Scene.textures, -1 = none (emission = factor * strength * texel)
+    normal_texture: int = -1         # index into Scene.textures, -1 = none (read by the debug view modes only)
+    metallic_roughness_texture: int = -1   # likewise
 
 
 @dataclass
@@ -269,6 +271,9 @@ def load(renderer, scene: Scene, begin: bool = True):
         mat = Material(base_color_factor=(*ins.base_color, 1.0), emissive_factor=ins.emissive,
                        emissive_strength=ins.emissive_strength,
                        base_color_texture=tex_h[ins.texture] if ins.texture >= 0 else L.INVALID_HANDLE,
-                       emissive_texture=tex_h[ins.emissive_texture] if ins.emissive_texture >= 0 else L.INVALID_HANDLE)
+                       emissive_texture=tex_h[ins.emissive_texture] if ins.emissive_texture >= 0 else L.INVALID_HANDLE,
+                       normal_texture=tex_h[ins.normal_texture] if ins.normal_texture >= 0 else L.INVALID_HANDLE,
+                       metallic_roughness_texture=(tex_h[ins.metallic_roughness_texture]
+                                                   if ins.metallic_roughness_texture >= 0 else L.INVALID_HANDLE))
         renderer.submit_render_mesh(mesh_h[ins.mesh], ins.transform, mat)
     return mesh_h, tex_h
