@@ -77,8 +77,10 @@ typedef struct wf_view {
     float render_dim[2]; float near_plane; float far_plane;
 } wf_view;
 
-/* TEXTURE_FORMAT subset accepted by create_render_texture */
-enum { WF_TEXTURE_FORMAT_RGBA8 = 1, WF_TEXTURE_FORMAT_RGBA8_SRGB = 2, WF_TEXTURE_FORMAT_RGB32F = 3, WF_TEXTURE_FORMAT_RGBA32F = 4 };
+/* TEXTURE_FORMAT subset accepted by create_render_texture (renderer_fwd.h:16-34;
+ * the block-compressed BC formats are not accepted) */
+enum { WF_TEXTURE_FORMAT_RGBA8 = 1, WF_TEXTURE_FORMAT_RGBA8_SRGB = 2, WF_TEXTURE_FORMAT_RGB32F = 3, WF_TEXTURE_FORMAT_RGBA32F = 4,
+       WF_TEXTURE_FORMAT_RG8 = 5 };
 
 /* renderer::render_texture_params_t (renderer.h:33-42) */
 typedef struct wf_render_texture_params { uint32_t width; uint32_t height; uint32_t bits_per_pixel; uint32_t format; const void* ptr_data; } wf_render_texture_params;

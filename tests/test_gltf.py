@@ -106,3 +106,21 @@ def test_helmet_blas_matches_oracle():
     tri = v["position"][i.reshape(-1, 3)].reshape(-1, 9).astype(np.float32)
     for bins in (8, 16):
         assert R.bvh_build(tri, interval_count=bins) == O.blas_blob(tri, bins)
+
+
+@pytest.mark.skipif(not os.path.exists(HELMET), reason="the reference's assets are not on this machine")
+def test_helmet_material_textures():
+    """DamagedHelmet's material: base colour and emissive maps decoded as
+    sRGB, normal and metallic-roughness maps as linear (asset_loader.cpp:
+    404-408), each in its own texture slot."""
+    sc = gltf.load(HELMET)
+    ins = sc.instances[0]
+    slots = [ins.texture, ins.normal_texture, ins.metallic_roughness_texture, ins.emissive_texture]
+    assert min(slots) >= 0 and len(set(slots)) == 4 and len(sc.textures) == 4
+    assert ins.emissive_strength > 0
+    for k in slots:
+        w, h, rgb = sc.textures[k]
+        assert rgb.shape == (h, w, 3) and rgb.dtype == np.float32 and 0.0 <= rgb.min() and rgb.max() <= 1.0
+    # a linear map keeps byte / 255; the normal map's mean blue is close to 1 (tangent-space z)
+    n = sc.textures[ins.normal_texture][2]
+    assert np.allclose(n * 255.0, np.round(n * 255.0), atol=1e-3) and n[..., 2].mean() > 0.7

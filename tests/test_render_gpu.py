@@ -405,3 +405,46 @@ def test_debug_view_modes_match_oracle():
                 r.update_settings(render_view_mode=12)
         finally:
             r.exit()
+
+
+def test_texture_formats_read_back_through_the_base_color_view():
+    """create_render_texture's formats (renderer_fwd.h:16-34): a 1x1 texture
+    as every instance's base-colour texture, read through the base-colour
+    view mode, gives factor x texel with RGBA8 = byte / 255, RGBA8_SRGB the
+    sRGB decode of it, RG8 (r, g, 0), and the float formats as given."""
+    from wavefrontpathtracing_amd import renderer as RR
+    sc = scenes.small_tlas_scene(width=24, height=16)
+    sc.eye, sc.target = (0.0, 3.0, -22.0), (0.0, 9.0, 0.0)
+    b8 = np.array([200, 30, 255, 128], np.uint8)
+    lin = b8[:3].astype(np.float32) / np.float32(255.0)
+    srgb = np.where(lin <= 0.04045, lin / np.float32(12.92), ((lin + np.float32(0.055)) / np.float32(1.055)) ** 2.4)
+    cases = [(RR.TEXTURE_FORMAT_RGBA8, b8, lin), (RR.TEXTURE_FORMAT_RGBA8_SRGB, b8, srgb),
+             (RR.TEXTURE_FORMAT_RG8, b8[:2].copy(), np.array([lin[0], lin[1], 0.0], np.float32)),
+             (RR.TEXTURE_FORMAT_RGB32F, np.array([0.25, 1.5, 0.0625], np.float32), np.float32([0.25, 1.5, 0.0625])),
+             (RR.TEXTURE_FORMAT_RGBA32F, np.array([0.5, 0.125, 2.0, 1.0], np.float32), np.float32([0.5, 0.125, 2.0]))]
+    r = Renderer(sc.width, sc.height)
+    try:
+        mesh_h = [r.create_render_mesh(v, i) for v, i in sc.meshes]
+        for fmt, data, want in cases:
+            tex = r.create_render_texture(1, 1, data, fmt)
+            r.begin_scene(RR.camera_create(sc.eye, sc.target, sc.vfov_deg))
+            for ins in sc.instances:
+                r.submit_render_mesh(mesh_h[ins.mesh], ins.transform,
+                                     RR.Material(base_color_factor=(*ins.base_color, 1.0), base_color_texture=tex))
+            r.end_scene()
+            r.update_settings(render_view_mode=6, spp=1, accumulate=0)
+            r.render()
+            img = r.read_energy().reshape(-1, 4)[:, :3]
+            lit = np.any(img != 0, axis=1)
+            assert lit.mean() > 0.1
+            bc = np.array([ins.base_color for ins in sc.instances], np.float32)
+            ok = np.zeros(lit.sum(), bool)
+            for c in bc:   # every lit pixel is some instance's factor x the texel
+                ok |= np.all(np.isclose(img[lit], c * want, rtol=2e-6, atol=0) | (want == 0), axis=1)
+            assert ok.all(), (fmt, img[lit][~ok][:3])
+        import ctypes as C
+        p = L.RenderTextureParams(1, 1, 32, 9, L.ptr(b8))
+        out = C.c_uint64()
+        assert L.lib().wf_renderer_create_render_texture(r.handle, C.byref(p), C.byref(out)) == 1   # unknown format
+    finally:
+        r.exit()

@@ -327,6 +327,16 @@ extern "C" WF_API wf_status wf_renderer_create_render_texture(wf_renderer* r, co
         }
         break;
     }
+    case WF_TEXTURE_FORMAT_RG8: {   // two unorm channels; blue 0 and alpha 1 as a sampled RG8 texture reads
+        const uint8_t* s = (const uint8_t*)p->ptr_data;
+        for (size_t i = 0; i < texels; ++i) {
+            rgba[4 * i] = (float)s[2 * i] / 255.0f;
+            rgba[4 * i + 1] = (float)s[2 * i + 1] / 255.0f;
+            rgba[4 * i + 2] = 0.0f;
+            rgba[4 * i + 3] = 1.0f;
+        }
+        break;
+    }
     case WF_TEXTURE_FORMAT_RGB32F: {
         const float* s = (const float*)p->ptr_data;
         for (size_t i = 0; i < texels; ++i) { rgba[4 * i] = s[3 * i]; rgba[4 * i + 1] = s[3 * i + 1]; rgba[4 * i + 2] = s[3 * i + 2]; rgba[4 * i + 3] = 1.0f; }

@@ -6,8 +6,9 @@ What is read, following the reference loader:
 - every primitive of every mesh becomes one render mesh (triangles only):
   POSITION and NORMAL are required, TEXCOORD_0 is read (zeros when absent),
   indices as u8/u16/u32 (a non-indexed primitive gets 0..n-1);
-- materials: pbrMetallicRoughness.baseColorFactor and baseColorTexture,
-  emissiveFactor, emissiveTexture and KHR_materials_emissive_strength
+- materials: pbrMetallicRoughness.baseColorFactor, baseColorTexture and
+  metallicRoughnessTexture, normalTexture, emissiveFactor, emissiveTexture
+  and KHR_materials_emissive_strength
   (default 1, as cgltf); as in the reference, the emissive fields are read
   only for materials that carry a pbrMetallicRoughness block;
 - every node with a mesh submits each of its primitives with the node's
@@ -104,7 +105,7 @@ def _node_matrix(node) -> np.ndarray:
     return m
 
 
-def _decode_image(doc, buffers, base_dir, img_idx):
+def _decode_image(doc, buffers, base_dir, img_idx, srgb=True):
     from PIL import Image
     import io
     img = doc["images"][img_idx]
@@ -121,6 +122,8 @@ def _decode_image(doc, buffers, base_dir, img_idx):
         blob = buffers[bv["buffer"]][o:o + bv["byteLength"]]
     im = Image.open(io.BytesIO(blob)).convert("RGB")
     rgb8 = np.asarray(im, dtype=np.uint8)
+    if not srgb:
+        return im.width, im.height, rgb8.astype(np.float32) / np.float32(255.0)
     return im.width, im.height, _srgb_table()[rgb8]
 
 
@@ -154,30 +157,33 @@ def load(path: str, width: int = 1280, height: int = 720, vfov_deg: float = 60.0
     # (asset_loader.cpp:409-470)
     tex_of_image, scene_textures = {}, []
 
-    def texture_of(ref):
+    def texture_of(ref, srgb=True):
         if not textures or ref is None:
             return -1
         src = doc["textures"][ref["index"]].get("source")
         if src is None:
             return -1
-        if src not in tex_of_image:
-            tex_of_image[src] = len(scene_textures)
-            scene_textures.append(_decode_image(doc, buffers, base_dir, src))
-        return tex_of_image[src]
+        if (src, srgb) not in tex_of_image:
+            tex_of_image[(src, srgb)] = len(scene_textures)
+            scene_textures.append(_decode_image(doc, buffers, base_dir, src, srgb))
+        return tex_of_image[(src, srgb)]
 
     mats = []
     for m in doc.get("materials", []):
         pbr = m.get("pbrMetallicRoughness")
         if pbr is None:
-            mats.append(((1.0, 1.0, 1.0), -1, (0.0, 0.0, 0.0), 0.0, -1))
+            mats.append(((1.0, 1.0, 1.0), -1, (0.0, 0.0, 0.0), 0.0, -1, -1, -1))
             continue
         bc = pbr.get("baseColorFactor", [1.0, 1.0, 1.0, 1.0])
         tex = texture_of(pbr.get("baseColorTexture"))
         strength = m.get("extensions", {}).get("KHR_materials_emissive_strength", {}).get("emissiveStrength", 1.0)
         em = tuple(float(x) for x in m.get("emissiveFactor", [0, 0, 0]))
         etex = texture_of(m.get("emissiveTexture")) if any(em) else -1
-        mats.append((tuple(float(x) for x in bc[:3]), tex, em, float(strength), etex))
-    default_mat = ((1.0, 1.0, 1.0), -1, (0.0, 0.0, 0.0), 1.0, -1)
+        # normal and metallic-roughness maps are linear (asset_loader.cpp:404-408); the debug views read them
+        ntex = texture_of(m.get("normalTexture"), srgb=False)
+        mrtex = texture_of(pbr.get("metallicRoughnessTexture"), srgb=False)
+        mats.append((tuple(float(x) for x in bc[:3]), tex, em, float(strength), etex, ntex, mrtex))
+    default_mat = ((1.0, 1.0, 1.0), -1, (0.0, 0.0, 0.0), 1.0, -1, -1, -1)
 
     # primitives -> meshes
     meshes, prim_index, prim_material = [], {}, {}
@@ -220,11 +226,12 @@ def load(path: str, width: int = 1280, height: int = 720, vfov_deg: float = 60.0
                 if key not in prim_index:
                     continue
                 mat_i = prim_material[key]
-                bc, tex, em, strength, etex = mats[mat_i] if mat_i is not None else default_mat
+                bc, tex, em, strength, etex, ntex, mrtex = mats[mat_i] if mat_i is not None else default_mat
                 instances.append(scenes.Instance(prim_index[key], world.T.astype(np.float32).reshape(16).copy(),
                                                  base_color=bc, emissive=em,
                                                  emissive_strength=strength if any(em) else 0.0, texture=tex,
-                                                 emissive_texture=etex))
+                                                 emissive_texture=etex, normal_texture=ntex,
+                                                 metallic_roughness_texture=mrtex))
         for c in reversed(node.get("children", [])):
             stack.append((c, world))
 

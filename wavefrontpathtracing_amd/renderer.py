@@ -20,6 +20,7 @@ TEXTURE_FORMAT_RGBA8 = 1
 TEXTURE_FORMAT_RGBA8_SRGB = 2
 TEXTURE_FORMAT_RGB32F = 3
 TEXTURE_FORMAT_RGBA32F = 4
+TEXTURE_FORMAT_RG8 = 5
 
 
 @dataclass
@@ -192,7 +193,8 @@ class Renderer:
     # ---- resources ----
     def create_render_texture(self, width: int, height: int, data: np.ndarray, fmt: int = TEXTURE_FORMAT_RGB32F) -> int:
         data = np.ascontiguousarray(data)
-        bpp = {TEXTURE_FORMAT_RGBA8: 32, TEXTURE_FORMAT_RGBA8_SRGB: 32, TEXTURE_FORMAT_RGB32F: 96, TEXTURE_FORMAT_RGBA32F: 128}[fmt]
+        bpp = {TEXTURE_FORMAT_RGBA8: 32, TEXTURE_FORMAT_RGBA8_SRGB: 32, TEXTURE_FORMAT_RGB32F: 96, TEXTURE_FORMAT_RGBA32F: 128,
+               TEXTURE_FORMAT_RG8: 16}[fmt]
         need = width * height * bpp // 8
         if data.nbytes != need:
             raise WfError(1, f"texture data holds {data.nbytes} bytes, expected {need}")
