@@ -590,6 +590,25 @@ def main():
         r.set_tlas_metric("reference")
         r.build_tlas()
 
+    # the scene's other ray set: first-bounce rays from the hits of the 1920x1080 camera rays
+    cam = scenes.primary_rays(sc)
+    b1 = scenes.bounce_rays(sc, cam, r.trace_closest(cam))
+    d_b1 = torch.from_numpy(b1.view(np.uint8)).to(dev)
+    d_b1h = torch.empty(len(b1) * 20, dtype=torch.uint8, device=dev)
+    for _ in range(args.warmup):
+        r.trace_closest_device(d_b1.data_ptr(), len(b1), d_b1h.data_ptr())
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        r.trace_closest_device(d_b1.data_ptr(), len(b1), d_b1h.data_ptr())
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    b1_ms = ev0.elapsed_time(ev1) / args.steps
+    b1_hits = d_b1h.cpu().numpy().view(L.HIT)
+    bounce1 = {"rays": len(b1), "mrays": round(len(b1) / (b1_ms * 1e-3) / 1e6, 2), "ms_per_step": round(b1_ms, 4),
+               "hit_fraction": round(float((b1_hits["instance_idx"] != L.MISS).mean()), 4),
+               "source": "cosine-distributed directions about the hit normals of the 1920x1080 camera rays"}
+
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -614,6 +633,7 @@ def main():
             "host_arrays_mrays": round(host_mrays, 2),
             "wall_s_timed": round(t_wall, 4),
             "union_tlas": union,
+            "bounce1": bounce1,
         },
         "roofline": {
             "bound": "hbm",

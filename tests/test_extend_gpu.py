@@ -239,8 +239,9 @@ def test_full_tlas_scene_subset():
 def test_benchmark_workload_bitexact_at_full_size():
     """The extend benchmark's own workload at full size: the 256-instance
     scene and all 2^21 incoherent rays (the same seed and bounds as the
-    benchmark), plus all 1920x1080 camera rays, bit-exact against the oracle,
-    with the reference's TLAS and with the union-metric TLAS."""
+    benchmark), all 1920x1080 camera rays and their first-bounce rays,
+    bit-exact against the oracle, with the reference's TLAS and with the
+    union-metric TLAS."""
     sc = scenes.tlas_scene()
     r = _setup(sc)
     try:
@@ -251,10 +252,14 @@ def test_benchmark_workload_bitexact_at_full_size():
         want_i = orc.trace_closest(rays, nthreads=16)
         want_c = orc.trace_closest(cam, nthreads=16)
         assert (want_i["instance_idx"] != L.MISS).mean() > 0.4
+        b1 = scenes.bounce_rays(sc, cam, want_c)
+        want_b = orc.trace_closest(b1, nthreads=16)
+        assert len(b1) > 5 * 10 ** 5 and (want_b["instance_idx"] != L.MISS).mean() > 0.2
         for metric in ("reference", "union"):
             r.set_tlas_metric(metric)
             r.build_tlas()
             _assert_bitexact(r.trace_closest(rays), want_i, f"2^21 incoherent rays, {metric} TLAS")
             _assert_bitexact(r.trace_closest(cam), want_c, f"1080p camera rays, {metric} TLAS")
+            _assert_bitexact(r.trace_closest(b1), want_b, f"first-bounce rays, {metric} TLAS")
     finally:
         r.exit()

@@ -122,6 +122,47 @@ def primary_rays(scene: "Scene") -> np.ndarray:
     return out
 
 
+def bounce_rays(scene: "Scene", rays: np.ndarray, hits: np.ndarray, seed: int = 7) -> np.ndarray:
+    """First-bounce rays of a path tracer: for every ray of `rays` that hit
+    (closest hits `hits`), a cosine-distributed direction about the hit's
+    interpolated world normal, from the hit point nudged by 1e-8 along the
+    new direction (make_ray, common.hlsl:145-154).  Directions come from a
+    seeded numpy generator, so the set is fixed for a scene, ray set and
+    seed; it is a workload, not the path tracer's own random numbers."""
+    hit = np.nonzero(hits["instance_idx"] != 0xFFFFFFFF)[0]
+    inst, prim, bary = hits["instance_idx"][hit], hits["primitive_idx"][hit], hits["bary"][hit].astype(np.float64)
+    o = rays["origin"][hit].astype(np.float64)
+    d = rays["direction"][hit].astype(np.float64)
+    P = o + hits["t"][hit, None].astype(np.float64) * d
+    N = np.zeros_like(P)
+    for k, ins in enumerate(scene.instances):
+        sel = np.nonzero(inst == k)[0]
+        if len(sel) == 0:
+            continue
+        v, i = scene.meshes[ins.mesh]
+        tri = i.reshape(-1, 3)[prim[sel]]
+        n0, n1, n2 = (v["normal"][tri[:, j]].astype(np.float64) for j in range(3))
+        ln = n0 + bary[sel, :1] * (n1 - n0) + bary[sel, 1:] * (n2 - n0)
+        M = np.asarray(ins.transform, np.float64).reshape(4, 4).T
+        N[sel] = ln @ M[:3, :3].T
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    rng = np.random.default_rng(seed)
+    r0, r1 = rng.random(len(hit)), rng.random(len(hit))
+    a = np.where(np.abs(N[:, :1]) > np.abs(N[:, 2:]), np.stack([-N[:, 1], N[:, 0], 0 * N[:, 0]], 1),
+                 np.stack([0 * N[:, 0], -N[:, 2], N[:, 1]], 1))
+    t = a / np.linalg.norm(a, axis=1, keepdims=True)
+    b = np.cross(N, t)
+    st, ph = np.sqrt(r0), 2.0 * np.pi * r1
+    L_ = (st * np.cos(ph))[:, None] * t + np.sqrt(1.0 - r0)[:, None] * N + (st * np.sin(ph))[:, None] * b
+    L_ /= np.linalg.norm(L_, axis=1, keepdims=True)
+    out = np.zeros(len(hit), dtype=L.RAY)
+    out["origin"] = (P + L_ * 1e-8).astype(np.float32)
+    out["direction"] = L_.astype(np.float32)
+    out["tmin"] = np.float32(1e-8)
+    out["tmax"] = np.finfo(np.float32).max
+    return out
+
+
 class Rng:
     def __init__(self, seed: int, seq: int = 0x5eed0010):
         self._s = L.Pcg32()
