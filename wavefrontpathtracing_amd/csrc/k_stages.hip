@@ -68,10 +68,43 @@ __global__ void k_generate(FrameArgs A)
 // in the next queue and costs the next extend launch its cache locality.
 constexpr uint32_t kShadeChunk = 1024;
 
+// Appends to two queues for a whole 256-thread workgroup with one atomic per
+// queue (a wave64 append would issue four): each wave's ballot counts go to
+// LDS, thread 0 reserves both ranges, and each lane's slot is the range base
+// plus the counts of the waves before it plus its rank in its wave.  Every
+// thread of the workgroup must call it; `par` alternates between calls so
+// that a wave one call ahead writes the other half of the LDS slots.
+__device__ __forceinline__ void block_append2(uint32_t* ca, bool fa, uint32_t* cb, bool fb, uint32_t (*s_cnt)[2][4],
+                                              uint32_t (*s_rng)[2], uint32_t par, uint32_t& slot_a, uint32_t& slot_b)
+{
+    const uint64_t ma = __ballot(fa), mb = __ballot(fb);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_cnt[par][0][wave] = (uint32_t)__popcll(ma);
+        s_cnt[par][1][wave] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t na = s_cnt[par][0][0] + s_cnt[par][0][1] + s_cnt[par][0][2] + s_cnt[par][0][3];
+        const uint32_t nb = s_cnt[par][1][0] + s_cnt[par][1][1] + s_cnt[par][1][2] + s_cnt[par][1][3];
+        s_rng[par][0] = na ? atomicAdd(ca, na) : 0u;
+        s_rng[par][1] = nb ? atomicAdd(cb, nb) : 0u;
+    }
+    __syncthreads();
+    uint32_t pa = s_rng[par][0], pb = s_rng[par][1];
+    for (uint32_t w = 0; w < wave; ++w) { pa += s_cnt[par][0][w]; pb += s_cnt[par][1][w]; }
+    const uint64_t below = (1ull << lane) - 1ull;
+    slot_a = pa + (uint32_t)__popcll(ma & below);
+    slot_b = pb + (uint32_t)__popcll(mb & below);
+}
+
 template <bool ADJ>
 __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
 {
     __shared__ uint32_t s_base;
+    __shared__ uint32_t s_cnt[2][2][4];
+    __shared__ uint32_t s_rng[2][2];
+    uint32_t par = 0;
     const uint32_t n = A.counts[d];
     const uint32_t cur = d & 1u, nxt = cur ^ 1u;
     const uint32_t seed = A.dyn->frame_seed;
@@ -119,13 +152,14 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
                 else A.energy[p] = e4;
             }
             if (ADJ) wave_add_keyed(A.grad_em, em_key, em_g, em_has);
-            const uint32_t slot = wave_append(&A.counts[d + 1], spawn);
+            uint32_t slot, sslot;
+            block_append2(&A.counts[d + 1], spawn, &A.counts[kShadowCountBase + d], shadow, s_cnt, s_rng, par, slot, sslot);
+            par ^= 1u;
             if (spawn) {
                 store_ray(A.rays[nxt], slot, new_o, new_d, kRayMinT, FLT_MAX);
                 A.path_of[nxt][slot] = p;
                 A.root_of[nxt][slot] = root;
             }
-            const uint32_t sslot = wave_append(&A.counts[kShadowCountBase + d], shadow);
             if (shadow) {
                 store_ray(A.shadow_rays, sslot, sh_o, sh_d, kRayMinT, sh_tmax);
                 A.shadow_path[sslot] = p;
