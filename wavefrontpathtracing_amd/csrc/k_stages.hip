@@ -66,16 +66,19 @@ __global__ void k_generate(FrameArgs A)
 // pixels together.  A grid-stride loop lets workgroups drift apart over a
 // long queue, which interleaves far apart entries (other scenes of a batch)
 // in the next queue and costs the next extend launch its cache locality.
-constexpr uint32_t kShadeChunk = 1024;
+constexpr uint32_t kShadeBlock = 1024;                 // 16 waves: one append atomic per 1,024 entries
+constexpr uint32_t kShadeWaves = kShadeBlock / 64u;
+constexpr uint32_t kShadeChunk = 2048;
 
-// Appends to two queues for a whole 256-thread workgroup with one atomic per
-// queue (a wave64 append would issue four): each wave's ballot counts go to
+// Appends to two queues for a whole workgroup with one atomic per queue (a
+// wave64 append would issue one per wave): each wave's ballot counts go to
 // LDS, thread 0 reserves both ranges, and each lane's slot is the range base
 // plus the counts of the waves before it plus its rank in its wave.  Every
 // thread of the workgroup must call it; `par` alternates between calls so
 // that a wave one call ahead writes the other half of the LDS slots.
-__device__ __forceinline__ void block_append2(uint32_t* ca, bool fa, uint32_t* cb, bool fb, uint32_t (*s_cnt)[2][4],
-                                              uint32_t (*s_rng)[2], uint32_t par, uint32_t& slot_a, uint32_t& slot_b)
+__device__ __forceinline__ void block_append2(uint32_t* ca, bool fa, uint32_t* cb, bool fb,
+                                              uint32_t (*s_cnt)[2][kShadeWaves], uint32_t (*s_rng)[2], uint32_t par,
+                                              uint32_t& slot_a, uint32_t& slot_b)
 {
     const uint64_t ma = __ballot(fa), mb = __ballot(fb);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -85,8 +88,8 @@ __device__ __forceinline__ void block_append2(uint32_t* ca, bool fa, uint32_t* c
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t na = s_cnt[par][0][0] + s_cnt[par][0][1] + s_cnt[par][0][2] + s_cnt[par][0][3];
-        const uint32_t nb = s_cnt[par][1][0] + s_cnt[par][1][1] + s_cnt[par][1][2] + s_cnt[par][1][3];
+        uint32_t na = 0, nb = 0;
+        for (uint32_t w = 0; w < kShadeWaves; ++w) { na += s_cnt[par][0][w]; nb += s_cnt[par][1][w]; }
         s_rng[par][0] = na ? atomicAdd(ca, na) : 0u;
         s_rng[par][1] = nb ? atomicAdd(cb, nb) : 0u;
     }
@@ -99,10 +102,10 @@ __device__ __forceinline__ void block_append2(uint32_t* ca, bool fa, uint32_t* c
 }
 
 template <bool ADJ>
-__global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
+__global__ __launch_bounds__(kShadeBlock) void k_shade(FrameArgs A, uint32_t d)
 {
     __shared__ uint32_t s_base;
-    __shared__ uint32_t s_cnt[2][2][4];
+    __shared__ uint32_t s_cnt[2][2][kShadeWaves];
     __shared__ uint32_t s_rng[2][2];
     uint32_t par = 0;
     const uint32_t n = A.counts[d];
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs A, uint32_t d)
         __syncthreads();
         if (base >= n) break;
         const uint32_t stop = n - base < kShadeChunk ? n : base + kShadeChunk;
-        for (uint32_t j = base; j < stop; j += 256u) {
+        for (uint32_t j = base; j < stop; j += kShadeBlock) {
             const uint32_t i = j + threadIdx.x;
             const bool active = i < stop;
             bool spawn = false, shadow = false;
@@ -286,7 +289,7 @@ hipError_t launch_adjoint_clear(const FrameArgs& A, uint32_t grid, hipStream_t s
 }
 hipError_t launch_adjoint_shade(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s)
 {
-    k_shade<true><<<grid, 256, 0, s>>>(A, d);
+    k_shade<true><<<grid * 256u / kShadeBlock, kShadeBlock, 0, s>>>(A, d);
     return hipGetLastError();
 }
 hipError_t launch_adjoint_connect(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s)
@@ -307,7 +310,7 @@ hipError_t launch_generate(const FrameArgs& A, uint32_t grid, hipStream_t s)
 }
 hipError_t launch_shade(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s)
 {
-    k_shade<false><<<grid, 256, 0, s>>>(A, d);
+    k_shade<false><<<grid * 256u / kShadeBlock, kShadeBlock, 0, s>>>(A, d);
     return hipGetLastError();
 }
 hipError_t launch_connect_accumulate(const FrameArgs& A, uint32_t d, uint32_t grid, hipStream_t s)
