@@ -44,12 +44,21 @@ static wf_status adjoint_run(wf_renderer* r, const float* d_dl, int32_t tex_slot
     if ((e = launch_adjoint_clear(A, grid, r->stream)) != hipSuccess) return r->hip_fail(e, "adjoint clear launch");
     if ((e = launch_generate(A, grid, r->stream)) != hipSuccess) return r->hip_fail(e, "generate launch");
     const bool batched = r->slots.size() > 1;
+    // as in the forward frame, bounce d-1's shadow rays may ride in bounce d's
+    // extend launch: adjoint_connect(d-1) then runs before adjoint_shade(d),
+    // the order in which the separate launches update the remaining estimate
+    const bool fuse = A.nee && !r->settings.separate_connect && r->ext_cfg.skip_popped && r->ext_grid_tlds == 0;
     for (uint32_t d = 0; d <= A.max_bounces; ++d) {
+        const bool with_shadow = fuse && d > 0;
+        wf_renderer::OcclusionQueue q{A.shadow_rays, A.counts + kShadowCountBase + (d > 0 ? d - 1 : 0), A.shadow_visible,
+                         batched ? A.shadow_root : nullptr};
         s = r->run_extend(A.rays[d & 1u], A.counts + d, A.n_paths, A.hits, nullptr, nullptr, false,
-                          batched ? A.root_of[d & 1u] : nullptr);
+                          batched ? A.root_of[d & 1u] : nullptr, with_shadow ? &q : nullptr);
         if (s != WF_OK) return s;
+        if (with_shadow && (e = launch_adjoint_connect(A, d - 1, grid, r->stream)) != hipSuccess)
+            return r->hip_fail(e, "adjoint connect launch");
         if ((e = launch_adjoint_shade(A, d, grid, r->stream)) != hipSuccess) return r->hip_fail(e, "adjoint shade launch");
-        if (A.nee && d < A.max_bounces) {
+        if (A.nee && !fuse && d < A.max_bounces) {
             s = r->run_extend(A.shadow_rays, A.counts + kShadowCountBase + d, A.n_paths, nullptr, A.shadow_visible, nullptr,
                               true, batched ? A.shadow_root : nullptr);
             if (s != WF_OK) return s;
