@@ -102,6 +102,33 @@ def time_cpu(orc, rays, threads, seconds_target):
     return round(passes * len(rays) / dt / 1e6, 4), passes, round(dt, 1)
 
 
+def bench_bounce1(r, sc, stream, steps: int, warmup: int):
+    """The 256-instance scene's other ray set: first-bounce rays (cosine-
+    distributed about the hit normals) of the 1920x1080 camera rays, closest
+    hit on the renderer `r` already holding the scene."""
+    import torch
+    from wavefrontpathtracing_amd import scenes
+    from wavefrontpathtracing_amd import _lib as L
+    cam = scenes.primary_rays(sc)
+    b1 = scenes.bounce_rays(sc, cam, r.trace_closest(cam))
+    d_b1 = torch.from_numpy(b1.view(np.uint8)).to("cuda:0")
+    d_b1h = torch.empty(len(b1) * 20, dtype=torch.uint8, device="cuda:0")
+    for _ in range(warmup):
+        r.trace_closest_device(d_b1.data_ptr(), len(b1), d_b1h.data_ptr())
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        r.trace_closest_device(d_b1.data_ptr(), len(b1), d_b1h.data_ptr())
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    hits = d_b1h.cpu().numpy().view(L.HIT)
+    return {"rays": len(b1), "mrays": round(len(b1) / (ms * 1e-3) / 1e6, 2), "ms_per_step": round(ms, 4),
+            "hit_fraction": round(float((hits["instance_idx"] != L.MISS).mean()), 4),
+            "source": "cosine-distributed directions about the hit normals of the 1920x1080 camera rays"}
+
+
 def bench_soup(n_tris: int, steps: int, cpu: bool):
     """Random-triangle soup (centroids in [-1,1]^3, offsets +-U(0.005,0.02)),
     16-bin SAH, camera (0,0,-3) -> origin, 60 deg, 512x512: closest hit of
@@ -472,6 +499,8 @@ def main():
     ap.add_argument("--no-build", action="store_true")
     ap.add_argument("--no-union", action="store_true",
                     help="skip the opt-in union-area TLAS measurements (headline rays and frame)")
+    ap.add_argument("--no-bounce1", action="store_true",
+                    help="skip the first-bounce ray set (profiled runs: the headline kernel then runs only on the headline rays)")
     ap.add_argument("--build-only", action="store_true",
                     help="only the BLAS/TLAS build section (for profiling the builders); prints its JSON")
     ap.add_argument("--tune", action="store_true", help="autotune the traversal kernel on the batched frame")
@@ -591,23 +620,9 @@ def main():
         r.build_tlas()
 
     # the scene's other ray set: first-bounce rays from the hits of the 1920x1080 camera rays
-    cam = scenes.primary_rays(sc)
-    b1 = scenes.bounce_rays(sc, cam, r.trace_closest(cam))
-    d_b1 = torch.from_numpy(b1.view(np.uint8)).to(dev)
-    d_b1h = torch.empty(len(b1) * 20, dtype=torch.uint8, device=dev)
-    for _ in range(args.warmup):
-        r.trace_closest_device(d_b1.data_ptr(), len(b1), d_b1h.data_ptr())
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        r.trace_closest_device(d_b1.data_ptr(), len(b1), d_b1h.data_ptr())
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    b1_ms = ev0.elapsed_time(ev1) / args.steps
-    b1_hits = d_b1h.cpu().numpy().view(L.HIT)
-    bounce1 = {"rays": len(b1), "mrays": round(len(b1) / (b1_ms * 1e-3) / 1e6, 2), "ms_per_step": round(b1_ms, 4),
-               "hit_fraction": round(float((b1_hits["instance_idx"] != L.MISS).mean()), 4),
-               "source": "cosine-distributed directions about the hit normals of the 1920x1080 camera rays"}
+    bounce1 = None
+    if not args.no_bounce1:
+        bounce1 = bench_bounce1(r, sc, stream, args.steps, args.warmup)
 
     out = {
         "metric": METRIC,
