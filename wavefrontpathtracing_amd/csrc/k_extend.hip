@@ -44,6 +44,9 @@ namespace {
 
 using namespace trav;
 
+constexpr uint32_t kClaim = 256;              // most queue entries a wave claims with one atomic (BIG)
+constexpr uint64_t kBigLaunch = 16ull << 20;  // queue capacity from which a mixed launch uses BIG claims
+
 // Persistent traversal with per-lane refill.  Every lane owns one ray at a
 // time and advances it one traversal pass per loop iteration (trace_pass).
 // A lane whose ray finishes writes its result and goes idle; once at least
@@ -51,7 +54,7 @@ using namespace trav;
 // from the global cursor with one atomic (ballot + mbcnt give each idle lane
 // its index).  The loop ends when every lane is idle and the queue is
 // drained.
-template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4, bool MIX, class Stack>
+template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4, bool MIX, bool BIG, class Stack>
 __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, const float4* s_tl, uint32_t lane, uint32_t n)
 {
     const TraceScene sc = L.scene;
@@ -66,21 +69,56 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
     const uint64_t ntot = (uint64_t)n + n2;
     bool lane_any = ANY;
 
+    // BIG (launches of tens of millions of entries): the wave claims up to
+    // kClaim entries per atomic and keeps the unstarted ones in [lb, le); the
+    // claim shrinks with the entries left after the wave's previous claim,
+    // (left) / (16 x waves in the grid), so ranges still held when the queue
+    // runs out stay small and the tail is not lengthened.  Otherwise each
+    // refill claims exactly the idle lanes' entries.
+    uint64_t lb = 0, le = 0, hint = 0;
+    bool last_claim = false;
+    const uint64_t share = 16ull * gridDim.x * (BLOCK / 64);
     for (;;) {
         const uint64_t idle = __ballot(ray_i == ~0u);
         const uint32_t n_idle = (uint32_t)__popcll(idle);
         if (drained && n_idle == 64u) break;
         int status = kRun;
         if (!drained && n_idle > 0u && (n_idle >= refill_min || n_idle == 64u)) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&L.work[0], n_idle);
-            base = __shfl(base, (int)leader, 64);
-            if ((uint64_t)base + n_idle >= ntot) drained = true;
-            if (ray_i != ~0u) continue;
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-            const uint32_t q = base + rank;
-            if ((uint64_t)q >= ntot) continue;
+            uint32_t q;
+            if constexpr (BIG) {
+                if (lb >= le) {
+                    uint32_t g = n_idle;
+                    const uint64_t big = (ntot > hint ? ntot - hint : 0ull) / share;
+                    if (big > g) g = big < kClaim ? (uint32_t)big : kClaim;
+                    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(&L.work[0], g);
+                    base = __shfl(base, (int)leader, 64);
+                    lb = base;
+                    le = (uint64_t)base + g < ntot ? (uint64_t)base + g : (ntot > base ? ntot : (uint64_t)base);
+                    last_claim = (uint64_t)base + g >= ntot;
+                    hint = (uint64_t)base + g;
+                }
+                const uint32_t take = (uint32_t)(le - lb < (uint64_t)n_idle ? le - lb : (uint64_t)n_idle);
+                const uint64_t first = lb;
+                lb += take;
+                if (last_claim && lb >= le) drained = true;
+                if (ray_i != ~0u) continue;
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                if (rank >= take) continue;
+                if (first + rank >= ntot) continue;
+                q = (uint32_t)(first + rank);
+            } else {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&L.work[0], n_idle);
+                base = __shfl(base, (int)leader, 64);
+                if ((uint64_t)base + n_idle >= ntot) drained = true;
+                if (ray_i != ~0u) continue;
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                q = base + rank;
+                if ((uint64_t)q >= ntot) continue;
+            }
             if constexpr (MIX) {
                 const bool second = q >= n;
                 const uint32_t i = second ? q - n : q;
@@ -123,7 +161,7 @@ extern __shared__ float4 s_tlas_dyn[];
 // TLDS: the workgroup first copies the TLAS records (at most
 // kTlasLdsPairs, in dynamic LDS sized by the launch) and reads TLAS nodes
 // from there.
-template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4 = false, bool MIX = false>
+template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4 = false, bool MIX = false, bool BIG = false>
 __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
 {
     constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
@@ -141,11 +179,11 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
     if constexpr (LDS > 0) {
         LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + gtid, L.spill_per_thread, gridDim.x * BLOCK,
                                 L.stack_cap};
-        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4, MIX>(L, st, s_tl, tid & 63u, n);
+        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4, MIX, BIG>(L, st, s_tl, tid & 63u, n);
     } else {
         PrivStack<128> st;
         st.cap = L.stack_cap;
-        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4, MIX>(L, st, s_tl, tid & 63u, n);
+        extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4, MIX, BIG>(L, st, s_tl, tid & 63u, n);
     }
 }
 
@@ -193,12 +231,13 @@ void* pick_quad(uint32_t block, uint32_t lds)
 }
 
 // closest-hit launches with a trailing occlusion queue: popped-interior skip
-// on, no counters, TLAS in global memory, either layout
-template <bool L4>
+// on, no counters, TLAS in global memory, either layout; BIG claims for the
+// 2-wide layout only
+template <bool L4, bool BIG>
 void* pick_mix(uint32_t block, uint32_t lds)
 {
 #define WF_CASE(B, S) \
-    if (block == B && lds == S) return reinterpret_cast<void*>(&k_extend<B, S, true, false, false, false, L4, true>);
+    if (block == B && lds == S) return reinterpret_cast<void*>(&k_extend<B, S, true, false, false, false, L4, true, BIG>);
     WF_CASE(64, 0) WF_CASE(64, 4) WF_CASE(64, 8) WF_CASE(64, 12) WF_CASE(64, 16)
     WF_CASE(128, 0) WF_CASE(128, 4) WF_CASE(128, 8) WF_CASE(128, 12) WF_CASE(128, 16)
     WF_CASE(256, 0) WF_CASE(256, 4) WF_CASE(256, 8) WF_CASE(256, 12) WF_CASE(256, 16)
@@ -206,11 +245,12 @@ void* pick_mix(uint32_t block, uint32_t lds)
     return nullptr;
 }
 
-void* pick_kernel(const ExtendConfig& c, bool count, bool any, bool tlds, bool mix = false)
+void* pick_kernel(const ExtendConfig& c, bool count, bool any, bool tlds, bool mix = false, bool big = false)
 {
     if (mix) {
         if (count || any || tlds || !c.skip_popped) return nullptr;
-        return c.layout == 1 ? pick_mix<true>(c.block, c.lds_stack) : pick_mix<false>(c.block, c.lds_stack);
+        if (c.layout == 1) return pick_mix<true, false>(c.block, c.lds_stack);
+        return big ? pick_mix<false, true>(c.block, c.lds_stack) : pick_mix<false, false>(c.block, c.lds_stack);
     }
     if (c.layout == 1) {
         if (count || tlds || !c.skip_popped) return nullptr;
@@ -233,7 +273,10 @@ hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStre
     const bool tlds = L.grid_tlds != 0;   // the host sets it when the TLAS records fit
     ExtendConfig c = cfg;
     if (L.counters) c.layout = 0;         // traversal counters come from the 2-wide kernels
-    void* k = pick_kernel(c, L.counters != nullptr, L.any_hit != 0, tlds, L.rays2 != nullptr);
+    // the host knows the queues' capacities, not their lengths: capacities of
+    // tens of millions mark the launches of a large batch
+    const bool big = (uint64_t)L.n + L.n2 >= kBigLaunch;
+    void* k = pick_kernel(c, L.counters != nullptr, L.any_hit != 0, tlds, L.rays2 != nullptr, big);
     if (!k) return hipErrorInvalidValue;
     ExtendLaunch arg = L;
     void* args[] = {&arg};
