@@ -37,39 +37,68 @@ def bytes_per_ray(counters: np.ndarray) -> float:
 PROFILE = os.path.join(ROOT, "profiles", "extend_tlas256_incoherent.md")
 
 
-def measured_traffic(cfg: dict):
-    """HBM bytes per extend launch measured with rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE (scaled by a plain copy's counts) for this workload, as
-    recorded in the committed profile summary; None when the summary was
-    taken with another traversal configuration."""
+PROFILE_KEYS = ("traffic_bytes_per_launch", "valu_busy", "lane_utilisation", "l2_hit", "waves_per_simd",
+                "salu_per_valu")
+
+
+def measured_profile(cfg: dict):
+    """Counters of the extend launch on this workload from the committed
+    rocprofv3 summary: HBM bytes per launch (FETCH_SIZE and WRITE_SIZE
+    scaled by a plain copy's counts), VALU busy, lane utilisation, L2 hit
+    rate, resident waves per SIMD and SALU per VALU instruction; None when
+    the summary was taken with another traversal configuration."""
     try:
         text = open(PROFILE).read()
     except OSError:
-        return None, None
+        return None
     want = " ".join(f"{k}={v}" for k, v in cfg.items())
-    got, traffic = None, None
+    got, vals = None, {}
     for line in text.splitlines():
         if line.startswith("traversal_config:"):
             got = line.split(":", 1)[1].strip()
-        if line.startswith("traffic_bytes_per_launch:"):
-            traffic = float(line.split(":", 1)[1])
-    if got != want or traffic is None:
-        return None, None
-    return traffic, os.path.relpath(PROFILE, ROOT)
+        key = line.split(":", 1)[0]
+        if key in PROFILE_KEYS:
+            vals[key] = float(line.split(":", 1)[1])
+    if got != want or "traffic_bytes_per_launch" not in vals:
+        return None
+    vals["source"] = os.path.relpath(PROFILE, ROOT)
+    return vals
 
 
-def cpu_baseline(scene, rays, seconds_target=12.0):
-    """The oracle's closest-hit traversal on the host cores: whole passes
-    over the same incoherent rays until about `seconds_target` of work."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_api as O
-    orc = O.OracleScene(scene)
-    threads = min(os.cpu_count() or 1, 16)
+def host_threads() -> int:
+    """Every CPU thread this process may use: the affinity mask, capped by
+    OMP_NUM_THREADS when the environment grants fewer (a shared host's CPU
+    share per GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n)
+
+
+def host_description() -> dict:
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    import subprocess
+    try:
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True, timeout=10).stdout.splitlines()[0]
+    except (OSError, IndexError, subprocess.SubprocessError):
+        cc = "unknown"
+    return {"cpu_model": model, "host_cpus": os.cpu_count(), "compiler": cc,
+            "flags": "-O3 -std=c11 -ffp-contract=off -fno-fast-math"}
+
+
+def _sample_rate(orc, rays, threads, seconds_target):
+    """Oracle closest-hit passes over a prefix of `rays` sized to about
+    `seconds_target` of work: (Mrays/s, rays in the sample, passes, seconds)."""
     n = 4096
     while True:
-        sample = rays[:n]
         t0 = time.perf_counter()
-        orc.trace_closest(sample, nthreads=threads)
+        orc.trace_closest(rays[:n], nthreads=threads)
         dt = time.perf_counter() - t0
         if dt > 1.0 or n >= len(rays):
             break
@@ -84,9 +113,24 @@ def cpu_baseline(scene, rays, seconds_target=12.0):
     for _ in range(passes):
         orc.trace_closest(sample, nthreads=threads)
     dt = time.perf_counter() - t0
-    return {"value": round(passes * n_final / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+    return passes * n_final / dt / 1e6, n_final, passes, dt
+
+
+def cpu_baseline(scene, rays, seconds_target=12.0):
+    """The oracle's closest-hit traversal on the host: on every CPU thread
+    the process is given (the value) and on one thread, whole passes over
+    the same incoherent rays until about `seconds_target` of work each."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as O
+    orc = O.OracleScene(scene)
+    threads = host_threads()
+    v, n_final, passes, dt = _sample_rate(orc, rays, threads, seconds_target)
+    v1, n1, p1, d1 = _sample_rate(orc, rays, 1, seconds_target / 3)
+    return {"value": round(v, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "one_thread_mrays": round(v1, 4), **host_description(),
             "sample": f"{passes} pass(es) over the first {n_final} of the {len(rays)} incoherent rays, oracle traversal "
-                      f"(C, -O3, no FP contraction), {threads} threads, {dt:.1f} s"}
+                      f"(C, -O3, no FP contraction), {threads} threads, {dt:.1f} s; one thread: {p1} pass(es) over "
+                      f"the first {n1}, {d1:.1f} s"}
 
 
 def time_cpu(orc, rays, threads, seconds_target):
@@ -164,7 +208,7 @@ def bench_soup(n_tris: int, steps: int, cpu: bool):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_api as O
         orc = O.OracleScene(sc, bins=16)
-        threads = min(os.cpu_count() or 1, 16)
+        threads = host_threads()
         prim = sets["primary"]
         v1, p1, d1 = time_cpu(orc, prim[: len(prim) // 8], 1, 4.0)
         vn, pn, dn = time_cpu(orc, prim, threads, 4.0)
@@ -210,7 +254,8 @@ def bench_build(cpu: bool):
         assert d_nodes[:count].cpu().numpy().tobytes() == nodes_h.tobytes(), "device BLAS nodes differ from the host build"
         assert d_idx.cpu().numpy().tobytes() == idx_h.tobytes(), "device BLAS indices differ from the host build"
         e = {"tris": n, "intervals": bins, "gpu_ms": round(gpu_s * 1e3, 3), "gpu_mtris_per_s": round(n / gpu_s / 1e6, 1),
-             "host_builder_ms": round(host_s * 1e3, 1), "host_threads": min(os.cpu_count() or 1, 16),
+             "host_builder_ms": round(host_s * 1e3, 1),
+             "host_builder": "the library's host builder, large subtrees of the recursion on std::async threads",
              "nodes": count, "level_passes": levels, "wave_subtrees": subtrees, "max_leaf": max_leaf,
              "equal_to_host_build": True}
         if cpu and n <= (1 << 20):
@@ -350,7 +395,26 @@ def bench_inverse(steps: int):
             "steps_timed": steps, "wall_s_with_setup": round(total, 1)}
 
 
-def bench_batched(frames: int, tune: bool):
+def batch_candidates(headline):
+    """The configurations the batch tuner times: the default, the four
+    fastest on the headline rays, and the fastest there with a scratch
+    (private) stack and with the 4-wide layout."""
+    from wavefrontpathtracing_amd import tuning
+    picks = [dict(tuning.DEFAULT)]
+    ranked = [c for c, _ in headline] if headline else list(tuning.candidates())
+    picks += ranked[:4]
+    for pred in (lambda c: c["lds_stack"] == 0, lambda c: c["bvh_layout"] == 1):
+        picks += [c for c in ranked if pred(c)][:1]
+    out, seen = [], set()
+    for c in picks:
+        key = tuning.label(c)
+        if key not in seen:
+            seen.add(key)
+            out.append(c)
+    return out
+
+
+def bench_batched(frames: int, tune: bool, headline=None):
     """64 uneven scenes (soup of 10K..2M triangles in a lit box room, 256^2 /
     512^2 / 1024^2 pixels), 4 spp, depth 4, next-event estimation: the whole
     batch as one frame of one launch per stage (captured hipGraph) against
@@ -373,21 +437,35 @@ def bench_batched(frames: int, tune: bool):
         st = r.frame_stats()
         return dt, st.rays_extended + st.shadow_rays
 
+    def plain_pass():
+        t = time.perf_counter()
+        for k in range(n):
+            r.render_scenes(k, 1)
+        return time.perf_counter() - t
+
     tuned = None
     if tune:
-        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, interior_visits=v, bvh_layout=lay)
-                for b in (64, 128, 256) for l in (8, 16) for v in (2, 4) for lay in (0, 1)]
-        tuned, table = tuning.autotune(r, lambda: batched_frame()[0], configs=cfgs, repeats=1)
-        tuned = dict(tuned, sweep={f"{c['block']}/{c['lds_stack']}/{c['interior_visits']}/{c['bvh_layout']}": round(t * 1e3, 1)
-                                   for c, t in table})
+        t_tune = time.perf_counter()
+        best, table = tuning.autotune(r, lambda: batched_frame()[0], configs=batch_candidates(headline), repeats=1)
+        tuned = {"winner": best, "seconds": round(time.perf_counter() - t_tune, 1),
+                 "table_ms_per_frame": {tuning.label(c): round(t * 1e3, 1) for c, t in table},
+                 "labels": "block/lds_stack/skip_popped/bvh_layout/interior_visits"}
     batched_frame()
     bt, brays = 0.0, 0
     for _ in range(frames):
         dt, nr = batched_frame()
         bt += dt
         brays += nr
-    # the same images, one scene at a time (no graph)
+    batch_imgs = [r.read_scene_energy(k, sc.width, sc.height) for k, sc in enumerate(sl)]
+    # the same images, one scene at a time (no graph), with the traversal
+    # configuration tuned for that loop
     r.update_settings(use_hip_graph=0)
+    tuned_plain = None
+    if tune:
+        t_tune = time.perf_counter()
+        best_p, table_p = tuning.autotune(r, plain_pass, configs=batch_candidates(headline), repeats=1)
+        tuned_plain = {"winner": best_p, "seconds": round(time.perf_counter() - t_tune, 1),
+                       "table_ms_per_pass": {tuning.label(c): round(t * 1e3, 1) for c, t in table_p}}
     for k in range(n):
         r.render_scenes(k, 1)
     pt, prays = 0.0, 0
@@ -398,6 +476,10 @@ def bench_batched(frames: int, tune: bool):
             pt += time.perf_counter() - t
             st = r.frame_stats()
             prays += st.rays_extended + st.shadow_rays
+    differ = [k for k, sc in enumerate(sl) if not np.array_equal(r.read_scene_energy(k, sc.width, sc.height), batch_imgs[k])]
+    del batch_imgs
+    if differ:
+        raise RuntimeError(f"batched and per-scene images differ for scenes {differ[:8]}")
     # the reference's frame structure: one scene, one sample per pixel per
     # frame, frames accumulated until the scene has 4 samples, one launch per
     # stage (shadow rays in their own launch)
@@ -429,7 +511,8 @@ def bench_batched(frames: int, tune: bool):
             "reference_loop_ms_per_batch": round(rt / frames * 1e3, 2),
             "rays_per_frame": brays // frames, "batched_ms_per_frame": round(bt / frames * 1e3, 2),
             "plain_ms_per_frame": round(pt / frames * 1e3, 2), "setup_s": round(setup_s, 1), "frames": frames,
-            "autotuned": tuned}
+            "images_identical": True,
+            "autotuned": tuned, "autotuned_plain_loop": tuned_plain}
 
 
 def bench_adjoint(frames: int):
@@ -503,9 +586,10 @@ def main():
                     help="skip the first-bounce ray set (profiled runs: the headline kernel then runs only on the headline rays)")
     ap.add_argument("--build-only", action="store_true",
                     help="only the BLAS/TLAS build section (for profiling the builders); prints its JSON")
-    ap.add_argument("--tune", action="store_true", help="autotune the traversal kernel on the batched frame")
-    ap.add_argument("--sweep", action="store_true",
-                    help="autotune the traversal kernel on the incoherent workload first (table on stderr)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner: run the default traversal configuration (or --traversal)")
+    ap.add_argument("--traversal", default="",
+                    help="fixed traversal configuration, e.g. block=64,lds_stack=16,interior_visits=4 (implies --no-tune)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -548,8 +632,13 @@ def main():
     def step():
         r.trace_closest_device(d_rays.data_ptr(), len(rays), d_hits.data_ptr())
 
-    if args.sweep:
-        # autotuner table on this workload (stderr), then keep the fastest
+    autotuned = None
+    if args.traversal:
+        tuning.set_extend_config(r, **{k: int(v) for k, v in (kv.split("=") for kv in args.traversal.split(","))})
+    elif not args.no_tune:
+        # the autotuner over every workgroup size x stack placement x
+        # popped-interior skip x node layout, at 2 and 4 interior visits per
+        # pass, on this workload; the winner is the configuration timed below
         ev_a, ev_b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
         def timed():
@@ -559,12 +648,11 @@ def main():
             ev_b.synchronize()
             return ev_a.elapsed_time(ev_b) * 1e-3
 
-        cfgs = [dict(block=b, lds_stack=l, skip_popped=1, refill_min=16, tlas_in_lds=0, interior_visits=v, bvh_layout=lay)
-                for b in (64, 128, 256) for l in (8, 12, 16) for v in (2, 4) for lay in (0, 1)]
+        t_tune = time.perf_counter()
+        cfgs = [dict(c, interior_visits=v) for c in tuning.candidates() for v in (2, 4)]
         best, table = tuning.autotune(r, timed, configs=cfgs, repeats=3)
-        for c, t in table:
-            print(f"sweep block={c['block']} lds={c['lds_stack']} layout={c['bvh_layout']} "
-                  f"visits={c['interior_visits']}: {len(rays) / t / 1e6:.1f} Mrays/s", file=sys.stderr)
+        autotuned = {"workload": "the headline rays", "candidates": len(table), "seconds": round(time.perf_counter() - t_tune, 1),
+                     "winner": best, "table_mrays": {tuning.label(c): round(len(rays) / t / 1e6, 1) for c, t in table}}
 
     for _ in range(args.warmup):
         step()
@@ -594,7 +682,8 @@ def main():
     b_ray = bytes_per_ray(counters)
     achieved_gbs = mrays * 1e6 * b_ray / 1e9
     cfg = tuning.get_extend_config(r)
-    traffic, traffic_src = measured_traffic(cfg)
+    prof = measured_profile(cfg)
+    traffic = None if prof is None else prof["traffic_bytes_per_launch"]
 
     # the same rays with the union-metric TLAS (opt-in): same hits, fewer node visits
     union = None
@@ -649,8 +738,11 @@ def main():
             "wall_s_timed": round(t_wall, 4),
             "union_tlas": union,
             "bounce1": bounce1,
+            "autotuned": autotuned,
         },
         "roofline": {
+            # priced against HBM, the only published peak that applies (no
+            # MFMA work); the counters say what limits the kernel instead
             "bound": "hbm",
             "achieved": round(achieved_gbs, 1),
             "peak": HBM_PEAK_GBS,
@@ -658,10 +750,16 @@ def main():
             "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
             "traffic": None if traffic is None else round(traffic),
             "traffic_unit": "B per launch",
-            "traffic_source": traffic_src,
             "bytes_per_ray": round(b_ray, 1),
+            "achieved_is": "algorithmic bytes (reference-algorithm traversal counters x record sizes) / kernel time",
             "mean_counters": {k: round(float(v), 3) for k, v in
                               zip(("tlas_nodes", "blas_nodes", "instances", "tri_tests"), counters.mean(axis=0))},
+            "measured": None if prof is None else {
+                "hbm_gbs": round(traffic / (ms_step * 1e-3) / 1e9, 1),
+                "hbm_frac": round(traffic / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                **{k: prof[k] for k in PROFILE_KEYS[1:] if k in prof}, "source": prof["source"]},
+            "limiter": "instruction issue and SIMT divergence of the traversal loop, not HBM: measured HBM traffic is a "
+                       "fraction of the algorithmic bytes (node fetches hit L2 / the Infinity Cache); see 'measured'",
         },
     }
     if not args.no_cpu:
@@ -675,7 +773,8 @@ def main():
     if not args.no_build:
         out["bvh_build"] = bench_build(cpu=not args.no_cpu)
     if not args.no_batch:
-        out["batched"] = bench_batched(frames=2, tune=args.tune)
+        out["batched"] = bench_batched(frames=2, tune=not args.no_tune and not args.traversal,
+                                       headline=None if autotuned is None else table)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(frames=3)
     if not args.no_inverse:

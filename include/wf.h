@@ -245,10 +245,15 @@ WF_API wf_status wf_renderer_set_tlas_metric(wf_renderer* r, uint32_t metric);
  * of LDS with the stack; 0 = read it through the caches);
  * interior_visits (1..8, 0 = default 4) is how many interior nodes a lane may
  * visit per pass of the traversal loop before the pass's instance-entry and
- * leaf steps. */
+ * leaf steps.
+ * big_claims selects how a wave of a closest-hit launch that also carries
+ * shadow rays claims queue entries: 0 = by the queues' capacity (at least
+ * 2^24 entries, as in a large batch: up to 256 entries per atomic; fewer:
+ * one entry per idle lane), 1 = always the large claims, 2 = never. */
 typedef struct wf_traversal_config {
     uint32_t workgroup_size; uint32_t lds_stack_entries; uint32_t skip_popped_interior; uint32_t bvh_layout;
     uint32_t stack_limit; uint32_t refill_min; uint32_t tlas_in_lds; uint32_t interior_visits;
+    uint32_t big_claims;
 } wf_traversal_config;
 WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, const wf_traversal_config* config);
 WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* r, wf_traversal_config* out);
@@ -284,8 +289,15 @@ WF_API wf_status wf_renderer_get_scene_bounds(const wf_renderer* r, float out_mi
 /* Closest hit for n rays (host arrays).  Returns WF_ERROR_STACK_OVERFLOW if
  * any ray overflowed its traversal stack (that ray reports a miss). */
 WF_API wf_status wf_trace_closest(wf_renderer* r, const wf_ray* rays, uint32_t n, wf_hit* hits);
-/* Same, on device arrays, ordered on the renderer's stream (no host sync). */
+/* Same, on device arrays, ordered on the renderer's stream (no host sync).
+ * A ray that overflows its traversal stack reports a miss, and the call
+ * cannot return WF_ERROR_STACK_OVERFLOW without a sync: the overflow count of
+ * this call is read by wf_renderer_read_stack_overflows. */
 WF_API wf_status wf_trace_closest_device(wf_renderer* r, const wf_ray* d_rays, uint32_t n, wf_hit* d_hits);
+/* Rays that overflowed the traversal stack in the last trace call or frame
+ * (synchronises the renderer's stream).  No reference counterpart: the
+ * reference's stacks are unchecked (accelstruct.hlsl:64,160). */
+WF_API wf_status wf_renderer_read_stack_overflows(wf_renderer* r, uint32_t* out);
 /* Occlusion: visible[i] = 1 if nothing is hit with t in [0, tmax). */
 WF_API wf_status wf_trace_occluded(wf_renderer* r, const wf_ray* rays, uint32_t n, uint32_t* visible);
 /* Per-ray traversal counters of the reference algorithm (4 x u32 per ray:

@@ -205,6 +205,13 @@ int orc_adjoint_f64(const orc_shading* sh, const orc_render_params* p, uint32_t 
                     const orc_vertex* rec, int32_t tex, const double* texels, const double* factor3,
                     const double* strength, const double* dl_dimage, double* d_texels, double* d_strength);
 
+/* ---- post-process (post_process.hlsl:25-72) ---------------------------------
+ * One frame over n pixels: energy_rgba (the frame's per-pixel mean) and the
+ * accumulator (in/out) -> RGBA8 final colour (R in the low byte); final_rgba,
+ * when not NULL, receives the final colour before the 8-bit store. */
+void orc_post_process(const float* energy_rgba, uint32_t n, uint32_t sample_count, uint32_t accumulate,
+                      uint32_t view_mode, float* accum_rgba, uint32_t* final_rgba8, float* final_rgba);
+
 /* RNG and trigonometry used by shade (exposed for known-answer tests). */
 uint32_t orc_pcg32_first(uint64_t initstate, uint64_t initseq);
 void     orc_path_rng(uint32_t path, uint32_t bounce, uint32_t frame_seed, float out[6]);

@@ -94,6 +94,7 @@ def olib():
             "orc_env_texel": (u32, [C.POINTER(OrcShading), vp]),
             "orc_adjoint_f64": (C.c_int, [C.POINTER(OrcShading), C.POINTER(OrcRenderParams), u32, u32, vp, C.c_int32,
                                           vp, vp, vp, vp, vp, vp]),
+            "orc_post_process": (None, [vp, u32, u32, u32, u32, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(o, name)
@@ -351,6 +352,19 @@ def env_texel(sh, d) -> int:
     """orc_env_texel for one direction (the equirect lookup of a miss)."""
     dd = np.ascontiguousarray(d, dtype=np.float32)
     return int(olib().orc_env_texel(C.byref(sh), p(dd)))
+
+
+def post_process(energy: np.ndarray, accum: np.ndarray, sample_count: int, accumulate: int, view_mode: int):
+    """post_process.hlsl:25-72 over one frame: returns (accumulator, RGBA8 final
+    colour, final colour before the 8-bit store) for an (..., 4) float32 energy
+    image and the accumulator before the frame."""
+    e = np.ascontiguousarray(energy, np.float32)
+    acc = np.ascontiguousarray(accum, np.float32).copy()
+    n = e.size // 4
+    out8 = np.zeros(e.shape[:-1], np.uint32)
+    outf = np.zeros_like(e)
+    olib().orc_post_process(p(e), n, sample_count, accumulate, view_mode, p(acc), p(out8), p(outf))
+    return acc, out8.view(np.uint8).reshape(e.shape), outf
 
 
 def hits_equal(a: np.ndarray, b: np.ndarray):

@@ -161,6 +161,39 @@ def test_stack_overflow_reports_status(small, lds):
     _assert_bitexact(r.trace_closest(sr), ref, "after the stack cap is lifted")
 
 
+def test_stack_overflow_on_the_device_path(small):
+    """wf_trace_closest_device cannot return the status without a sync: its
+    overflowing rays come back as misses, every other ray is exact, and
+    wf_renderer_read_stack_overflows reads the count, which equals the host
+    entry point's count for the same rays and restarts at each call."""
+    import torch
+    sc, r, orc = small
+    from wavefrontpathtracing_amd import tuning
+    bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
+    sr = scenes.sphere_rays(13, bmin, bmax, 5000)
+    ref = orc.trace_closest(sr, nthreads=8)
+    d_rays = torch.from_numpy(np.frombuffer(sr.tobytes(), np.uint8).copy()).cuda()
+    d_hits = torch.zeros(len(sr) * ref.dtype.itemsize, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    tuning.set_extend_config(r, block=128, lds_stack=8, skip_popped=1, stack_limit=2)
+    try:
+        with pytest.raises(WfError):
+            r.trace_closest(sr)
+        n_host = r.read_stack_overflows()
+        r.trace_closest_device(d_rays.data_ptr(), len(sr), d_hits.data_ptr())
+        n_dev = r.read_stack_overflows()
+        got = np.frombuffer(d_hits.cpu().numpy().tobytes(), ref.dtype)
+    finally:
+        tuning.set_extend_config(r)
+    assert n_dev == n_host > 0
+    bad = O.hits_equal(got, ref)
+    assert 0 < len(bad) <= n_dev
+    assert np.all(got["instance_idx"][bad] == L.MISS)
+    r.trace_closest_device(d_rays.data_ptr(), len(sr), d_hits.data_ptr())
+    assert r.read_stack_overflows() == 0
+    _assert_bitexact(np.frombuffer(d_hits.cpu().numpy().tobytes(), ref.dtype), ref, "device path after the cap is lifted")
+
+
 @pytest.mark.parametrize("block,lds,skip,tlds,refill,visits,layout", [
     (64, 8, 1, 1, 16, 1, 0), (128, 0, 1, 1, 1, 2, 0), (256, 4, 0, 1, 64, 3, 0), (128, 16, 0, 0, 16, 8, 0),
     (256, 8, 1, 0, 32, 4, 0), (64, 0, 1, 0, 48, 1, 0),

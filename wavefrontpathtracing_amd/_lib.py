@@ -92,7 +92,7 @@ class FrameStats(C.Structure):
 class TraversalConfig(C.Structure):
     _fields_ = [("workgroup_size", C.c_uint32), ("lds_stack_entries", C.c_uint32), ("skip_popped_interior", C.c_uint32),
                 ("bvh_layout", C.c_uint32), ("stack_limit", C.c_uint32), ("refill_min", C.c_uint32),
-                ("tlas_in_lds", C.c_uint32), ("interior_visits", C.c_uint32)]
+                ("tlas_in_lds", C.c_uint32), ("interior_visits", C.c_uint32), ("big_claims", C.c_uint32)]
 
 
 class Pcg32(C.Structure):
@@ -148,6 +148,7 @@ SIGNATURES = {
     "wf_renderer_get_scene_bounds": (_st, [_vp, _vp, _vp]),
     "wf_trace_closest": (_st, [_vp, _vp, _u32, _vp]),
     "wf_trace_closest_device": (_st, [_vp, _vp, _u32, _vp]),
+    "wf_renderer_read_stack_overflows": (_st, [_vp, C.POINTER(_u32)]),
     "wf_trace_occluded": (_st, [_vp, _vp, _u32, _vp]),
     "wf_trace_closest_counters": (_st, [_vp, _vp, _u32, _vp, _vp]),
     "wf_bvh_build": (_st, [_vp, _u32, C.POINTER(BuildOptions), _vp, _u64, C.POINTER(_u64), C.POINTER(_u32)]),

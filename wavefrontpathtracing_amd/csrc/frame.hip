@@ -227,6 +227,7 @@ wf_status wf_renderer::ensure_frame_buffers(size_t n_paths, size_t n_pixels)
     need(d_shadow_root, 4 * n_paths);
     need(d_shadow_contrib, 16 * n_paths);
     need(d_shadow_visible, 4 * n_paths);
+    const void* accum_before = d_accum.p;
     const bool fresh_images = n_pixels != image_pixels;
     need(d_image, 16 * n_pixels);
     need(d_accum, 16 * n_pixels);
@@ -235,8 +236,16 @@ wf_status wf_renderer::ensure_frame_buffers(size_t n_paths, size_t n_pixels)
     need(d_dyn, sizeof(FrameDyn));
     if (e != hipSuccess) return hip_fail(e, "frame buffer allocation");
     if (moved) drop_graph();
-    if (fresh_images)
+    if (fresh_images || d_accum.p != accum_before) {
+        // the accumulator starts zeroed, as the reference's render target
+        // does (renderer.cpp:484-487): a first frame's weight-1 blend,
+        // accum * 0 + energy, is then exactly the frame
         for (auto& sl : slots) sl.accum_count = 0;
+        const size_t words = 4 * n_pixels, chunk = size_t(1) << 28;
+        for (size_t off = 0; off < words && e == hipSuccess; off += chunk)
+            e = launch_fill_u32((uint32_t*)d_accum.p + off, 0u, (uint32_t)std::min(chunk, words - off), stream);
+        if (e != hipSuccess) return hip_fail(e, "accumulator clear");
+    }
     image_pixels = n_pixels;
     frame_capacity = n_paths;
     return WF_OK;
@@ -428,7 +437,7 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
             (uint64_t)(uintptr_t)d_tnodes.p, (uint64_t)(uintptr_t)d_tpairs.p, ext_cfg.refill_min, ext_cfg.tlas_lds, ext_cfg.visits, ext_grid_tlds, n_tpairs, (uint64_t)(uintptr_t)d_inst.p, (uint64_t)(uintptr_t)d_shade_inst.p,
             (uint64_t)(uintptr_t)d_emitters.p, (uint64_t)(uintptr_t)d_textures.p, (uint64_t)(uintptr_t)d_spill.p,
             (uint64_t)(uintptr_t)d_rays[0].p, (uint64_t)(uintptr_t)d_image.p, slots.size(), ext_cfg.layout, settings.separate_connect, (uint64_t)(uint32_t)A.env_tex,
-            (uint64_t)(uintptr_t)d_tquads.p, A.uniform_diffuse, A.view_mode};
+            (uint64_t)(uintptr_t)d_tquads.p, A.uniform_diffuse, A.view_mode, ext_cfg.big};
         if (!graph_exec || key != graph_key) {
             drop_graph();
             hipGraph_t g = nullptr;
@@ -484,7 +493,9 @@ wf_status wf_renderer::render_range(uint32_t first, uint32_t count)
     image_valid = true;
     last_args = A;
     forward_valid = true;
-    for (uint32_t k = first; k < first + count; ++k) slots[k].accum_count++;
+    // renderer.cpp:593-596: the sample count grows only while accumulating
+    if (settings.accumulate)
+        for (uint32_t k = first; k < first + count; ++k) slots[k].accum_count++;
     if (ov) return fail(WF_ERROR_STACK_OVERFLOW, "%u rays overflowed the traversal stack this frame", ov);
     return WF_OK;
 }

@@ -275,7 +275,7 @@ hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStre
     if (L.counters) c.layout = 0;         // traversal counters come from the 2-wide kernels
     // the host knows the queues' capacities, not their lengths: capacities of
     // tens of millions mark the launches of a large batch
-    const bool big = (uint64_t)L.n + L.n2 >= kBigLaunch;
+    const bool big = c.big == 1 || (c.big == 0 && (uint64_t)L.n + L.n2 >= kBigLaunch);
     void* k = pick_kernel(c, L.counters != nullptr, L.any_hit != 0, tlds, L.rays2 != nullptr, big);
     if (!k) return hipErrorInvalidValue;
     ExtendLaunch arg = L;

@@ -177,10 +177,24 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(FrameArgs A, uint32_t d)
     }
 }
 
-// post_process.hlsl:25-72 after averaging the spp paths of each pixel.
+// R8G8B8A8_UNORM store of one channel: NaN -> 0, clamp to [0, 1], nearest.
+__device__ __forceinline__ uint32_t unorm8(float x)
+{
+    return (uint32_t)fminf(fmaxf(x * 255.0f + 0.5f, 0.0f), 255.0f);
+}
+
+// post_process.hlsl:25-72 after averaging the spp paths of each pixel.  The
+// accumulator blends only for the path tracer's own image (:44-51); debug
+// views are written as they are.  Reinhard applies to the path tracer's image
+// only (:56-60) and linear -> sRGB (common.hlsl:251-259) to it and to the two
+// colour views, base colour and emissive (:63-69).
 __global__ void k_resolve_post(FrameArgs A)
 {
     const uint32_t stride = gridDim.x * blockDim.x;
+    const bool tracer = A.view_mode == WF_RENDER_VIEW_MODE_NONE;
+    const bool blend = A.accumulate && tracer;
+    const bool srgb = tracer || A.view_mode == WF_RENDER_VIEW_MODE_MATERIAL_BASE_COLOR ||
+                      A.view_mode == WF_RENDER_VIEW_MODE_MATERIAL_EMISSIVE;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n_pix; i += stride) {
         const uint32_t q = A.pix_base + i;
         const GpuScene& sc = A.scenes[scene_of_pixel(A, q)];
@@ -195,9 +209,10 @@ __global__ void k_resolve_post(FrameArgs A)
         A.image[q] = e;
         // badness detector: NaN / Inf -> magenta
         if (!(isfinite(e.x) && isfinite(e.y) && isfinite(e.z))) e = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
-        float4 acc = A.accum[q];
-        const float w = 1.0f / (float)sc.sample_count;
-        if (A.accumulate) {
+        float4 acc;
+        if (blend) {
+            acc = A.accum[q];
+            const float w = 1.0f / (float)sc.sample_count;
             acc.x = acc.x * (1.0f - w) + e.x * w;
             acc.y = acc.y * (1.0f - w) + e.y * w;
             acc.z = acc.z * (1.0f - w) + e.z * w;
@@ -206,17 +221,15 @@ __global__ void k_resolve_post(FrameArgs A)
             acc = e;
         }
         A.accum[q] = acc;
-        // Reinhard, then linear -> sRGB (common.hlsl:251-259)
-        const float c[3] = {acc.x / (1.0f + acc.x), acc.y / (1.0f + acc.y), acc.z / (1.0f + acc.z)};
-        uint32_t rgba = 0;
+        float c[3] = {acc.x, acc.y, acc.z};
         for (int k = 0; k < 3; ++k) {
-            const float x = fminf(fmaxf(c[k], 0.0f), 1.0f);
-            const float sv = (x <= 0.0031308f) ? x * 12.92f : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
-            const uint32_t b = (uint32_t)fminf(fmaxf(sv * 255.0f + 0.5f, 0.0f), 255.0f);
-            rgba |= b << (8 * k);
+            if (tracer) c[k] = c[k] / (1.0f + c[k]);
+            if (srgb) {
+                const float x = fminf(fmaxf(c[k], 0.0f), 1.0f);
+                c[k] = (x <= 0.0031308f) ? x * 12.92f : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f;
+            }
         }
-        rgba |= 255u << 24;
-        A.final_rgba8[q] = rgba;
+        A.final_rgba8[q] = unorm8(c[0]) | unorm8(c[1]) << 8 | unorm8(c[2]) << 16 | unorm8(acc.w) << 24;
     }
 }
 

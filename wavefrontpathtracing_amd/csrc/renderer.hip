@@ -217,8 +217,15 @@ extern "C" WF_API wf_status wf_renderer_set_settings(wf_renderer* r, const wf_re
     if (s->use_software_rt == 0) return r->fail(WF_ERROR_INVALID_ARGUMENT, "only software ray tracing is available");
     if (s->render_view_mode >= WF_RENDER_VIEW_MODE_COUNT)
         return r->fail(WF_ERROR_INVALID_ARGUMENT, "render_view_mode %u is not a view mode", s->render_view_mode);
+    // renderer.cpp:1099-1151 resets the accumulator when the view mode, max
+    // bounces, accumulate, cosine-weighted sampling or the environment strength
+    // change; spp and next-event estimation have no reference counterpart and
+    // change the image too
     const bool reset = s->max_bounces != r->settings.max_bounces || s->spp != r->settings.spp ||
-                       s->next_event_estimation != r->settings.next_event_estimation;
+                       s->next_event_estimation != r->settings.next_event_estimation ||
+                       s->render_view_mode != r->settings.render_view_mode || s->accumulate != r->settings.accumulate ||
+                       s->cosine_weighted_diffuse != r->settings.cosine_weighted_diffuse ||
+                       std::memcmp(&s->hdr_env_strength, &r->settings.hdr_env_strength, sizeof(float)) != 0;
     if (std::memcmp(s, &r->settings, sizeof(*s)) != 0) r->forward_valid = false;
     r->settings = *s;
     if (reset)
@@ -239,6 +246,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
         return r->fail(WF_ERROR_INVALID_ARGUMENT, "the 4-wide layout needs skip_popped_interior = 1 and tlas_in_lds = 0");
     if (c->refill_min > 64) return r->fail(WF_ERROR_INVALID_ARGUMENT, "refill_min must be in [0, 64]");
     if (c->interior_visits > 8) return r->fail(WF_ERROR_INVALID_ARGUMENT, "interior_visits must be in [0, 8]");
+    if (c->big_claims > 2) return r->fail(WF_ERROR_INVALID_ARGUMENT, "big_claims must be 0 (by capacity), 1 (always) or 2 (never)");
     const ExtendConfig old = r->ext_cfg;
     r->ext_cfg.block = c->workgroup_size;
     r->ext_cfg.lds_stack = c->lds_stack_entries;
@@ -248,6 +256,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     r->ext_cfg.tlas_lds = c->tlas_in_lds ? 1u : 0u;
     r->ext_cfg.visits = c->interior_visits ? c->interior_visits : ExtendConfig{}.visits;
     r->ext_cfg.layout = c->bvh_layout;
+    r->ext_cfg.big = c->big_claims;
     if (r->ext_cfg.layout != old.layout) r->tlas_valid = false;   // instance records point at the layout's arrays
     r->drop_graph();
     const wf_status s = r->ensure_trace_scratch();
@@ -266,6 +275,7 @@ extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* 
     out->refill_min = r->ext_cfg.refill_min;
     out->tlas_in_lds = r->ext_cfg.tlas_lds;
     out->interior_visits = r->ext_cfg.visits;
+    out->big_claims = r->ext_cfg.big;
     return WF_OK;
 }
 
@@ -674,10 +684,12 @@ extern "C" WF_API wf_status wf_renderer_get_scene_bounds(const wf_renderer* r, f
 
 wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
                                   uint32_t* d_visible, uint32_t* d_counters, bool any, const uint32_t* d_roots,
-                                  const OcclusionQueue* second)
+                                  const OcclusionQueue* second, bool reset_overflow)
 {
     if (second && (any || d_counters)) return fail(WF_ERROR_INVALID_ARGUMENT, "a second queue needs a closest-hit launch");
-    hipError_t e = launch_fill_u32((uint32_t*)d_work.p, 0u, 1, stream);   // work cursor; word 1 counts overflows
+    // word 0 is the work cursor; word 1 counts overflows, cleared here only
+    // when the caller asks (a frame counts them over all of its launches)
+    hipError_t e = launch_fill_u32((uint32_t*)d_work.p, 0u, reset_overflow ? 2 : 1, stream);
     if (e != hipSuccess) return hip_fail(e, "work counter reset");
     ExtendLaunch L{};
     L.scene.tnodes = (const wf_tlas_node*)d_tnodes.p;
@@ -778,6 +790,13 @@ extern "C" WF_API wf_status wf_trace_closest_device(wf_renderer* r, const wf_ray
     if (!r || (n && (!d_rays || !d_hits))) return WF_ERROR_INVALID_ARGUMENT;
     wf_status s = r->upload_scene();
     if (s != WF_OK) return s;
-    if (n == 0) return WF_OK;
-    return r->run_extend(d_rays, nullptr, n, d_hits, nullptr, nullptr, false);
+    if (n == 0) return r->reset_overflows();
+    return r->run_extend(d_rays, nullptr, n, d_hits, nullptr, nullptr, false, nullptr, nullptr, true);
+}
+
+extern "C" WF_API wf_status wf_renderer_read_stack_overflows(wf_renderer* r, uint32_t* out)
+{
+    if (!r || !out) return WF_ERROR_INVALID_ARGUMENT;
+    if (!r->d_work.p) { *out = 0; return WF_OK; }
+    return r->read_overflows(out);
 }

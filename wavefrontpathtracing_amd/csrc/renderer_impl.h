@@ -136,7 +136,7 @@ struct wf_renderer {
     };
     wf_status run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
                          uint32_t* d_visible, uint32_t* d_counters, bool any, const uint32_t* d_roots = nullptr,
-                         const OcclusionQueue* second = nullptr);
+                         const OcclusionQueue* second = nullptr, bool reset_overflow = false);
     wf_status read_overflows(uint32_t* out);
     wf_status reset_overflows();
     wf_status ensure_frame_buffers(size_t n_paths, size_t n_pixels);

@@ -107,6 +107,7 @@ struct ExtendConfig {
     uint32_t tlas_lds = 0;      // copy the TLAS records to LDS when they fit
     uint32_t visits = 4;        // interior visits per pass of the traversal loop
     uint32_t layout = 0;        // 0: 2-wide pair records, 1: 4-wide records
+    uint32_t big = 0;           // large claims in mixed launches: 0 by capacity, 1 always, 2 never
 };
 
 // Launch the closest-hit kernel.  d_count (device) overrides n when non-null.

@@ -295,6 +295,12 @@ class Renderer:
         """Device pointers (e.g. torch tensor data_ptr()); ordered on the renderer's stream."""
         self._check(lib().wf_trace_closest_device(self.handle, C.c_void_p(d_rays), n, C.c_void_p(d_hits)), "trace_closest_device")
 
+    def read_stack_overflows(self) -> int:
+        """Rays that overflowed the traversal stack in the last trace call or frame (synchronises)."""
+        out = C.c_uint32()
+        self._check(lib().wf_renderer_read_stack_overflows(self.handle, C.byref(out)), "read_stack_overflows")
+        return out.value
+
     # ---- readbacks ----
     def read_energy(self) -> np.ndarray:
         out = np.empty((self.render_height, self.render_width, 4), dtype=np.float32)

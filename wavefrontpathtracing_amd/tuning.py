@@ -18,9 +18,9 @@ DEFAULT = dict(block=64, lds_stack=16, skip_popped=1, bvh_layout=0, refill_min=1
 def set_extend_config(renderer, block: int = DEFAULT["block"], lds_stack: int = DEFAULT["lds_stack"],
                       skip_popped: int = DEFAULT["skip_popped"], bvh_layout: int = DEFAULT["bvh_layout"], stack_limit: int = 0,
                       refill_min: int = DEFAULT["refill_min"], tlas_in_lds: int = DEFAULT["tlas_in_lds"],
-                      interior_visits: int = DEFAULT["interior_visits"]) -> None:
+                      interior_visits: int = DEFAULT["interior_visits"], big_claims: int = 0) -> None:
     c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout, stack_limit, refill_min, tlas_in_lds,
-                          interior_visits)
+                          interior_visits, big_claims)
     L.check(L.lib().wf_renderer_set_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "set_traversal_config")
 
 
@@ -29,7 +29,7 @@ def get_extend_config(renderer) -> dict:
     L.check(L.lib().wf_renderer_get_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "get_traversal_config")
     return dict(block=c.workgroup_size, lds_stack=c.lds_stack_entries, skip_popped=c.skip_popped_interior,
                 bvh_layout=c.bvh_layout, stack_limit=c.stack_limit, refill_min=c.refill_min,
-                tlas_in_lds=c.tlas_in_lds, interior_visits=c.interior_visits)
+                tlas_in_lds=c.tlas_in_lds, interior_visits=c.interior_visits, big_claims=c.big_claims)
 
 
 def candidates():
@@ -41,10 +41,19 @@ def candidates():
         yield dict(block=block, lds_stack=lds, skip_popped=skip, bvh_layout=layout)
 
 
-def autotune(renderer, time_fn, configs=None, repeats: int = 3):
+def label(cfg: dict) -> str:
+    """Short name of a configuration: block/lds/skip/layout/visits."""
+    c = {**DEFAULT, **cfg}
+    return f"{c['block']}/{c['lds_stack']}/{c['skip_popped']}/{c['bvh_layout']}/{c['interior_visits']}"
+
+
+def autotune(renderer, time_fn, configs=None, repeats: int = 3, margin: float = 0.01):
     """Time every configuration with ``time_fn()`` (returns seconds for one
-    measured pass, run after the config is set) and keep the fastest.
-    Returns (best_config, [(config, seconds)])."""
+    measured pass, run after the config is set) and keep the fastest, unless
+    it beats the first configuration in the list that equals DEFAULT by less
+    than `margin` (then the default stays, so run-to-run noise between
+    near-equal configurations does not flip the choice).
+    Returns (chosen_config, [(config, seconds)] fastest first)."""
     results = []
     for cfg in (configs or list(candidates())):
         set_extend_config(renderer, **cfg)
@@ -52,5 +61,10 @@ def autotune(renderer, time_fn, configs=None, repeats: int = 3):
         best = min(time_fn() for _ in range(repeats))
         results.append((cfg, best))
     results.sort(key=lambda x: x[1])
-    set_extend_config(renderer, **results[0][0])
-    return results[0][0], results
+    chosen = results[0][0]
+    for cfg, t in results:
+        if {**DEFAULT, **cfg} == DEFAULT and results[0][1] > t * (1.0 - margin):
+            chosen = cfg
+            break
+    set_extend_config(renderer, **chosen)
+    return chosen, results
