@@ -14,7 +14,8 @@ import numpy as np
 from wavefrontpathtracing_amd import _lib as L
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+# WF_ORACLE_LIB selects another build of the oracle (the sanitised one, for instance)
+ORACLE_PATH = os.environ.get("WF_ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 
 COUNTERS = np.dtype([("tlas_nodes", "<u4"), ("blas_nodes", "<u4"), ("instances", "<u4"), ("tri_tests", "<u4")])
 VERTEX_REC = np.dtype([("kind", "<u4"), ("inst", "<u4"), ("emit_counted", "<u4"), ("rr", "<u4"), ("noL", "<f4"),

@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libwf.so")
+# WF_LIB selects another build of the library (the sanitised host build of
+# build.build_sanitized, for instance); by default the in-tree libwf.so
+LIB_PATH = os.environ.get("WF_LIB") or os.path.join(_HERE, "libwf.so")
 
 # ---- wire formats (numpy) ---------------------------------------------------
 VERTEX = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3), ("tangent", "<f4", 4), ("uv", "<f4", 2)])

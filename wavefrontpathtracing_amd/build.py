@@ -70,3 +70,37 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)
+
+
+SAN_FLAGS = ["-fsanitize=address,undefined,float-cast-overflow", "-fno-sanitize-recover=undefined",
+             "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized(out_dir: str, verbose: bool = False) -> str:
+    """A copy of the library whose host C++ sources (the BLAS/TLAS builders,
+    camera, scene generators, image files) are compiled with AddressSanitizer
+    and UndefinedBehaviorSanitizer, linked with the regular HIP objects.  Load
+    it with WF_LIB=<path> and LD_PRELOAD of gcc's libasan (the CPU tests do)."""
+    os.makedirs(out_dir, exist_ok=True)
+    hip_objs = [os.path.join(BUILD, os.path.basename(s) + ".o") for s in _sources() if s.endswith(".hip")]
+    if not all(os.path.exists(o) for o in hip_objs):
+        build()
+    objs = []
+    for src in _sources():
+        if not src.endswith(".cpp"):
+            continue
+        obj = os.path.join(out_dir, os.path.basename(src) + ".o")
+        cmd = ["g++", "-c", src, "-o", obj] + [f for f in COMMON if f != "-O3"] + ["-O1"] + SAN_FLAGS
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    lib = os.path.join(out_dir, "libwf.so")
+    cmd = ["g++", "-shared", "-fPIC", "-o", lib] + objs + hip_objs + SAN_FLAGS + \
+          ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lamdhip64", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {lib}")
+    return lib
