@@ -58,3 +58,47 @@ def test_inverse_steps_lower_the_loss():
                    fit_emission=False)
     assert hist[-1]["loss"] < 0.7 * hist[0]["loss"], [h["loss"] for h in hist]
     assert hist[-1]["texel_mae"] < hist[0]["texel_mae"]
+
+
+def test_texels_near_zero_keep_their_gradient():
+    """Path replay divides a vertex's remaining radiance by its albedo, so
+    where the bilinear albedo is exactly 0 (every texel of the footprint at
+    0, here a 3x3 block around texel j in one channel) the texels get no
+    gradient; inverse.run keeps optimised texels at or above TEXEL_MIN
+    instead.  With the block at TEXEL_MIN the gradient of the image sum is
+    the derivative of an estimate that is linear in the texels (up to paths
+    that meet the block twice): equal within 1 % with the block at TEXEL_MIN
+    and at 2 * TEXEL_MIN, and positive.  A run with a learning rate that
+    would drive texels below 0 ends with every texel >= TEXEL_MIN."""
+    import torch
+    sc = textured_small_scene()
+    views = INV.orbit_views(2, 22.0, (0.0, 0.0, 0.0), 5)
+    prob = INV.InverseProblem(sc, views, 40, 32, spp=4, max_bounces=3)
+    try:
+        prob.r.update_settings(frame_seed=13)
+        th, tw = prob.tex_shape
+        em = torch.tensor([sc.instances[k].emissive_strength for k in prob.emitters], dtype=torch.float32)
+
+        def grad_at(j, value):
+            t = torch.full((th, tw, 4), 0.5, dtype=torch.float32, device="cuda")
+            t[..., 3] = 1.0
+            if j is not None:
+                y, x = divmod(j, tw)
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        t[(y + dy) % th, (x + dx) % tw, 0] = value
+            t.requires_grad_(True)
+            prob.render(torch, t, em)[:, :3].sum().backward()
+            return t.grad.view(-1, 4)[:, 0].detach().cpu().numpy()
+
+        g = grad_at(None, None)
+        j = int(np.argmax(g))
+        assert g[j] > 0
+        assert grad_at(j, 0.0)[j] == 0.0
+        g1, g2 = grad_at(j, INV.TEXEL_MIN)[j], grad_at(j, 2 * INV.TEXEL_MIN)[j]
+        assert g1 > 0 and abs(g1 - g2) <= 0.01 * g2, (g1, g2)
+    finally:
+        prob.close()
+    hist = INV.run(steps=3, views=2, res=40, spp=2, max_bounces=3, lr=2.0, seed=4, log=None, scene=textured_small_scene(),
+                   frame_seed=11, fit_emission=False)
+    assert min(h["texel_min"] for h in hist) >= INV.TEXEL_MIN * (1 - 1e-6)

@@ -8,6 +8,11 @@ CUDA tensor.  Its backward runs the library's path-replay adjoint of that
 frame (wf_renderer_render_adjoint_device), so a backward must follow its
 forward before the renderer renders again.
 
+The gradient with respect to a texel whose value is exactly 0 is 0: path
+replay divides a vertex's remaining radiance by its albedo, and both vanish
+there.  Optimisers should keep texels at or above a small positive value
+(inverse.TEXEL_MIN), where the gradient is the derivative of the estimate.
+
 Both directions synchronise the caller's current torch stream before the
 library reads torch memory, and the library synchronises its own stream
 before returning, so no stream ordering is left to the caller.

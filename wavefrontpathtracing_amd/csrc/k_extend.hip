@@ -52,7 +52,9 @@ constexpr uint64_t kBigLaunch = 16ull << 20;  // queue capacity from which a mix
 // A lane whose ray finishes writes its result and goes idle; once at least
 // `refill_min` lanes of the wave are idle, the wave claims that many rays
 // from the global cursor with one atomic (ballot + mbcnt give each idle lane
-// its index).  The loop ends when every lane is idle and the queue is
+// its index), and the new rays start with their root box test in the same
+// iteration in which the other lanes make their pass (and then make their
+// own first pass).  The loop ends when every lane is idle and the queue is
 // drained.
 template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4, bool MIX, bool BIG, class Stack>
 __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, const float4* s_tl, uint32_t lane, uint32_t n)
@@ -83,14 +85,17 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
         const uint32_t n_idle = (uint32_t)__popcll(idle);
         if (drained && n_idle == 64u) break;
         int status = kRun;
+        // refill: idle lanes claim queue entries and start them; lanes that
+        // hold rays make their pass in the same iteration
         if (!drained && n_idle > 0u && (n_idle >= refill_min || n_idle == 64u)) {
-            uint32_t q;
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+            uint64_t q = ~0ull;
             if constexpr (BIG) {
                 if (lb >= le) {
                     uint32_t g = n_idle;
                     const uint64_t big = (ntot > hint ? ntot - hint : 0ull) / share;
                     if (big > g) g = big < kClaim ? (uint32_t)big : kClaim;
-                    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
                     uint32_t base = 0;
                     if (lane == leader) base = atomicAdd(&L.work[0], g);
                     base = __shfl(base, (int)leader, 64);
@@ -103,38 +108,32 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
                 const uint64_t first = lb;
                 lb += take;
                 if (last_claim && lb >= le) drained = true;
-                if (ray_i != ~0u) continue;
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-                if (rank >= take) continue;
-                if (first + rank >= ntot) continue;
-                q = (uint32_t)(first + rank);
+                if (ray_i == ~0u && rank < take && first + rank < ntot) q = first + rank;
             } else {
-                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
                 uint32_t base = 0;
                 if (lane == leader) base = atomicAdd(&L.work[0], n_idle);
                 base = __shfl(base, (int)leader, 64);
                 if ((uint64_t)base + n_idle >= ntot) drained = true;
-                if (ray_i != ~0u) continue;
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-                q = base + rank;
-                if ((uint64_t)q >= ntot) continue;
+                if (ray_i == ~0u && (uint64_t)base + rank < ntot) q = (uint64_t)base + rank;
             }
-            if constexpr (MIX) {
-                const bool second = q >= n;
-                const uint32_t i = second ? q - n : q;
-                ray_i = i;
-                lane_any = second;
-                const uint32_t root = second ? (L.ray_root2 ? L.ray_root2[i] : 0u) : (L.ray_root ? L.ray_root[i] : 0u);
-                if (!trace_begin<COUNT, L4>(s, sc, second ? L.rays2[i] : L.rays[i], root, cnt)) status = kDone;
-            } else {
-                ray_i = q;
-                const uint32_t root = L.ray_root ? L.ray_root[q] : 0u;
-                if (!trace_begin<COUNT, L4>(s, sc, L.rays[q], root, cnt)) status = kDone;
+            if (q != ~0ull) {
+                if constexpr (MIX) {
+                    const bool second = q >= n;
+                    const uint32_t i = second ? (uint32_t)(q - n) : (uint32_t)q;
+                    ray_i = i;
+                    lane_any = second;
+                    const uint32_t root = second ? (L.ray_root2 ? L.ray_root2[i] : 0u) : (L.ray_root ? L.ray_root[i] : 0u);
+                    if (!trace_begin<COUNT, L4>(s, sc, second ? L.rays2[i] : L.rays[i], root, cnt)) status = kDone;
+                } else {
+                    ray_i = (uint32_t)q;
+                    const uint32_t root = L.ray_root ? L.ray_root[q] : 0u;
+                    if (!trace_begin<COUNT, L4>(s, sc, L.rays[q], root, cnt)) status = kDone;
+                }
             }
-        } else if (ray_i != ~0u) {
-            status = trace_pass<SKIP, COUNT, TLDS, L4>(s, sc, st, s_tl, visits, MIX ? lane_any : ANY, cnt);
         }
-        if (status != kRun) {
+        if (ray_i != ~0u && status == kRun)
+            status = trace_pass<SKIP, COUNT, TLDS, L4>(s, sc, st, s_tl, visits, MIX ? lane_any : ANY, cnt);
+        if (ray_i != ~0u && status != kRun) {
             if (status == kOverflow) {
                 atomicAdd(&L.work[1], 1u);
                 s.h_inst = ~0u; s.h_prim = ~0u; s.h_u = 0.0f; s.h_v = 0.0f;
@@ -166,7 +165,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
 {
     constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
     __shared__ uint32_t s_ent[kLdsN];
-    __shared__ uint16_t s_dist[kLdsN];
+    __shared__ uint16_t s_dist[SKIP ? kLdsN : 1];   // entry distances: only the popped-interior skip reads them
     float4* s_tl = s_tlas_dyn;
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
@@ -177,8 +176,8 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
         __syncthreads();
     }
     if constexpr (LDS > 0) {
-        LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, L.spill + gtid, L.spill_per_thread, gridDim.x * BLOCK,
-                                L.stack_cap};
+        LdsStack<BLOCK, LDS, SKIP> st{s_ent + tid, s_dist + (SKIP ? tid : 0u), L.spill + gtid, L.spill_per_thread,
+                                      gridDim.x * BLOCK, L.stack_cap};
         extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4, MIX, BIG>(L, st, s_tl, tid & 63u, n);
     } else {
         PrivStack<128> st;

@@ -67,6 +67,44 @@ __device__ __forceinline__ bool tri_test(const float4& a, const float4& b, const
     return true;
 }
 
+__device__ __forceinline__ float4 ld4(const void* p) { return *reinterpret_cast<const float4*>(p); }
+
+// Loads and stores through the global address space (the pointers below all
+// point to device memory; generic pointers would compile to flat accesses).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef float __attribute__((ext_vector_type(4))) native_float4;
+typedef uint32_t __attribute__((ext_vector_type(2))) native_uint2;
+typedef __attribute__((address_space(1))) const native_float4 gfloat4;
+typedef __attribute__((address_space(1))) const uint32_t guint;
+typedef __attribute__((address_space(1))) const native_uint2 guint2;
+typedef __attribute__((address_space(1))) uint32_t gwuint;
+typedef __attribute__((address_space(1))) native_uint2 gwuint2;
+__device__ __forceinline__ float4 gld4(const void* p)
+{
+    const native_float4 v = *(gfloat4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t gldu(const void* p) { return *(guint*)p; }
+__device__ __forceinline__ uint2 gldu2(const void* p)
+{
+    const native_uint2 v = *(guint2*)p;
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ void gstu(void* p, uint32_t v) { *(gwuint*)p = v; }
+__device__ __forceinline__ void gstu2(void* p, uint2 v)
+{
+    native_uint2 w;
+    w.x = v.x; w.y = v.y;
+    *(gwuint2*)p = w;
+}
+#else
+__device__ __forceinline__ float4 gld4(const void* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ uint32_t gldu(const void* p) { return *reinterpret_cast<const uint32_t*>(p); }
+__device__ __forceinline__ uint2 gldu2(const void* p) { return *reinterpret_cast<const uint2*>(p); }
+__device__ __forceinline__ void gstu(void* p, uint32_t v) { *reinterpret_cast<uint32_t*>(p) = v; }
+__device__ __forceinline__ void gstu2(void* p, uint2 v) { *reinterpret_cast<uint2*>(p) = v; }
+#endif
+
 // ---- stacks -------------------------------------------------------------------
 // An entry is a child reference plus its slab entry distance.  Encodings:
 //   interior child:  ref                      (bit 31 clear; the child's pair record index)
@@ -91,8 +129,12 @@ __device__ __forceinline__ float half_up_to_float(uint16_t b) { return (float)__
 
 // Entries past the LDS depth spill to global memory, level-major: level k of
 // thread g sits at spill[k * spill_stride + g], so lanes of a wave at the same
-// depth touch adjacent 8-byte words.
-template <int BLOCK, int LDS>
+// depth touch adjacent 8-byte words.  DIST: the entry distance is kept (the
+// popped-interior skip reads it); without the skip only references are
+// stored.  A pop always reads LDS (at the clamped level) and reads the spill
+// area only past the LDS depth, so the two reads stay separate LDS and global
+// loads rather than one generic (flat) load through a selected pointer.
+template <int BLOCK, int LDS, bool DIST = true>
 struct LdsStack {
     uint32_t* ent;
     uint16_t* dist;
@@ -103,17 +145,27 @@ struct LdsStack {
     __device__ __forceinline__ bool push(uint32_t sp, uint32_t e, float d)
     {
         if (sp >= cap) return false;
-        if (sp < (uint32_t)LDS) { ent[sp * BLOCK] = e; dist[sp * BLOCK] = half_down(d); return true; }
+        if (sp < (uint32_t)LDS) {
+            ent[sp * BLOCK] = e;
+            if (DIST) dist[sp * BLOCK] = half_down(d);
+            return true;
+        }
         const uint32_t k = sp - LDS;
         if (k >= spill_n) return false;
-        spill[(size_t)k * spill_stride] = make_uint2(e, __float_as_uint(d));
+        if (DIST) gstu2(spill + (size_t)k * spill_stride, make_uint2(e, __float_as_uint(d)));
+        else gstu(spill + (size_t)k * spill_stride, e);
         return true;
     }
     __device__ __forceinline__ void get(uint32_t sp, uint32_t& e, float& d) const
     {
-        if (sp < (uint32_t)LDS) { e = ent[sp * BLOCK]; d = half_up_to_float(dist[sp * BLOCK]); return; }
-        const uint2 v = spill[(size_t)(sp - LDS) * spill_stride];
-        e = v.x; d = __uint_as_float(v.y);
+        const uint32_t i = sp < (uint32_t)LDS ? sp : (uint32_t)LDS - 1u;
+        e = ent[i * BLOCK];
+        d = DIST ? half_up_to_float(dist[i * BLOCK]) : 0.0f;
+        if (sp >= (uint32_t)LDS) {
+            const uint2* q = spill + (size_t)(sp - LDS) * spill_stride;
+            if (DIST) { const uint2 v = gldu2(q); e = v.x; d = __uint_as_float(v.y); }
+            else e = gldu(q);
+        }
     }
 };
 
@@ -131,26 +183,15 @@ struct PrivStack {
     __device__ __forceinline__ void get(uint32_t sp, uint32_t& e, float& d) const { e = ent[sp]; d = dist[sp]; }
 };
 
-__device__ __forceinline__ float4 ld4(const void* p) { return *reinterpret_cast<const float4*>(p); }
-
-// Loads through the global address space (the pointers below all point to
-// device memory; generic pointers would compile to flat loads).
-#if defined(__HIP_DEVICE_COMPILE__)
-typedef float __attribute__((ext_vector_type(4))) native_float4;
-typedef __attribute__((address_space(1))) const native_float4 gfloat4;
-typedef __attribute__((address_space(1))) const uint32_t guint;
-__device__ __forceinline__ float4 gld4(const void* p)
-{
-    const native_float4 v = *(gfloat4*)p;
-    return make_float4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ uint32_t gldu(const void* p) { return *(guint*)p; }
-#else
-__device__ __forceinline__ float4 gld4(const void* p) { return *reinterpret_cast<const float4*>(p); }
-__device__ __forceinline__ uint32_t gldu(const void* p) { return *reinterpret_cast<const uint32_t*>(p); }
-#endif
-
 enum LaneStatus { kRun = 0, kDone = 1, kOverflow = 2 };
+
+// Stack entry of a child (encodings above), as selects rather than branches.
+__device__ __forceinline__ uint32_t encode_entry(uint32_t ref, uint32_t kind, uint32_t id)
+{
+    const bool packs = kind < 16u && ref < (1u << 27);
+    const uint32_t leaf = 0x80000000u | (packs ? (kind << 27) | ref : id);
+    return kind == 0u ? ref : leaf;
+}
 
 // Per-lane traversal state.  `ref`/`kind` name the lane's node: kind 0 =
 // interior node whose child pair sits at nodes + ref * 32 bytes (`nodes` =
@@ -237,10 +278,7 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
     // push a far child (an interior child's reference; a leaf's ref/kind when
     // they fit the entry, else the node's index to read them back at the pop)
     auto push_far = [&](uint32_t f_ref, uint32_t f_kind, uint32_t f_id, float df) {
-        uint32_t e;
-        if (f_kind == 0) e = f_ref;
-        else if (f_kind < 16u && f_ref < (1u << 27)) e = 0x80000000u | (f_kind << 27) | f_ref;
-        else e = 0x80000000u | f_id;
+        const uint32_t e = encode_entry(f_ref, f_kind, f_id);
         if (st.push(s.sp, e, df)) ++s.sp;
         else status = kOverflow;
     };
@@ -298,10 +336,7 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
         } else {
             s.ref = n_ref; s.kind = n_kind;
             if (df != kRayMaxT) {
-                uint32_t e;
-                if (f_kind == 0) e = f_ref;
-                else if (f_kind < 16u && f_ref < (1u << 27)) e = 0x80000000u | (f_kind << 27) | f_ref;
-                else e = 0x80000000u | f_id;
+                const uint32_t e = encode_entry(f_ref, f_kind, f_id);
                 if (st.push(s.sp, e, df)) ++s.sp;
                 else status = kOverflow;
             }

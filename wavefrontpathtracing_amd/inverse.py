@@ -6,7 +6,13 @@ in one set of launches and the path-replay adjoint of that frame accumulates
 the texel and emission gradients of every view with atomics into one texture
 gradient.  Paths are recomputed by the adjoint, never stored.  The images are
 compared with target images rendered from the true texture; Adam updates the
-texture (torch), clamped to [0, 1].
+texture (torch), clamped to [TEXEL_MIN, 1].
+
+Path replay obtains a texel's gradient as the remaining radiance of a path
+vertex divided by the vertex's albedo; at an albedo of exactly 0 both are 0
+and the gradient is reported as 0.  The optimiser therefore keeps texels at
+or above TEXEL_MIN, where the gradient is the derivative of the (nearly
+linear) estimate and a texel driven towards 0 can come back.
 
 `python -m wavefrontpathtracing_amd.inverse --steps 200` runs the full
 optimisation on the 256-instance scene (64 orbit views at 512x512, 8 spp,
@@ -24,6 +30,8 @@ import numpy as np
 from . import _lib as L
 from . import scenes
 from .renderer import Material, Renderer, camera_create
+
+TEXEL_MIN = 1e-4   # lower clamp of optimised texels (see the module docstring)
 
 
 def orbit_views(n: int, radius: float, center, seed: int):
@@ -136,13 +144,14 @@ def run(steps: int = 200, views: int = 64, res: int = 512, spp: int = 8, max_bou
                 texels.grad[..., 3] = 0.0
             opt.step()
             with torch.no_grad():
-                texels.clamp_(0.0, 1.0)
+                texels.clamp_(TEXEL_MIN, 1.0)
                 if fit_emission:
                     em.clamp_(min=1e-3)
             torch.cuda.synchronize()
             st = prob.r.frame_stats()
             rec = {"step": it, "loss": float(loss.item()), "s": time.perf_counter() - t0,
                    "texel_mae": float((texels.detach()[..., :3] - t_true[..., :3]).abs().mean().item()),
+                   "texel_min": float(texels.detach()[..., :3].min().item()),
                    "emission": [round(float(x), 4) for x in em.detach()], "forward_rays": st.rays_extended + st.shadow_rays}
             hist.append(rec)
             if log:
