@@ -322,6 +322,16 @@ WF_API wf_status wf_bvh_build(const float* tri_positions, uint32_t n, const wf_b
 WF_API wf_status wf_bvh_build_device(const float* d_tri_positions, uint32_t n, const wf_build_options* options,
                                      wf_bvh_node* d_nodes, uint64_t node_capacity, uint32_t* d_indices,
                                      uint32_t* out_node_count, void* stream);
+/* The same build as wf_bvh_build_device without allocating: no reference
+ * counterpart; for callers that cannot allocate device memory (graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Its scratch is a
+ * fixed device array of 32 MiB per GPU, held until the process ends, enough
+ * for n <= 65536 triangles at any interval count: larger meshes return
+ * WF_ERROR_CAPACITY.  Calls on one GPU share that array and are serialised
+ * within a process.  Same arguments and results as wf_bvh_build_device. */
+WF_API wf_status wf_bvh_build_device_noalloc(const float* d_tri_positions, uint32_t n, const wf_build_options* options,
+                                             wf_bvh_node* d_nodes, uint64_t node_capacity, uint32_t* d_indices,
+                                             uint32_t* out_node_count, void* stream);
 /* A reusable device builder (bvh_builder_t, bvh/bvh_builder.h:14-37): the same
  * build as wf_bvh_build_device with its scratch kept across builds (it grows to
  * the largest mesh and is freed by destroy).  last_build reports the tree

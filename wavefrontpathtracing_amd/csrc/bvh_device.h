@@ -11,12 +11,25 @@ namespace wf {
 constexpr uint32_t kDevBuildMaxBins = 64;   // interval_count the device builder accepts (2..64)
 
 // Device scratch of one build, grown on demand and reusable across builds.
+// A fixed scratch (fixed = true) is never grown: its buffers point into
+// memory the caller owns, and a build that needs more fails with
+// WF_ERROR_CAPACITY; level_buf (level_cap entries) then holds the per-level
+// bookkeeping the build otherwise keeps in a host vector.
 struct DevBuildScratch {
     DevBuf rc[2], rl[2], rh[2];     // per-triangle records in position order (ping-pong)
     DevBuf tfr, tbl;                // partition tables, node-relative positions
     DevBuf nodes, lists, small, blocks, info, bins, chunks, cnt, off, slot_ch, ctr;
+    bool fixed = false;
+    uint32_t (*level_buf)[2] = nullptr;
+    size_t level_cap = 0;
     void release();
 };
+
+// Bytes of every scratch buffer of a build of n triangles at B intervals, in
+// the order rc, rl, rh (each twice), tfr, tbl, nodes, lists, small, blocks,
+// info, bins, chunks, cnt, off, slot_ch, ctr.
+constexpr int kDevBuildBuffers = 19;
+void device_build_sizes(uint32_t n, uint32_t B, size_t out[kDevBuildBuffers]);
 
 struct DevBuildResult {
     uint32_t node_count = 0;

@@ -39,7 +39,10 @@
 #include "bvh_device.h"
 
 #include <cfloat>
+#include <mutex>
 #include <new>
+#include <utility>
+#include <vector>
 
 #pragma clang fp contract(off)
 
@@ -1003,8 +1006,39 @@ inline uint32_t grid_for(uint64_t items, uint32_t per_block, uint32_t cap)
 
 void DevBuildScratch::release()
 {
+    if (fixed) return;   // the memory is the caller's
     for (int b = 0; b < 2; ++b) { rc[b].release(); rl[b].release(); rh[b].release(); }
     for (DevBuf* d : {&tfr, &tbl, &nodes, &lists, &small, &blocks, &info, &bins, &chunks, &cnt, &off, &slot_ch, &ctr}) d->release();
+}
+
+static DevBuf* scratch_buffers(DevBuildScratch& S, DevBuf* out[kDevBuildBuffers])
+{
+    DevBuf* list[kDevBuildBuffers] = {&S.rc[0], &S.rl[0], &S.rh[0], &S.rc[1], &S.rl[1], &S.rh[1], &S.tfr, &S.tbl, &S.nodes,
+                                      &S.lists, &S.small, &S.blocks, &S.info, &S.bins, &S.chunks, &S.cnt, &S.off,
+                                      &S.slot_ch, &S.ctr};
+    for (int k = 0; k < kDevBuildBuffers; ++k) out[k] = list[k];
+    return out[0];
+}
+
+void device_build_sizes(uint32_t n, uint32_t B, size_t out[kDevBuildBuffers])
+{
+    const size_t maxa = n / (kSmall + 1) + 2;
+    const size_t maxch = n / kChunk + maxa + 2;
+    const size_t scan_n = (maxch > maxa ? maxch : maxa) + 1;
+    for (int k = 0; k < 6; ++k) out[k] = 16ull * n;
+    out[6] = 4ull * n;                               // tfr
+    out[7] = 4ull * n;                               // tbl
+    out[8] = sizeof(BNode) * (2ull * n + 1);         // nodes
+    out[9] = 4ull * (2ull * n + 1);                  // lists
+    out[10] = 4ull * n;                              // small
+    out[11] = sizeof(wf_bvh_node) * 2ull * n;        // blocks
+    out[12] = sizeof(SplitInfo) * maxa;              // info
+    out[13] = 4ull * 3 * B * 7 * maxa;               // bins
+    out[14] = 16ull * maxch;                         // chunks
+    out[15] = 8ull * scan_n;                         // cnt
+    out[16] = 8ull * (scan_n + 1);                   // off
+    out[17] = 8ull * maxa;                           // slot_ch
+    out[18] = sizeof(Counters);                      // ctr
 }
 
 hipError_t launch_tris_from_index(const float* d_pos, const uint32_t* d_idx, uint32_t n, GpuTri* d_tris, hipStream_t stream)
@@ -1021,26 +1055,21 @@ wf_status device_build_blas(const float* d_pos, uint32_t n, const wf_build_optio
     if (B < 2 || B > kDevBuildMaxBins) { err = "the device builder takes 2..64 intervals"; return WF_ERROR_INVALID_ARGUMENT; }
     if (n > (1u << 28)) { err = "mesh too large for the device builder"; return WF_ERROR_CAPACITY; }
     const uint32_t single = opt.subdivide_single_prim ? 1u : 0u;
-    const size_t maxa = n / (kSmall + 1) + 2;
-    const size_t maxch = n / kChunk + maxa + 2;
-    const size_t scan_n = (maxch > maxa ? maxch : maxa) + 1;
+    {
+        size_t sizes[kDevBuildBuffers];
+        DevBuf* bufs[kDevBuildBuffers];
+        device_build_sizes(n, B, sizes);
+        scratch_buffers(S, bufs);
+        for (int k = 0; k < kDevBuildBuffers; ++k) {
+            if (S.fixed) {
+                if (!bufs[k]->p || bufs[k]->bytes < sizes[k]) { err = "mesh too large for the fixed scratch"; return WF_ERROR_CAPACITY; }
+            } else if (bufs[k]->ensure(sizes[k]) != hipSuccess) {
+                err = "device builder scratch allocation failed";
+                return WF_ERROR_OUT_OF_MEMORY;
+            }
+        }
+    }
     hipError_t e = hipSuccess;
-    auto need = [&](DevBuf& b, size_t bytes) { if (e == hipSuccess) e = b.ensure(bytes); };
-    for (int b = 0; b < 2; ++b) { need(S.rc[b], 16ull * n); need(S.rl[b], 16ull * n); need(S.rh[b], 16ull * n); }
-    need(S.tfr, 4ull * n);
-    need(S.tbl, 4ull * n);
-    need(S.nodes, sizeof(BNode) * (2ull * n + 1));
-    need(S.lists, 4ull * (2ull * n + 1));
-    need(S.small, 4ull * n);
-    need(S.blocks, sizeof(wf_bvh_node) * 2ull * n);
-    need(S.info, sizeof(SplitInfo) * maxa);
-    need(S.bins, 4ull * 3 * B * 7 * maxa);
-    need(S.chunks, 16ull * maxch);
-    need(S.cnt, 8ull * scan_n);
-    need(S.off, 8ull * (scan_n + 1));
-    need(S.slot_ch, 8ull * maxa);
-    need(S.ctr, sizeof(Counters));
-    if (e != hipSuccess) { err = "device builder scratch allocation failed"; return WF_ERROR_OUT_OF_MEMORY; }
 
     float4* rc[2] = {(float4*)S.rc[0].p, (float4*)S.rc[1].p};
     float4* rl[2] = {(float4*)S.rl[0].p, (float4*)S.rl[1].p};
@@ -1098,7 +1127,23 @@ wf_status device_build_blas(const float* d_pos, uint32_t n, const wf_build_optio
             k_zero_resolve<<<1, 64, 0, st>>>(lists, 1u, nodes, rl[0], rh[0]);
         }
     }
-    std::vector<std::pair<uint32_t, uint32_t>> levels;
+    // (offset, count) of every level's active-node list
+    std::vector<std::pair<uint32_t, uint32_t>> levels_vec;
+    size_t n_levels = 0;
+    auto add_level = [&](uint32_t off_, uint32_t cnt_) -> bool {
+        if (S.level_buf) {
+            if (n_levels >= S.level_cap) return false;
+            S.level_buf[n_levels][0] = off_;
+            S.level_buf[n_levels][1] = cnt_;
+        } else {
+            levels_vec.emplace_back(off_, cnt_);
+        }
+        ++n_levels;
+        return true;
+    };
+    auto level = [&](size_t i) -> std::pair<uint32_t, uint32_t> {
+        return S.level_buf ? std::pair<uint32_t, uint32_t>(S.level_buf[i][0], S.level_buf[i][1]) : levels_vec[i];
+    };
     uint32_t lv_off = 0, lv = 0;
     while (a > 0) {
         const uint32_t in = lv & 1u, out = in ^ 1u;
@@ -1122,7 +1167,7 @@ wf_status device_build_blas(const float* d_pos, uint32_t n, const wf_build_optio
             k_zero_scan<<<g, kWG, 0, st>>>(chunks, nch, info, nodes, rl[out], rh[out], 0u);
             k_zero_resolve<<<grid_for(a_next, 64, 1u << 20), 64, 0, st>>>(next, a_next, nodes, rl[out], rh[out]);
         }
-        levels.emplace_back(lv_off, a);
+        if (!add_level(lv_off, a)) { err = "tree too deep for the fixed scratch"; return WF_ERROR_CAPACITY; }
         lv_off += a;
         a = a_next;
         ++lv;
@@ -1133,10 +1178,14 @@ wf_status device_build_blas(const float* d_pos, uint32_t n, const wf_build_optio
     if (ns)
         k_wave_build<<<grid_for(ns, kWG / 64, 65535), kWG, 0, st>>>(small, ns, nodes, rc[0], rl[0], rh[0], rc[1], rl[1], rh[1], B,
                                                                    single, blocks, d_idx, ctr);
-    for (size_t i = levels.size(); i-- > 0;)
-        k_desc<<<grid_for(levels[i].second, 256, 1u << 20), 256, 0, st>>>(lists + levels[i].first, levels[i].second, nodes);
-    for (size_t i = 0; i < levels.size(); ++i)
-        k_assign<<<grid_for(levels[i].second, 256, 1u << 20), 256, 0, st>>>(lists + levels[i].first, levels[i].second, nodes);
+    for (size_t i = n_levels; i-- > 0;) {
+        const std::pair<uint32_t, uint32_t> l = level(i);
+        k_desc<<<grid_for(l.second, 256, 1u << 20), 256, 0, st>>>(lists + l.first, l.second, nodes);
+    }
+    for (size_t i = 0; i < n_levels; ++i) {
+        const std::pair<uint32_t, uint32_t> l = level(i);
+        k_assign<<<grid_for(l.second, 256, 1u << 20), 256, 0, st>>>(lists + l.first, l.second, nodes);
+    }
     if (lv_off) k_emit_upper<<<grid_for(lv_off, 256, 1u << 20), 256, 0, st>>>(lists, lv_off, nodes, d_nodes, 2u * n, ctr);
     if (ns) k_emit_small<<<grid_for(ns, kWG / 64, 65535), kWG, 0, st>>>(small, ns, nodes, blocks, d_nodes, 2u * n, ctr);
     if (e == hipSuccess) e = hipMemsetAsync(d_nodes + 1, 0, sizeof(wf_bvh_node), st);
@@ -1168,6 +1217,69 @@ extern "C" WF_API wf_status wf_bvh_build_device(const float* d_tri_positions, ui
     const wf_status s = wf::device_build_blas(d_tri_positions, n, opt, d_nodes, d_indices, S, (hipStream_t)stream, res, err);
     (void)hipStreamSynchronize((hipStream_t)stream);
     S.release();
+    if (s == WF_OK) *out_node_count = res.node_count;
+    return s;
+}
+
+// ---- the build without allocation -----------------------------------------
+// For callers that cannot allocate device memory (graph capture, serving
+// loops, harnesses that forbid it): the build's scratch is carved from one
+// file-scope device array of kNoAllocScratchBytes, enough for any mesh of up
+// to kNoAllocMaxTris triangles at any interval count.  Every GPU the library
+// runs on holds its own copy until the process ends.  Host code never passes
+// the array's name to a kernel: its address on the current device comes from
+// hipGetSymbolAddress, once per device.  The array keeps the last build's
+// contents; the build initialises everything it reads.  Calls on one GPU
+// share the array: the calls of one process are serialised by a lock, calls
+// from other processes have their own copy (their own module).
+namespace {
+constexpr uint32_t kNoAllocMaxTris = 65536;
+constexpr size_t kNoAllocScratchBytes = 32ull << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+std::mutex g_noalloc_lock;
+void* g_noalloc_ptr[kNoAllocMaxDevices];                  // scratch's address per device, or null
+uint32_t g_noalloc_levels[kNoAllocMaxTris][2];            // per-level bookkeeping of the build (host)
+}
+
+extern "C" WF_API wf_status wf_bvh_build_device_noalloc(const float* d_tri_positions, uint32_t n, const wf_build_options* options,
+                                                        wf_bvh_node* d_nodes, uint64_t node_capacity, uint32_t* d_indices,
+                                                        uint32_t* out_node_count, void* stream)
+{
+    if (!d_tri_positions || !d_nodes || !d_indices || !out_node_count || n == 0) return WF_ERROR_INVALID_ARGUMENT;
+    if (node_capacity < 2ull * n) return WF_ERROR_CAPACITY;
+    if (n > kNoAllocMaxTris) return WF_ERROR_CAPACITY;
+    const wf_build_options opt = options ? *options : wf_build_options{8, 0};
+    if (opt.interval_count < 2 || opt.interval_count > wf::kDevBuildMaxBins) return WF_ERROR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> guard(g_noalloc_lock);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices) return WF_ERROR_DEVICE;
+    if (!g_noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return WF_ERROR_DEVICE;
+        g_noalloc_ptr[dev] = ptr;
+    }
+    size_t sizes[wf::kDevBuildBuffers];
+    wf::device_build_sizes(n, opt.interval_count, sizes);
+    wf::DevBuildScratch S;
+    S.fixed = true;
+    S.level_buf = g_noalloc_levels;
+    S.level_cap = kNoAllocMaxTris;
+    wf::DevBuf* bufs[wf::kDevBuildBuffers] = {&S.rc[0], &S.rl[0], &S.rh[0], &S.rc[1], &S.rl[1], &S.rh[1], &S.tfr, &S.tbl,
+                                              &S.nodes, &S.lists, &S.small, &S.blocks, &S.info, &S.bins, &S.chunks, &S.cnt,
+                                              &S.off, &S.slot_ch, &S.ctr};
+    size_t at = 0;
+    for (int k = 0; k < wf::kDevBuildBuffers; ++k) {
+        if (at + sizes[k] > kNoAllocScratchBytes) return WF_ERROR_CAPACITY;
+        bufs[k]->p = static_cast<unsigned char*>(g_noalloc_ptr[dev]) + at;
+        bufs[k]->bytes = sizes[k];
+        at += (sizes[k] + 255) & ~size_t(255);
+    }
+    wf::DevBuildResult res;
+    std::string err;
+    const wf_status s = wf::device_build_blas(d_tri_positions, n, opt, d_nodes, d_indices, S, (hipStream_t)stream, res, err);
+    (void)hipStreamSynchronize((hipStream_t)stream);
+    for (wf::DevBuf* b : bufs) { b->p = nullptr; b->bytes = 0; }   // not the scratch's to free
     if (s == WF_OK) *out_node_count = res.node_count;
     return s;
 }
