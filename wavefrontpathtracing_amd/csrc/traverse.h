@@ -156,6 +156,11 @@ struct LdsStack {
         else gstu(spill + (size_t)k * spill_stride, e);
         return true;
     }
+    static constexpr bool kLds = true;
+    static constexpr uint32_t kLdsDepth = LDS;
+    // the entry at slot sp as an LDS read at the clamped level (meaningful
+    // for sp < LDS), for a pop whose slot is known before it is taken
+    __device__ __forceinline__ uint32_t peek(uint32_t sp) const { return ent[(sp < (uint32_t)LDS ? sp : (uint32_t)LDS - 1u) * BLOCK]; }
     __device__ __forceinline__ void get(uint32_t sp, uint32_t& e, float& d) const
     {
         const uint32_t i = sp < (uint32_t)LDS ? sp : (uint32_t)LDS - 1u;
@@ -171,6 +176,8 @@ struct LdsStack {
 
 template <int N>
 struct PrivStack {
+    static constexpr bool kLds = false;
+    static constexpr uint32_t kLdsDepth = 0;
     uint32_t ent[N];
     float dist[N];
     uint32_t cap;
@@ -181,6 +188,7 @@ struct PrivStack {
         return true;
     }
     __device__ __forceinline__ void get(uint32_t sp, uint32_t& e, float& d) const { e = ent[sp]; d = dist[sp]; }
+    __device__ __forceinline__ uint32_t peek(uint32_t) const { return 0u; }
 };
 
 enum LaneStatus { kRun = 0, kDone = 1, kOverflow = 2 };
@@ -247,8 +255,10 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
 {
     static_assert(!L4 || (!COUNT && !TLDS), "the 4-wide layout has no counting or LDS-TLAS variant");
     int status = kRun;
-    // pop until the next node to process is known (or the stack is empty)
-    auto pop_next = [&]() {
+    // pop until the next node to process is known (or the stack is empty);
+    // `pre` is the entry at slot sp-1 when the caller read it ahead (LDS
+    // stack, no popped-interior skip, that slot in LDS)
+    auto pop_next = [&](bool have_pre = false, uint32_t pre = 0u) {
         for (;;) {
             if (s.in_blas && s.sp == s.base_sp) {
                 s.in_blas = false;
@@ -258,7 +268,8 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
             if (s.sp == 0) { status = kDone; return; }
             uint32_t e; float dd;
             --s.sp;
-            st.get(s.sp, e, dd);
+            if (have_pre) { e = pre; dd = 0.0f; have_pre = false; }
+            else st.get(s.sp, e, dd);
             if (e & 0x80000000u) {
                 const uint32_t k4 = (e >> 27) & 15u;
                 if (k4) { s.kind = k4; s.ref = e & 0x07FFFFFFu; }
@@ -315,6 +326,9 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
         }
         // interior node: near child next, far child pushed (swap only when
         // dist_left > dist_right)
+        // the entry a pop would take, read before the node loads return
+        constexpr bool kPre = !SKIP && Stack::kLds;
+        const uint32_t pre = kPre ? st.peek(s.sp - 1u) : 0u;
         float4 a, b, c, d;
         if (TLDS && !s.in_blas) {
             const float4* q = s_tl + 2u * s.ref;
@@ -332,7 +346,7 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
         const uint32_t f_ref = __float_as_uint(sw ? a.w : c.w), f_kind = __float_as_uint(sw ? b.w : d.w);
         const uint32_t f_id = s.ref + (sw ? 0u : 1u);
         if (dn == kRayMaxT) {
-            pop_next();
+            pop_next(kPre && s.sp - 1u < Stack::kLdsDepth, pre);
         } else {
             s.ref = n_ref; s.kind = n_kind;
             if (df != kRayMaxT) {
