@@ -233,7 +233,7 @@ WF_API wf_status wf_renderer_set_tlas_builder(wf_renderer* r, uint32_t builder);
 WF_API wf_status wf_renderer_set_tlas_metric(wf_renderer* r, uint32_t metric);
 
 /* Traversal kernel configuration (no reference counterpart: the autotuner's
- * axes).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
+ * axes; defaults 128, 12, 0, BVH2, no cap, 16, 0, 4, 0).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
  * scratch), 4, 8, 12 or 16 (LDS, spilling to global memory past that depth);
  * skip_popped_interior drops popped interior nodes whose entry distance is
  * not below the current hit distance; bvh_layout 0 = BVH2; stack_limit caps

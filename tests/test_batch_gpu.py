@@ -28,10 +28,12 @@ def batch():
     r.exit()
 
 
-@pytest.fixture(params=[0, 1], ids=["claims_by_capacity", "big_claims"])
+@pytest.fixture(params=[(0, 1), (1, 1), (0, 0), (1, 0)],
+                ids=["claims_by_capacity", "big_claims", "claims_by_capacity_no_skip", "big_claims_no_skip"])
 def claims(request, batch):
     _, r = batch
-    tuning.set_extend_config(r, big_claims=request.param)
+    big, skip = request.param
+    tuning.set_extend_config(r, big_claims=big, skip_popped=skip)
     yield request.param
     tuning.set_extend_config(r)
 

@@ -189,14 +189,18 @@ def test_quad_layout_frames_equal_pair_layout(wavefront):
 
 
 @pytest.mark.parametrize("batch", [False, True])
-def test_fused_connect_equals_separate_connect(batch):
+@pytest.mark.parametrize("skip", [1, 0], ids=["skip_popped", "no_skip"])
+def test_fused_connect_equals_separate_connect(batch, skip):
     """Shadow rays traced in the next bounce's extend launch (the default)
     give the image of one launch per stage, bit for bit, with the same ray
-    counts; for one scene and for a batch."""
+    counts; for one scene and for a batch, with and without the
+    popped-interior skip (each has its own fused kernels)."""
+    from wavefrontpathtracing_amd import tuning
     out = []
     for sep in (1, 0):
         r = Renderer(64, 64)
         try:
+            tuning.set_extend_config(r, block=128, lds_stack=12, skip_popped=skip)
             if batch:
                 sl = scenes.batch_scenes(seed=35, n_scenes=4, tri_min=500, tri_max=20_000, resolutions=(24, 40))
                 scenes.load_batch(r, sl)

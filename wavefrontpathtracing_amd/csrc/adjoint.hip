@@ -47,7 +47,8 @@ static wf_status adjoint_run(wf_renderer* r, const float* d_dl, int32_t tex_slot
     // as in the forward frame, bounce d-1's shadow rays may ride in bounce d's
     // extend launch: adjoint_connect(d-1) then runs before adjoint_shade(d),
     // the order in which the separate launches update the remaining estimate
-    const bool fuse = A.nee && !r->settings.separate_connect && r->ext_cfg.skip_popped && r->ext_grid_tlds == 0;
+    const bool fuse = A.nee && !r->settings.separate_connect && (r->ext_cfg.skip_popped || r->ext_cfg.layout == 0) &&
+                      r->ext_grid_tlds == 0;
     for (uint32_t d = 0; d <= A.max_bounces; ++d) {
         const bool with_shadow = fuse && d > 0;
         wf_renderer::OcclusionQueue q{A.shadow_rays, A.counts + kShadowCountBase + (d > 0 ? d - 1 : 0), A.shadow_visible,

@@ -294,8 +294,10 @@ wf_status wf_renderer::record_frame(const FrameArgs& A, uint32_t grid, bool time
     // sees the same operations in the same order as with separate launches.
     // separate_connect keeps the reference's one launch per stage.
     const bool batched = slots.size() > 1;
-    // (the fused kernels exist for the popped-interior skip with the TLAS in global memory)
-    const bool fuse = settings.next_event_estimation && !settings.separate_connect && ext_cfg.skip_popped && ext_grid_tlds == 0;
+    // (the fused kernels exist with the TLAS in global memory, and for the
+    // 4-wide layout with the popped-interior skip)
+    const bool fuse = settings.next_event_estimation && !settings.separate_connect && (ext_cfg.skip_popped || ext_cfg.layout == 0) &&
+                      ext_grid_tlds == 0;
     for (uint32_t d = 0; d <= settings.max_bounces; ++d) {
         const bool with_shadow = fuse && d > 0;
         OcclusionQueue q{A.shadow_rays, A.counts + kShadowCountBase + (d > 0 ? d - 1 : 0), A.shadow_visible,

@@ -229,14 +229,14 @@ void* pick_quad(uint32_t block, uint32_t lds)
     return nullptr;
 }
 
-// closest-hit launches with a trailing occlusion queue: popped-interior skip
-// on, no counters, TLAS in global memory, either layout; BIG claims for the
-// 2-wide layout only
-template <bool L4, bool BIG>
+// closest-hit launches with a trailing occlusion queue: no counters, TLAS in
+// global memory; the 4-wide layout with the popped-interior skip only, the
+// 2-wide layout with or without it and with or without BIG claims
+template <bool SKIP, bool L4, bool BIG>
 void* pick_mix(uint32_t block, uint32_t lds)
 {
 #define WF_CASE(B, S) \
-    if (block == B && lds == S) return reinterpret_cast<void*>(&k_extend<B, S, true, false, false, false, L4, true, BIG>);
+    if (block == B && lds == S) return reinterpret_cast<void*>(&k_extend<B, S, SKIP, false, false, false, L4, true, BIG>);
     WF_CASE(64, 0) WF_CASE(64, 4) WF_CASE(64, 8) WF_CASE(64, 12) WF_CASE(64, 16)
     WF_CASE(128, 0) WF_CASE(128, 4) WF_CASE(128, 8) WF_CASE(128, 12) WF_CASE(128, 16)
     WF_CASE(256, 0) WF_CASE(256, 4) WF_CASE(256, 8) WF_CASE(256, 12) WF_CASE(256, 16)
@@ -247,9 +247,11 @@ void* pick_mix(uint32_t block, uint32_t lds)
 void* pick_kernel(const ExtendConfig& c, bool count, bool any, bool tlds, bool mix = false, bool big = false)
 {
     if (mix) {
-        if (count || any || tlds || !c.skip_popped) return nullptr;
-        if (c.layout == 1) return pick_mix<true, false>(c.block, c.lds_stack);
-        return big ? pick_mix<false, true>(c.block, c.lds_stack) : pick_mix<false, false>(c.block, c.lds_stack);
+        if (count || any || tlds) return nullptr;
+        if (c.layout == 1) return c.skip_popped ? pick_mix<true, true, false>(c.block, c.lds_stack) : nullptr;
+        if (c.skip_popped)
+            return big ? pick_mix<true, false, true>(c.block, c.lds_stack) : pick_mix<true, false, false>(c.block, c.lds_stack);
+        return big ? pick_mix<false, false, true>(c.block, c.lds_stack) : pick_mix<false, false, false>(c.block, c.lds_stack);
     }
     if (c.layout == 1) {
         if (count || tlds || !c.skip_popped) return nullptr;

@@ -99,9 +99,10 @@ constexpr uint32_t kTlasLdsPairs = 1024;
 
 // Traversal configuration (the autotuner's axes).
 struct ExtendConfig {
-    uint32_t block = 64;        // workgroup size: 64, 128, 256
-    uint32_t lds_stack = 16;    // LDS stack entries per lane (0 = private/scratch stack)
-    uint32_t skip_popped = 1;   // skip popped interior nodes whose entry distance >= t
+    // defaults: the autotuner's choice on the incoherent 256-instance workload
+    uint32_t block = 128;       // workgroup size: 64, 128, 256
+    uint32_t lds_stack = 12;    // LDS stack entries per lane (0 = private/scratch stack)
+    uint32_t skip_popped = 0;   // skip popped interior nodes whose entry distance >= t
     uint32_t stack_limit = 0;   // cap on stack entries per lane (0 = capacity)
     uint32_t refill_min = 16;   // idle lanes of a wave that trigger a refill
     uint32_t tlas_lds = 0;      // copy the TLAS records to LDS when they fit
