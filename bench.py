@@ -65,6 +65,13 @@ def measured_profile(cfg: dict):
     return vals
 
 
+def parse_traversal(text: str):
+    """'block=64,lds_stack=16' -> {'block': 64, 'lds_stack': 16}; '' -> None."""
+    if not text:
+        return None
+    return {k.strip(): int(v) for k, v in (kv.split("=") for kv in text.split(","))}
+
+
 def host_threads() -> int:
     """Every CPU thread this process may use: the affinity mask, capped by
     OMP_NUM_THREADS when the environment grants fewer (a shared host's CPU
@@ -414,12 +421,13 @@ def batch_candidates(headline):
     return out
 
 
-def bench_batched(frames: int, tune: bool, headline=None):
+def bench_batched(frames: int, tune: bool, headline=None, fixed=None):
     """64 uneven scenes (soup of 10K..2M triangles in a lit box room, 256^2 /
     512^2 / 1024^2 pixels), 4 spp, depth 4, next-event estimation: the whole
     batch as one frame of one launch per stage (captured hipGraph) against
     the plain loop of one frame per scene.  Both produce the same images
-    (the batch path numbering equals the per-scene one)."""
+    (the batch path numbering equals the per-scene one).  `fixed` (a
+    traversal configuration) replaces the tuner for both."""
     from wavefrontpathtracing_amd import Renderer, scenes, tuning
     t0 = time.perf_counter()
     sl = scenes.batch_scenes()
@@ -443,6 +451,9 @@ def bench_batched(frames: int, tune: bool, headline=None):
             r.render_scenes(k, 1)
         return time.perf_counter() - t
 
+    if fixed:
+        tuning.set_extend_config(r, **fixed)
+        tune = False
     tuned = None
     if tune:
         t_tune = time.perf_counter()
@@ -512,7 +523,7 @@ def bench_batched(frames: int, tune: bool, headline=None):
             "rays_per_frame": brays // frames, "batched_ms_per_frame": round(bt / frames * 1e3, 2),
             "plain_ms_per_frame": round(pt / frames * 1e3, 2), "setup_s": round(setup_s, 1), "frames": frames,
             "images_identical": True,
-            "autotuned": tuned, "autotuned_plain_loop": tuned_plain}
+            "autotuned": tuned, "autotuned_plain_loop": tuned_plain, "fixed_traversal": fixed}
 
 
 def bench_adjoint(frames: int):
@@ -590,6 +601,8 @@ def main():
                     help="skip the autotuner: run the default traversal configuration (or --traversal)")
     ap.add_argument("--traversal", default="",
                     help="fixed traversal configuration, e.g. block=64,lds_stack=16,interior_visits=4 (implies --no-tune)")
+    ap.add_argument("--batch-traversal", default="",
+                    help="fixed traversal configuration of the batched section alone (its tuner then does not run)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -634,7 +647,7 @@ def main():
 
     autotuned = None
     if args.traversal:
-        tuning.set_extend_config(r, **{k: int(v) for k, v in (kv.split("=") for kv in args.traversal.split(","))})
+        tuning.set_extend_config(r, **parse_traversal(args.traversal))
     elif not args.no_tune:
         # the autotuner over every workgroup size x stack placement x
         # popped-interior skip x node layout, at 2 and 4 interior visits per
@@ -758,8 +771,14 @@ def main():
                 "hbm_gbs": round(traffic / (ms_step * 1e-3) / 1e9, 1),
                 "hbm_frac": round(traffic / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 **{k: prof[k] for k in PROFILE_KEYS[1:] if k in prof}, "source": prof["source"]},
-            "limiter": "instruction issue and SIMT divergence of the traversal loop, not HBM: measured HBM traffic is a "
-                       "fraction of the algorithmic bytes (node fetches hit L2 / the Infinity Cache); see 'measured'",
+            "limiter": "issue+divergence",
+            "limiter_evidence": None if prof is None else (
+                f"measured HBM traffic {traffic / (ms_step * 1e-3) / 1e9:,.0f} GB/s "
+                f"({traffic / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS:.3f} of peak; node fetches hit L2 at "
+                f"{prof.get('l2_hit', float('nan')):.2f} and the Infinity Cache); VALU busy "
+                f"{prof.get('valu_busy', float('nan')):.2f}; lane utilisation {prof.get('lane_utilisation', float('nan')):.2f} "
+                f"({64 * prof.get('lane_utilisation', float('nan')):.1f} of 64 lanes active per VALU instruction: SIMT "
+                f"divergence and the tail of the longest rays)"),
         },
     }
     if not args.no_cpu:
@@ -774,7 +793,8 @@ def main():
         out["bvh_build"] = bench_build(cpu=not args.no_cpu)
     if not args.no_batch:
         out["batched"] = bench_batched(frames=2, tune=not args.no_tune and not args.traversal,
-                                       headline=None if autotuned is None else table)
+                                       headline=None if autotuned is None else table,
+                                       fixed=parse_traversal(args.batch_traversal))
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(frames=3)
     if not args.no_inverse:

@@ -19,5 +19,5 @@ def test_help_lists_the_options():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=300)
     assert p.returncode == 0
-    for opt in ("--gpus", "--steps", "--warmup", "--no-union", "--no-bounce1", "--build-only", "--no-tune", "--traversal"):
+    for opt in ("--gpus", "--steps", "--warmup", "--no-union", "--no-bounce1", "--build-only", "--no-tune", "--traversal", "--batch-traversal"):
         assert opt in p.stdout
