@@ -271,7 +271,8 @@ WF_API wf_status wf_make_view(const wf_camera* camera, uint32_t width, uint32_t 
 WF_API wf_status wf_renderer_read_energy(wf_renderer* r, float* out_rgba);
 /* progressive accumulator (post_process.hlsl:40-51): width*height RGBA float */
 WF_API wf_status wf_renderer_read_accumulator(wf_renderer* r, float* out_rgba);
-/* tone-mapped sRGB output (post_process.hlsl:53-71): width*height RGBA8 */
+/* display output (post_process.hlsl:53-71): width*height RGBA8; Reinhard
+ * for view mode 0 only, the sRGB curve for modes 0, 6 and 9 only */
 WF_API wf_status wf_renderer_read_final_color(wf_renderer* r, uint8_t* out_rgba8);
 WF_API wf_status wf_renderer_get_frame_stats(const wf_renderer* r, wf_frame_stats* out);
 /* image files (host only; no reference counterpart, which presents to a
