@@ -140,6 +140,62 @@ def test_single_triangle_single_instance():
         r.exit()
 
 
+@pytest.mark.parametrize("builder", ["host", "device"])
+def test_large_leaves(builder):
+    """Leaves of 16 or more triangles: clusters of identical triangles, which
+    no split can separate, sit in one leaf each (sizes 5, 15, 16, 17, 40 and
+    300); such a leaf's reference carries its first triangle only and its
+    count comes from the triangle array.  Every builder path and traversal
+    configuration must give the oracle's hits."""
+    rng = np.random.default_rng(5)
+    tris = []
+    for k, size in enumerate((5, 15, 16, 17, 40, 300)):
+        c = np.array([k * 0.7 - 2.0, 0.1 * k, 0.0])
+        base = c + rng.uniform(-0.2, 0.2, (3, 3))
+        if np.cross(base[1] - base[0], base[2] - base[0])[2] > 0:   # face the rays below, which travel along +z
+            base = base[[0, 2, 1]]
+        tris += [base] * size
+    # a soup behind the clusters (as seen from z < 0)
+    tris += list(rng.uniform((-2.5, -2.5, 0.5), (2.5, 2.5, 3.0), (20000, 1, 3)) + rng.uniform(-0.05, 0.05, (20000, 3, 3)))
+    pos = np.concatenate(tris).astype(np.float32).reshape(-1, 3)
+    v = np.zeros(len(pos), dtype=L.VERTEX)
+    v["position"] = pos
+    sc = scenes.Scene(meshes=[(v, np.arange(len(pos), dtype=np.uint32))],
+                      instances=[scenes.Instance(0, scenes.identity()),
+                                 scenes.Instance(0, scenes.translate_scale((0.3, 0.2, 1.0), (1.5, 1.5, 1.5)))],
+                      eye=(0, 0, -6), target=(0, 0, 0), vfov_deg=60.0, width=64, height=64)
+    r = Renderer(sc.width, sc.height)
+    try:
+        r.set_blas_builder(builder)
+        scenes.load(r, sc)
+        r.build_tlas()
+        orc = O.OracleScene(sc)
+        leaf_sizes = orc.nodes["prim_count"][orc.nodes["prim_count"] > 0]
+        assert leaf_sizes.max() >= 300 and (leaf_sizes >= 16).sum() >= 4
+        rays = scenes.sphere_rays(23, (-3, -3, -3), (3, 3, 3), 20000)
+        # rays aimed through the clusters, so the large leaves are tested
+        aim = np.zeros(6 * 200, dtype=L.RAY)
+        for k in range(6):
+            cen = np.array([k * 0.7 - 2.0, 0.1 * k, 0.0])
+            o = cen + np.array([0.0, 0.0, -4.0]) + rng.uniform(-0.3, 0.3, (200, 3))
+            d = cen + rng.uniform(-0.2, 0.2, (200, 3)) - o
+            aim["origin"][k * 200:(k + 1) * 200] = o
+            aim["direction"][k * 200:(k + 1) * 200] = d / np.linalg.norm(d, axis=1, keepdims=True)
+        aim["tmax"] = 3.4028235e38
+        rays = np.concatenate([rays, aim])
+        ref = orc.trace_closest(rays, nthreads=8)
+        assert (ref["primitive_idx"][-1200:] < 393).sum() > 100   # the aimed rays hit the clusters
+        from wavefrontpathtracing_amd import tuning
+        for cfg in (dict(), dict(lds_stack=0), dict(skip_popped=1), dict(bvh_layout=1, skip_popped=1, lds_stack=16)):
+            tuning.set_extend_config(r, **cfg)
+            _assert_bitexact(r.trace_closest(rays), ref, f"large leaves, {builder} builder, {cfg}")
+        tuning.set_extend_config(r)
+        vis = r.trace_occluded(rays)
+        assert np.array_equal(vis == 0, ref["instance_idx"] != L.MISS)
+    finally:
+        r.exit()
+
+
 @pytest.mark.parametrize("lds", [16, 0])
 def test_stack_overflow_reports_status(small, lds):
     """Binned SAH keeps real trees shallow (< 50 levels), so the overflow path

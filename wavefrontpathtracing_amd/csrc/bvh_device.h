@@ -54,6 +54,11 @@ wf_status device_build_tlas(const wf_bvh_instance* d_inst, uint32_t m, wf_tlas_n
                             uint32_t metric = WF_TLAS_METRIC_REFERENCE);
 
 // The 4-wide records (kQuadBytes each) of a BLAS node array, node_count / 2 of them.
+// Leaves' left_first words become traversal references, in place (the
+// renderer's own node copies only); *d_bad is set when a leaf's triangle range
+// is out of bounds.
+hipError_t launch_encode_blas_refs(wf_bvh_node* d_nodes, uint32_t node_count, GpuTri* d_tris, uint32_t tri_count,
+                                   uint32_t* d_bad, hipStream_t stream);
 hipError_t launch_make_quads(const wf_bvh_node* d_nodes, uint32_t node_count, void* d_quads, hipStream_t stream);
 
 // GpuTri records in leaf order from the positions and the index array.

@@ -1368,7 +1368,34 @@ __global__ void k_make_quads(const wf_bvh_node* __restrict__ nodes, uint32_t nod
     q[5] = b.prim_count == 0 ? rw(nodes[b.left_first + 1u]) : z;
 }
 
+// The traversal's form of a BLAS node array, in place: every leaf's
+// left_first word becomes its reference word (0x80000000 | count << 27 |
+// first, or 0x80000000 | first with the count in tris[first].pad0 for a leaf
+// of 16 or more triangles); interior nodes keep left_first, which is already
+// their reference.  prim_count is unchanged, so the reference layout stays
+// recoverable (first = left_first & 0x07FFFFFF).
+__global__ void k_encode_blas_refs(wf_bvh_node* nodes, uint32_t node_count, GpuTri* tris, uint32_t tri_count,
+                                   uint32_t* bad)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= node_count) return;
+    const uint32_t c = nodes[i].prim_count;
+    if (c == 0u) return;
+    const uint32_t first = nodes[i].left_first;
+    if (first >= kMaxLeafRef || first >= tri_count || c > tri_count - first) { atomicOr(bad, 1u); return; }
+    nodes[i].left_first = encode_ref(first, c);
+    if (c >= 16u) tris[first].pad0 = c;
+}
+
 } // namespace
+
+hipError_t launch_encode_blas_refs(wf_bvh_node* d_nodes, uint32_t node_count, GpuTri* d_tris, uint32_t tri_count,
+                                   uint32_t* d_bad, hipStream_t stream)
+{
+    if (node_count == 0) return hipErrorInvalidValue;
+    k_encode_blas_refs<<<(node_count + 255) / 256, 256, 0, stream>>>(d_nodes, node_count, d_tris, tri_count, d_bad);
+    return hipGetLastError();
+}
 
 hipError_t launch_make_quads(const wf_bvh_node* d_nodes, uint32_t node_count, void* d_quads, hipStream_t stream)
 {

@@ -16,19 +16,22 @@
 //   * persistent waves pull 64-ray batches from a global atomic cursor, so
 //     the grid is sized to the machine and the queue length is read on the
 //     device (no indirect dispatch on HIP);
-//   * one combined TLAS+BLAS stack of 8-byte entries (node reference +
-//     entry distance) lives in LDS, strided by workgroup size so that lanes
-//     hit distinct banks, and spills to a per-thread global region past the
-//     LDS depth;
-//   * stack entries carry what the reference keeps in its whole-node stack
-//     (children offset, or a leaf's first triangle and count), so a pop
-//     issues no memory load;
-//   * a popped interior node whose entry distance is >= the current ray.t is
-//     dropped without fetching its children: its children's slab entries are
-//     >= its own (rounded subtraction and multiplication by the same inv_dir
-//     are monotone and child boxes nest inside parent boxes), so the
-//     reference would test both children, find both missing, and pop again;
-//     the traversal counters still count those two child fetches;
+//   * one combined TLAS+BLAS stack of 32-bit node references lives in LDS,
+//     strided by workgroup size so that lanes hit distinct banks, and spills
+//     to a per-thread global region past the LDS depth;
+//   * node references are stored pre-encoded in the node records (an
+//     interior child's pair, or a leaf's first triangle and count: what the
+//     reference keeps in its whole-node stack), so a visit writes its far
+//     child to the stack top unconditionally and a pop is a select of the
+//     entry read at the start of the visit, with no memory load and no
+//     divergent push or pop branch;
+//   * with the opt-in popped-interior skip, entries also carry their entry
+//     distance, and a popped interior node whose entry distance is >= the
+//     current ray.t is dropped without fetching its children: its children's
+//     slab entries are >= its own (rounded subtraction and multiplication by
+//     the same inv_dir are monotone and child boxes nest inside parent boxes),
+//     so the reference would test both children, find both missing, and pop
+//     again; the traversal counters still count those two child fetches;
 //   * triangles are pre-permuted into leaf order with edges precomputed
 //     (48-byte rows), instances are 80-byte records with 3x4 matrices;
 //   * an optional 4-wide node layout (L4) visits two levels per step in the
@@ -163,7 +166,7 @@ extern __shared__ float4 s_tlas_dyn[];
 template <int BLOCK, int LDS, bool SKIP, bool COUNT, bool ANY, bool TLDS, bool L4 = false, bool MIX = false, bool BIG = false>
 __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
 {
-    constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
+    constexpr int kLdsN = LDS > 0 ? (LDS + 2) * BLOCK : 1;   // LDS levels plus the floor and the scratch level
     __shared__ uint32_t s_ent[kLdsN];
     __shared__ uint16_t s_dist[SKIP ? kLdsN : 1];   // entry distances: only the popped-interior skip reads them
     float4* s_tl = s_tlas_dyn;

@@ -138,13 +138,13 @@ __device__ __forceinline__ void mega_loop(const FrameArgs& A, const MegaLaunch& 
 template <int BLOCK, int LDS, bool L4>
 __global__ __launch_bounds__(BLOCK, 4) void k_megakernel(FrameArgs A, MegaLaunch M)
 {
-    constexpr int kLdsN = LDS > 0 ? LDS * BLOCK : 1;
+    constexpr int kLdsN = LDS > 0 ? (LDS + 2) * BLOCK : 1;   // LDS levels plus the floor and the scratch level
     __shared__ uint32_t s_ent[kLdsN];
     __shared__ uint16_t s_dist[kLdsN];
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     if constexpr (LDS > 0) {
-        LdsStack<BLOCK, LDS> st{s_ent + tid, s_dist + tid, M.spill + gtid, M.spill_per_thread, gridDim.x * BLOCK, M.stack_cap};
+        LdsStack<BLOCK, LDS, true> st{s_ent + tid, s_dist + tid, M.spill + gtid, M.spill_per_thread, gridDim.x * BLOCK, M.stack_cap};
         mega_loop<BLOCK, LDS, L4>(A, M, st, tid & 63u);
     } else {
         PrivStack<128> st;

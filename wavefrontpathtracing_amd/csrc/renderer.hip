@@ -178,7 +178,7 @@ extern "C" WF_API wf_status wf_renderer_exit(wf_renderer* r)
     r->d_emitters.release(); r->d_scenes.release(); r->d_dyn.release();
     r->d_lrem.release(); r->d_delta.release(); r->d_shadow_unit.release(); r->d_shadow_light.release();
     r->d_adj_dl.release(); r->d_adj_tex.release(); r->d_adj_em.release();
-    r->d_work.release(); r->d_spill.release(); r->d_tmp_rays.release(); r->d_tmp_hits.release(); r->d_tmp_aux.release();
+    r->d_work.release(); r->d_check.release(); r->d_spill.release(); r->d_tmp_rays.release(); r->d_tmp_hits.release(); r->d_tmp_aux.release();
     for (int i = 0; i < kNumStageEvents; ++i) if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
@@ -421,6 +421,8 @@ extern "C" WF_API wf_status wf_renderer_create_render_mesh(wf_renderer* r, const
         m.tri_count = n;
         m.node_count = res.node_count;
         m.max_leaf = res.max_leaf;
+        const wf_status es = r->encode_mesh_refs(m);
+        if (es != WF_OK) { m.nodes.release(); m.tris.release(); m.shade.release(); return es; }
         std::memcpy(m.blas_min, res.bmin, 12);   // renderer.cpp:107-111
         std::memcpy(m.blas_max, res.bmax, 12);
         m.alive = true;
@@ -457,12 +459,30 @@ extern "C" WF_API wf_status wf_renderer_create_render_mesh(wf_renderer* r, const
     m.tri_count = n;
     m.node_count = b.node_count;
     m.max_leaf = b.max_leaf;
+    s = r->encode_mesh_refs(m);
+    if (s != WF_OK) { m.nodes.release(); m.tris.release(); m.shade.release(); return s; }
     std::memcpy(m.blas_min, b.nodes[0].aabb_min, 12);   // renderer.cpp:107-111
     std::memcpy(m.blas_max, b.nodes[0].aabb_max, 12);
     m.alive = true;
     m.version = 1;
     r->meshes.push_back(std::move(m));
     *out = make_handle((uint32_t)r->meshes.size() - 1, 1);
+    return WF_OK;
+}
+
+// The renderer's node copy in the traversal's form: every leaf's left_first
+// word becomes its reference (kernel k_encode_blas_refs), checked in range.
+wf_status wf_renderer::encode_mesh_refs(wf::Mesh& m)
+{
+    hipError_t e = d_check.ensure(4);
+    if (e == hipSuccess) e = launch_fill_u32((uint32_t*)d_check.p, 0u, 1, stream);
+    if (e == hipSuccess)
+        e = launch_encode_blas_refs((wf_bvh_node*)m.nodes.p, m.node_count, (GpuTri*)m.tris.p, m.tri_count, (uint32_t*)d_check.p, stream);
+    uint32_t bad = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_check.p, 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "BLAS reference encoding");
+    if (bad) return fail(WF_ERROR_INVALID_STATE, "a BLAS leaf's triangle range is out of bounds");
     return WF_OK;
 }
 
@@ -602,7 +622,7 @@ wf_status wf_renderer::upload_scene()
             const wf_tlas_node& cn = all_nodes[child[c]];
             float* slot = &pairs[16 * rec_of[k] + 8 * c];
             const bool leaf = cn.left_right == 0;
-            const uint32_t ref = leaf ? cn.instance_idx : 2u * rec_of[child[c]];
+            const uint32_t ref = leaf ? encode_ref(cn.instance_idx, 1u) : 2u * rec_of[child[c]];
             const uint32_t kind = leaf ? 1u : 0u;
             std::memcpy(slot, cn.aabb_min, 12);
             std::memcpy(slot + 3, &ref, 4);

@@ -110,6 +110,7 @@ struct wf_renderer {
     uint32_t n_tpairs = 0;
     uint32_t spill_per_thread = 0;
     wf::DevBuf d_work, d_spill, d_tmp_rays, d_tmp_hits, d_tmp_aux;
+    wf::DevBuf d_check;   // one word: the BLAS reference encoding's range check
 
     // wavefront buffers (allocated on first render)
     wf::DevBuf d_rays[2], d_paths[2], d_roots[2], d_hits, d_thr, d_path_energy, d_path_scene, d_counts;
@@ -121,6 +122,7 @@ struct wf_renderer {
 
     wf_status fail(wf_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
     wf_status hip_fail(hipError_t e, const char* what);
+    wf_status encode_mesh_refs(wf::Mesh& m);
     wf::Mesh* find_mesh(uint64_t h);
     wf::Texture* find_texture(uint64_t h);
     wf_status ensure_trace_scratch();

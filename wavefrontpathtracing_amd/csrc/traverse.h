@@ -105,19 +105,12 @@ __device__ __forceinline__ void gstu(void* p, uint32_t v) { *reinterpret_cast<ui
 __device__ __forceinline__ void gstu2(void* p, uint2 v) { *reinterpret_cast<uint2*>(p) = v; }
 #endif
 
-// ---- stacks -------------------------------------------------------------------
-// An entry is a child reference plus its slab entry distance.  Encodings:
-//   interior child:  ref                      (bit 31 clear; the child's pair record index)
-//   leaf child:      0x80000000 | kind << 27 | ref   (kind 1..15, ref < 2^27; TLAS leaves are kind 1)
-//                    0x80000000 | node index          (BLAS leaves with more triangles or a far
-//                                                      first triangle; the node is re-read)
-//
-// In LDS an entry takes 6 bytes: the reference plus its entry distance as a
-// half-precision value rounded toward -inf.  The distance only decides the
-// popped-interior skip (skip when distance >= t); a rounded-down value never
-// skips a node the exact value would keep, and a node it keeps anyway is
-// visited, its children both miss, and the traversal pops on exactly as the
-// reference does.  Spilled entries keep the full float.
+// In LDS the popped-interior skip also keeps each entry's distance as a
+// half-precision value rounded toward -inf: the distance only decides the
+// skip (skip when distance >= t); a rounded-down value never skips a node the
+// exact value would keep, and a node it keeps anyway is visited, its children
+// both miss, and the traversal pops on exactly as the reference does.  Spilled
+// entries keep the full float.
 __device__ __forceinline__ uint16_t half_down(float d)
 {
     const _Float16 h = (_Float16)d;                        // round to nearest even
@@ -127,50 +120,62 @@ __device__ __forceinline__ uint16_t half_down(float d)
 }
 __device__ __forceinline__ float half_up_to_float(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
 
-// Entries past the LDS depth spill to global memory, level-major: level k of
-// thread g sits at spill[k * spill_stride + g], so lanes of a wave at the same
-// depth touch adjacent 8-byte words.  DIST: the entry distance is kept (the
-// popped-interior skip reads it); without the skip only references are
-// stored.  A pop always reads LDS (at the clamped level) and reads the spill
-// area only past the LDS depth, so the two reads stay separate LDS and global
-// loads rather than one generic (flat) load through a selected pointer.
-template <int BLOCK, int LDS, bool DIST = true>
+// LDS stack of one thread: levels of BLOCK words (lanes of a wave hit
+// distinct banks).  Level 0 is a floor and level LDS + 1 a scratch level;
+// stack slot k (k < LDS) is level k + 1.  With sp entries held, level
+// min(sp, LDS) holds the top entry (slot sp - 1) when sp <= LDS, and the level
+// above it is where slot sp goes, or the scratch level once slots spill: a
+// visit reads the top and writes its far child there unconditionally, and
+// only a push advances sp.  Slots from LDS on spill to global memory,
+// level-major (slot LDS + k of thread g at spill[k * spill_stride + g], so a
+// wave's spills coalesce), up to `cap` entries.
+template <int BLOCK, int LDS, bool DIST = false>
 struct LdsStack {
-    uint32_t* ent;
-    uint16_t* dist;
-    uint2* spill;            // this thread's level-0 spill slot
+    uint32_t* ent;           // this thread's level 0
+    uint16_t* dist;          // entry distances (DIST), same levels
+    uint2* spill;            // this thread's first spill slot
     uint32_t spill_n;        // spill levels
     uint32_t spill_stride;   // threads of the grid
     uint32_t cap;
-    __device__ __forceinline__ bool push(uint32_t sp, uint32_t e, float d)
+    static constexpr bool kLds = true;
+    static constexpr uint32_t kLdsDepth = LDS;
+    __device__ __forceinline__ uint32_t level(uint32_t sp) const { return (sp < (uint32_t)LDS ? sp : (uint32_t)LDS) * BLOCK; }
+    // the top entry (valid for 0 < sp <= LDS)
+    __device__ __forceinline__ uint32_t top(uint32_t sp) const { return ent[level(sp)]; }
+    __device__ __forceinline__ float top_dist(uint32_t sp) const { return DIST ? half_up_to_float(dist[level(sp)]) : 0.0f; }
+    // the word at slot sp when sp < LDS, else the scratch level
+    __device__ __forceinline__ void put(uint32_t sp, uint32_t e, float d)
+    {
+        ent[level(sp) + BLOCK] = e;
+        if (DIST) dist[level(sp) + BLOCK] = half_down(d);
+    }
+    // a push at sp >= LDS: the spill area, or false when the stack is full
+    __device__ __forceinline__ bool push_spill(uint32_t sp, uint32_t e, float d)
     {
         if (sp >= cap) return false;
-        if (sp < (uint32_t)LDS) {
-            ent[sp * BLOCK] = e;
-            if (DIST) dist[sp * BLOCK] = half_down(d);
-            return true;
-        }
         const uint32_t k = sp - LDS;
         if (k >= spill_n) return false;
         if (DIST) gstu2(spill + (size_t)k * spill_stride, make_uint2(e, __float_as_uint(d)));
         else gstu(spill + (size_t)k * spill_stride, e);
         return true;
     }
-    static constexpr bool kLds = true;
-    static constexpr uint32_t kLdsDepth = LDS;
-    // the entry at slot sp as an LDS read at the clamped level (meaningful
-    // for sp < LDS), for a pop whose slot is known before it is taken
-    __device__ __forceinline__ uint32_t peek(uint32_t sp) const { return ent[(sp < (uint32_t)LDS ? sp : (uint32_t)LDS - 1u) * BLOCK]; }
+    __device__ __forceinline__ bool push(uint32_t sp, uint32_t e, float d)
+    {
+        if (sp >= cap) return false;
+        if (sp < (uint32_t)LDS) { put(sp, e, d); return true; }
+        return push_spill(sp, e, d);
+    }
+    // the entry at slot sp (any depth)
     __device__ __forceinline__ void get(uint32_t sp, uint32_t& e, float& d) const
     {
-        const uint32_t i = sp < (uint32_t)LDS ? sp : (uint32_t)LDS - 1u;
-        e = ent[i * BLOCK];
-        d = DIST ? half_up_to_float(dist[i * BLOCK]) : 0.0f;
-        if (sp >= (uint32_t)LDS) {
-            const uint2* q = spill + (size_t)(sp - LDS) * spill_stride;
-            if (DIST) { const uint2 v = gldu2(q); e = v.x; d = __uint_as_float(v.y); }
-            else e = gldu(q);
+        if (sp < (uint32_t)LDS) {
+            e = ent[(sp + 1u) * BLOCK];
+            d = DIST ? half_up_to_float(dist[(sp + 1u) * BLOCK]) : 0.0f;
+            return;
         }
+        const uint2* q = spill + (size_t)(sp - LDS) * spill_stride;
+        if (DIST) { const uint2 v = gldu2(q); e = v.x; d = __uint_as_float(v.y); }
+        else { e = gldu(q); d = 0.0f; }
     }
 };
 
@@ -188,29 +193,20 @@ struct PrivStack {
         return true;
     }
     __device__ __forceinline__ void get(uint32_t sp, uint32_t& e, float& d) const { e = ent[sp]; d = dist[sp]; }
-    __device__ __forceinline__ uint32_t peek(uint32_t) const { return 0u; }
 };
 
 enum LaneStatus { kRun = 0, kDone = 1, kOverflow = 2 };
 
-// Stack entry of a child (encodings above), as selects rather than branches.
-__device__ __forceinline__ uint32_t encode_entry(uint32_t ref, uint32_t kind, uint32_t id)
-{
-    const bool packs = kind < 16u && ref < (1u << 27);
-    const uint32_t leaf = 0x80000000u | (packs ? (kind << 27) | ref : id);
-    return kind == 0u ? ref : leaf;
-}
-
-// Per-lane traversal state.  `ref`/`kind` name the lane's node: kind 0 =
-// interior node whose child pair sits at nodes + ref * 32 bytes (`nodes` =
-// the TLAS pair array, or the BLAS node array, whose children are adjacent),
-// kind != 0 = leaf (a TLAS leaf's ref is the instance, a BLAS leaf's
-// ref/kind its first triangle/count).  The world ray and the current
-// (object-space inside a BLAS) ray share t, as in the reference.
+// Per-lane traversal state.  `cur` is the lane's node (encoding above):
+// interior nodes are visited from `nodes` (the TLAS pair array, or the BLAS
+// node array, whose children are adjacent; 4-wide records in layout 1), a
+// TLAS leaf is an instance to enter, a BLAS leaf a run of triangles.  The
+// world ray and the current (object-space inside a BLAS) ray share t, as in
+// the reference.
 struct TraceLane {
     Ray3 wr, cr;
     float t;
-    uint32_t ref, kind, sp, base_sp;
+    uint32_t cur, sp, base_sp;
     bool in_blas;
     const char* nodes;
     const GpuTri* tris;
@@ -237,104 +233,89 @@ __device__ __forceinline__ bool trace_begin(TraceLane& s, const TraceScene& sc, 
     const float4 rb = gld4(reinterpret_cast<const char*>(&sc.tnodes[root]) + 16);
     if (slab(ra.x, ra.y, ra.z, rb.x, rb.y, rb.z, s.wr, s.t) == kRayMaxT) return false;
     const uint32_t lr = __float_as_uint(ra.w);
-    // instance, or the root's record: 2 * record (pair records) / record (4-wide)
-    s.ref = lr == 0 ? __float_as_uint(rb.w) : (L4 ? 1u : 2u) * __float_as_uint(rb.w);
-    s.kind = lr == 0 ? 1u : 0u;
+    // the root's instance, or its record: 2 * record (pair records) / record (4-wide)
+    s.cur = lr == 0 ? encode_ref(__float_as_uint(rb.w), 1u) : (L4 ? 1u : 2u) * __float_as_uint(rb.w);
     return true;
 }
 
 // One traversal pass of a lane: up to `visits` interior visits (TLAS or
 // BLAS: both children tested, near child next, far child pushed; each with
-// its pops), then an instance entry or a triangle leaf if that is what the
-// lane now points at, then pops until its next node is known.  Returns kRun,
-// kDone (hit or miss found; any-hit rays stop at their first hit) or
-// kOverflow (the stack was full).
+// its pop), then an instance entry or a triangle leaf if that is what the
+// lane now points at, then a pop.  Returns kRun, kDone (hit or miss found;
+// any-hit rays stop at their first hit) or kOverflow (the stack was full).
 template <bool SKIP, bool COUNT, bool TLDS, bool L4 = false, class Stack>
 __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, Stack& st, const float4* s_tl,
                                           uint32_t visits, bool any, uint32_t* cnt)
 {
     static_assert(!L4 || (!COUNT && !TLDS), "the 4-wide layout has no counting or LDS-TLAS variant");
+    // the LDS stack without entry distances: a visit writes the far child at
+    // the top unconditionally and pops with the entry it read first
+    constexpr bool FAST = Stack::kLds && !SKIP;
+    constexpr uint32_t LDSD = Stack::kLdsDepth;
     int status = kRun;
-    // pop until the next node to process is known (or the stack is empty);
-    // `pre` is the entry at slot sp-1 when the caller read it ahead (LDS
-    // stack, no popped-interior skip, that slot in LDS)
-    auto pop_next = [&](bool have_pre = false, uint32_t pre = 0u) {
+    // pushes below this depth go to LDS with no further check
+    uint32_t fast_lim = 0u;
+    if constexpr (FAST) fast_lim = st.cap < LDSD ? st.cap : LDSD;
+    const char* tlas_nodes = reinterpret_cast<const char*>(L4 ? sc.tquads : sc.tpairs);
+    // the BLAS part of the stack is used up: back to the world ray and the TLAS
+    auto leave_blas = [&]() {
+        s.in_blas = false;
+        s.cr = s.wr;
+        s.nodes = tlas_nodes;
+    };
+    // pop until the next node is known (or the stack is empty)
+    auto pop = [&]() {
         for (;;) {
-            if (s.in_blas && s.sp == s.base_sp) {
-                s.in_blas = false;
-                s.cr = s.wr;
-                s.nodes = reinterpret_cast<const char*>(L4 ? sc.tquads : sc.tpairs);
-            }
+            if (s.in_blas && s.sp == s.base_sp) leave_blas();
             if (s.sp == 0) { status = kDone; return; }
             uint32_t e; float dd;
             --s.sp;
-            if (have_pre) { e = pre; dd = 0.0f; have_pre = false; }
-            else st.get(s.sp, e, dd);
-            if (e & 0x80000000u) {
-                const uint32_t k4 = (e >> 27) & 15u;
-                if (k4) { s.kind = k4; s.ref = e & 0x07FFFFFFu; }
-                else {
-                    const char* nd = s.nodes + (size_t)(e & 0x07FFFFFFu) * 32u;
-                    s.ref = gldu(nd + 12); s.kind = gldu(nd + 28);
-                }
-                return;
-            }
-            if (SKIP && dd >= s.t) { if (COUNT) cnt[s.in_blas ? 1 : 0] += 2; continue; }
-            s.ref = e; s.kind = 0;
+            st.get(s.sp, e, dd);
+            if (SKIP && !(e & kLeafBit) && dd >= s.t) { if (COUNT) cnt[s.in_blas ? 1 : 0] += 2; continue; }
+            s.cur = e;
             return;
         }
     };
-    // up to `visits` interior nodes per pass, so the rarer instance and leaf
-    // steps below are shared by more work
-    // push a far child (an interior child's reference; a leaf's ref/kind when
-    // they fit the entry, else the node's index to read them back at the pop)
-    auto push_far = [&](uint32_t f_ref, uint32_t f_kind, uint32_t f_id, float df) {
-        const uint32_t e = encode_entry(f_ref, f_kind, f_id);
-        if (st.push(s.sp, e, df)) ++s.sp;
+    auto push = [&](uint32_t e, float d) {
+        if (st.push(s.sp, e, d)) ++s.sp;
         else status = kOverflow;
     };
     for (uint32_t v = 0; v < visits; ++v) {
-        if (status != kRun || s.kind != 0) break;
+        if (status != kRun || (s.cur & kLeafBit)) break;
         if constexpr (L4) {
             // 4-wide record: this node's children, then (same record) the
             // near child's children when it is interior
-            const uint32_t rec = s.ref;
-            const char* rp = s.nodes + (size_t)rec * kQuadBytes;
+            const char* rp = s.nodes + (size_t)s.cur * kQuadBytes;
             const float4 a = gld4(rp), b = gld4(rp + 16), c = gld4(rp + 32), d = gld4(rp + 48);
             const float dl = slab(a.x, a.y, a.z, b.x, b.y, b.z, s.cr, s.t);
             const float dr = slab(c.x, c.y, c.z, d.x, d.y, d.z, s.cr, s.t);
             const bool sw = dl > dr;
             const float dn = sw ? dr : dl, df = sw ? dl : dr;
-            const uint32_t n_ref = __float_as_uint(sw ? c.w : a.w), n_kind = __float_as_uint(sw ? d.w : b.w);
-            if (dn == kRayMaxT) { pop_next(); continue; }
-            if (df != kRayMaxT)
-                push_far(__float_as_uint(sw ? a.w : c.w), __float_as_uint(sw ? b.w : d.w), rec * 6u + (sw ? 0u : 1u), df);
-            if (n_kind != 0 || status != kRun) { s.ref = n_ref; s.kind = n_kind; continue; }
+            const uint32_t en = __float_as_uint(sw ? c.w : a.w), ef = __float_as_uint(sw ? a.w : c.w);
+            if (dn == kRayMaxT) { pop(); continue; }
+            if (df != kRayMaxT) push(ef, df);
+            if ((en & kLeafBit) || status != kRun) { s.cur = en; continue; }
             const char* np = rp + (sw ? 128u : 64u);
             const float4 e = gld4(np), f = gld4(np + 16), g = gld4(np + 32), h = gld4(np + 48);
             const float dl2 = slab(e.x, e.y, e.z, f.x, f.y, f.z, s.cr, s.t);
             const float dr2 = slab(g.x, g.y, g.z, h.x, h.y, h.z, s.cr, s.t);
             const bool sw2 = dl2 > dr2;
             const float dn2 = sw2 ? dr2 : dl2, df2 = sw2 ? dl2 : dr2;
-            if (dn2 == kRayMaxT) { pop_next(); continue; }
-            s.ref = __float_as_uint(sw2 ? g.w : e.w);
-            s.kind = __float_as_uint(sw2 ? h.w : f.w);
-            if (df2 != kRayMaxT)
-                push_far(__float_as_uint(sw2 ? e.w : g.w), __float_as_uint(sw2 ? f.w : h.w),
-                         rec * 6u + (sw ? 4u : 2u) + (sw2 ? 0u : 1u), df2);
+            if (dn2 == kRayMaxT) { pop(); continue; }
+            s.cur = __float_as_uint(sw2 ? g.w : e.w);
+            if (df2 != kRayMaxT) push(__float_as_uint(sw2 ? e.w : g.w), df2);
             continue;
         }
         // interior node: near child next, far child pushed (swap only when
         // dist_left > dist_right)
-        // the entry a pop would take, read before the node loads return
-        constexpr bool kPre = !SKIP && Stack::kLds;
-        const uint32_t pre = kPre ? st.peek(s.sp - 1u) : 0u;
+        uint32_t pre = 0u;
+        if constexpr (FAST) pre = st.top(s.sp);   // read before the node loads return
         float4 a, b, c, d;
         if (TLDS && !s.in_blas) {
-            const float4* q = s_tl + 2u * s.ref;
+            const float4* q = s_tl + 2u * s.cur;
             a = q[0]; b = q[1]; c = q[2]; d = q[3];
         } else {
-            const char* cp = s.nodes + (size_t)s.ref * 32u;
+            const char* cp = s.nodes + (size_t)s.cur * 32u;
             a = gld4(cp); b = gld4(cp + 16); c = gld4(cp + 32); d = gld4(cp + 48);
         }
         if (COUNT) cnt[s.in_blas ? 1 : 0] += 2;
@@ -342,25 +323,45 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
         const float dr = slab(c.x, c.y, c.z, d.x, d.y, d.z, s.cr, s.t);
         const bool sw = dl > dr;
         const float dn = sw ? dr : dl, df = sw ? dl : dr;
-        const uint32_t n_ref = __float_as_uint(sw ? c.w : a.w), n_kind = __float_as_uint(sw ? d.w : b.w);
-        const uint32_t f_ref = __float_as_uint(sw ? a.w : c.w), f_kind = __float_as_uint(sw ? b.w : d.w);
-        const uint32_t f_id = s.ref + (sw ? 0u : 1u);
-        if (dn == kRayMaxT) {
-            pop_next(kPre && s.sp - 1u < Stack::kLdsDepth, pre);
+        const uint32_t en = __float_as_uint(sw ? c.w : a.w), ef = __float_as_uint(sw ? a.w : c.w);
+        if constexpr (FAST) {
+            // selects for the common cases; branches only for a push that
+            // spills, leaving a BLAS, and a pop from an empty or spilled top
+            st.put(s.sp, ef, df);
+            const uint32_t sp0 = s.sp;
+            const bool hit = dn != kRayMaxT, pushf = df != kRayMaxT;   // no hit: no push either
+            s.cur = hit ? en : pre;
+            s.sp = sp0 + (pushf ? 1u : 0u) - (hit ? 0u : 1u);
+            if (!hit && s.in_blas && sp0 == s.base_sp) leave_blas();
+            if (sp0 >= fast_lim || (!hit && sp0 == 0u)) {
+                // a push into the spill area or past the cap, a pop of a
+                // spilled entry, or the stack is empty
+                if (pushf) {
+                    if (st.push(sp0, ef, df)) s.sp = sp0 + 1u;
+                    else { s.sp = sp0; status = kOverflow; }
+                } else if (!hit && sp0 == 0u) {
+                    s.sp = 0u;
+                    status = kDone;
+                } else if (!hit && sp0 > LDSD) {
+                    float dd;
+                    st.get(sp0 - 1u, s.cur, dd);
+                }
+            }
         } else {
-            s.ref = n_ref; s.kind = n_kind;
-            if (df != kRayMaxT) {
-                const uint32_t e = encode_entry(f_ref, f_kind, f_id);
-                if (st.push(s.sp, e, df)) ++s.sp;
-                else status = kOverflow;
+            if (dn == kRayMaxT) {
+                pop();
+            } else {
+                s.cur = en;
+                if (df != kRayMaxT) push(ef, df);
             }
         }
     }
-    if (status == kRun && s.kind != 0 && !s.in_blas) {
+    if (status == kRun && (s.cur & kLeafBit) && !s.in_blas) {
         // TLAS leaf: enter the instance (trace_ray_bvh_instance); the BLAS
         // root is entered without a box test
         if (COUNT) { cnt[2]++; cnt[1]++; }
-        const GpuInstance* in = sc.inst + s.ref;
+        const uint32_t id = s.cur & kRefMask;
+        const GpuInstance* in = sc.inst + id;
         const float4 m0 = gld4(&in->m[0]), m1 = gld4(&in->m[4]), m2 = gld4(&in->m[8]);
         const float4 pp = L4 ? gld4(&in->quads) : gld4(&in->nodes);   // (nodes or 4-wide records, triangles)
         const uint64_t u0 = (uint64_t)__float_as_uint(pp.x) | ((uint64_t)__float_as_uint(pp.y) << 32);
@@ -377,16 +378,18 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
         s.cr.dy = (m0.y * w.dx + m1.x * w.dy) + (m1.w * w.dz + m2.z * 0.0f);
         s.cr.dz = (m0.z * w.dx + m1.y * w.dy) + (m2.x * w.dz + m2.w * 0.0f);
         s.cr.ix = 1.0f / s.cr.dx; s.cr.iy = 1.0f / s.cr.dy; s.cr.iz = 1.0f / s.cr.dz;
-        s.inst_id = s.ref;
-        s.ref = gldu(bn + 12);    // root left_first
-        s.kind = gldu(bn + 28);   // root prim_count
+        s.inst_id = id;
+        s.cur = gldu(bn + 12);    // the root's reference (its left_first word)
         s.base_sp = s.sp;
         s.in_blas = true;
     }
-    if (status == kRun && s.kind != 0 && s.in_blas) {
+    if (status == kRun && (s.cur & kLeafBit) && s.in_blas) {
         // BLAS leaf: triangles in index order; a hit replaces the closest
         // only if t < ray.t
-        for (uint32_t k = s.ref; k < s.ref + s.kind; ++k) {
+        const uint32_t first = s.cur & kRefMask;
+        uint32_t count = (s.cur >> 27) & 15u;
+        if (count == 0u) count = s.tris[first].pad0;
+        for (uint32_t k = first; k < first + count; ++k) {
             const GpuTri* tr = s.tris + k;
             const float4 a = gld4(&tr->v0[0]), b = gld4(&tr->e1[0]), c = gld4(&tr->e2[0]);
             if (COUNT) cnt[3]++;
@@ -396,7 +399,7 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
                 if (any) { status = kDone; break; }
             }
         }
-        if (status == kRun) pop_next();
+        if (status == kRun) pop();
     }
     return status;
 }

@@ -39,6 +39,29 @@ struct alignas(16) GpuInstance {
 };
 static_assert(sizeof(GpuInstance) == 80, "GpuInstance is 80 B");
 
+// Node references and stack entries of the traversal.
+// A node reference (the lane's current node, and every stack entry) is one
+// 32-bit word, stored pre-encoded in the .w of a child's lower half in the
+// node arrays the traversal reads (BLAS nodes after kernel
+// encode_blas_entries, TLAS pair records, 4-wide records):
+//   interior:  ref (bit 31 clear): the child pair at nodes + ref * 32 bytes
+//              (4-wide layout: the record at nodes + ref * kQuadBytes)
+//   leaf:      0x80000000 | count << 27 | first, count 1..15 (a TLAS leaf:
+//              count 1, first = the instance)
+//              0x80000000 | first: a BLAS leaf of 16 or more triangles, whose
+//              count sits in tris[first].pad0
+// so a visit pushes the far child's word as it is, and a pop restores the
+// node without a memory load (the reference keeps the whole node on its
+// stack, accelstruct.hlsl:107-125).
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kRefMask = 0x07FFFFFFu;
+constexpr uint32_t kMaxLeafRef = 1u << 27;
+
+__host__ __device__ inline uint32_t encode_ref(uint32_t ref, uint32_t count)
+{
+    return count == 0u ? ref : kLeafBit | (count < 16u ? count << 27 : 0u) | ref;
+}
+
 // 4-wide records (traversal layout 1).  Record r of a hierarchy holds, for
 // the child pair r of the 2-wide layout (nodes 2r and 2r+1), those two
 // children followed by the children of the first and then of the second
@@ -81,9 +104,9 @@ struct GpuTexture {
 // order; record j (64 B) holds the two children of interior node j as
 // {aabb_min, ref, aabb_max, kind}, with kind 0 and ref = 2 * (the child's
 // record number) for an interior child (its record sits at ref * 32 bytes)
-// and kind 1, ref = instance for a leaf; the same layout as two adjacent
-// BLAS nodes {aabb_min, left_first, aabb_max, prim_count}, so one visit
-// routine serves both levels.  The device copy of the reference-layout
+// and kind 1, ref = encode_ref(instance, 1) for a leaf; the same layout as
+// two adjacent BLAS nodes {aabb_min, left_first, aabb_max, prim_count} in
+// the traversal's form, so one visit routine serves both levels.  The device copy of the reference-layout
 // nodes (used for the root box test) carries an interior node's record
 // number in its otherwise unused instance_idx.
 struct TraceScene {
