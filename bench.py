@@ -405,12 +405,15 @@ def bench_inverse(steps: int):
 def batch_candidates(headline):
     """The configurations the batch tuner times: the default, the four
     fastest on the headline rays, and the fastest there with a scratch
-    (private) stack and with the 4-wide layout."""
+    (private) stack, with the 4-wide layout, and with and without the
+    popped-interior skip (2-wide, LDS stack)."""
     from wavefrontpathtracing_amd import tuning
     picks = [dict(tuning.DEFAULT)]
     ranked = [c for c, _ in headline] if headline else list(tuning.candidates())
     picks += ranked[:4]
-    for pred in (lambda c: c["lds_stack"] == 0, lambda c: c["bvh_layout"] == 1):
+    for pred in (lambda c: c["lds_stack"] == 0, lambda c: c["bvh_layout"] == 1,
+                 lambda c: c["bvh_layout"] == 0 and c["lds_stack"] > 0 and c["skip_popped"] == 1,
+                 lambda c: c["bvh_layout"] == 0 and c["lds_stack"] > 0 and c["skip_popped"] == 0):
         picks += [c for c in ranked if pred(c)][:1]
     out, seen = [], set()
     for c in picks:

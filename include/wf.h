@@ -233,7 +233,7 @@ WF_API wf_status wf_renderer_set_tlas_builder(wf_renderer* r, uint32_t builder);
 WF_API wf_status wf_renderer_set_tlas_metric(wf_renderer* r, uint32_t metric);
 
 /* Traversal kernel configuration (no reference counterpart: the autotuner's
- * axes; defaults 128, 12, 0, BVH2, no cap, 16, 0, 4, 0).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
+ * axes; defaults 256, 16, 1, BVH2, no cap, 16, 0, 4, 0).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
  * scratch), 4, 8, 12 or 16 (LDS, spilling to global memory past that depth);
  * skip_popped_interior drops popped interior nodes whose entry distance is
  * not below the current hit distance; bvh_layout 0 = BVH2; stack_limit caps
@@ -371,6 +371,16 @@ WF_API wf_status wf_tlas_build_device_metric(const wf_bvh_instance* d_instances,
 /* The software instance record submit_render_mesh derives (renderer.cpp:1318-1336). */
 WF_API wf_status wf_make_bvh_instance(const float transform[16], const float blas_min[3], const float blas_max[3],
                                       uint32_t bvh_index, wf_bvh_instance* out);
+
+/* Self-check of the device arithmetic the traversal relies on (no reference
+ * counterpart).  The traversal forms 1/x (ray and triangle reciprocals) as
+ * the hardware reciprocal refined by one fused Newton step where |x| is in
+ * [2^-126, 2^126], and as the IEEE division elsewhere; this compares that
+ * value with the division 1.0f / x, bit for bit, on all 2^32 float inputs on
+ * the current GPU (NaN results compare equal).  *out_mismatches receives the
+ * count and *out_first_bits the bit pattern of the smallest mismatching input
+ * (0xFFFFFFFF when none). */
+WF_API wf_status wf_selftest_reciprocal(uint64_t* out_mismatches, uint32_t* out_first_bits);
 
 #ifdef __cplusplus
 }

@@ -166,6 +166,7 @@ SIGNATURES = {
     "wf_renderer_set_tlas_metric": (_st, [_vp, _u32]),
     "wf_bvh_build_device": (_st, [_vp, _u32, C.POINTER(BuildOptions), _vp, _u64, _vp, C.POINTER(_u32), _vp]),
     "wf_bvh_build_device_noalloc": (_st, [_vp, _u32, C.POINTER(BuildOptions), _vp, _u64, _vp, C.POINTER(_u32), _vp]),
+    "wf_selftest_reciprocal": (_st, [C.POINTER(_u64), C.POINTER(_u32)]),
     "wf_tlas_build": (_st, [_vp, _u32, _vp, _u64, C.POINTER(_u64), C.POINTER(_u32)]),
     "wf_tlas_build_metric": (_st, [_vp, _u32, _u32, _vp, _u64, C.POINTER(_u64), C.POINTER(_u32)]),
     "wf_make_bvh_instance": (_st, [_vp, _vp, _vp, _u32, _vp]),

@@ -83,6 +83,26 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
     uint64_t lb = 0, le = 0, hint = 0;
     bool last_claim = false;
     const uint64_t share = 16ull * gridDim.x * (BLOCK / 64);
+    // the ray's result
+    auto finish = [&](int status) {
+        if (status == kOverflow) {
+            atomicAdd(&L.work[1], 1u);
+            s.h_inst = ~0u; s.h_prim = ~0u; s.h_u = 0.0f; s.h_v = 0.0f;
+        }
+        if (ANY) {
+            L.visible[ray_i] = (s.h_inst == ~0u && status != kOverflow) ? 1u : 0u;
+        } else if (MIX && lane_any) {
+            L.visible2[ray_i] = (s.h_inst == ~0u && status != kOverflow) ? 1u : 0u;
+        } else {
+            wf_hit o;
+            o.instance_idx = s.h_inst; o.primitive_idx = s.h_prim;
+            o.t = s.h_inst == ~0u ? kRayMaxT : s.t;
+            o.bary[0] = s.h_u; o.bary[1] = s.h_v;
+            L.hits[ray_i] = o;
+        }
+        if (COUNT) reinterpret_cast<uint4*>(L.counters)[ray_i] = make_uint4(cnt[0], cnt[1], cnt[2], cnt[3]);
+        ray_i = ~0u;
+    };
     for (;;) {
         const uint64_t idle = __ballot(ray_i == ~0u);
         const uint32_t n_idle = (uint32_t)__popcll(idle);
@@ -101,7 +121,7 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
                     if (big > g) g = big < kClaim ? (uint32_t)big : kClaim;
                     uint32_t base = 0;
                     if (lane == leader) base = atomicAdd(&L.work[0], g);
-                    base = __shfl(base, (int)leader, 64);
+                    base = __builtin_amdgcn_readlane(base, leader);   // wave-uniform (scalar registers)
                     lb = base;
                     le = (uint64_t)base + g < ntot ? (uint64_t)base + g : (ntot > base ? ntot : (uint64_t)base);
                     last_claim = (uint64_t)base + g >= ntot;
@@ -115,7 +135,7 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
             } else {
                 uint32_t base = 0;
                 if (lane == leader) base = atomicAdd(&L.work[0], n_idle);
-                base = __shfl(base, (int)leader, 64);
+                base = __builtin_amdgcn_readlane(base, leader);
                 if ((uint64_t)base + n_idle >= ntot) drained = true;
                 if (ray_i == ~0u && (uint64_t)base + rank < ntot) q = (uint64_t)base + rank;
             }
@@ -136,25 +156,7 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
         }
         if (ray_i != ~0u && status == kRun)
             status = trace_pass<SKIP, COUNT, TLDS, L4>(s, sc, st, s_tl, visits, MIX ? lane_any : ANY, cnt);
-        if (ray_i != ~0u && status != kRun) {
-            if (status == kOverflow) {
-                atomicAdd(&L.work[1], 1u);
-                s.h_inst = ~0u; s.h_prim = ~0u; s.h_u = 0.0f; s.h_v = 0.0f;
-            }
-            if (ANY) {
-                L.visible[ray_i] = (s.h_inst == ~0u && status != kOverflow) ? 1u : 0u;
-            } else if (MIX && lane_any) {
-                L.visible2[ray_i] = (s.h_inst == ~0u && status != kOverflow) ? 1u : 0u;
-            } else {
-                wf_hit o;
-                o.instance_idx = s.h_inst; o.primitive_idx = s.h_prim;
-                o.t = s.h_inst == ~0u ? kRayMaxT : s.t;
-                o.bary[0] = s.h_u; o.bary[1] = s.h_v;
-                L.hits[ray_i] = o;
-            }
-            if (COUNT) reinterpret_cast<uint4*>(L.counters)[ray_i] = make_uint4(cnt[0], cnt[1], cnt[2], cnt[3]);
-            ray_i = ~0u;
-        }
+        if (ray_i != ~0u && status != kRun) finish(status);
     }
 }
 
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
         __syncthreads();
     }
     if constexpr (LDS > 0) {
-        LdsStack<BLOCK, LDS, SKIP> st{s_ent + tid, s_dist + (SKIP ? tid : 0u), L.spill + gtid, L.spill_per_thread,
+        LdsStack<BLOCK, LDS, SKIP> st{s_ent + tid, s_dist + (SKIP ? tid : 0u), L.spill, gtid, L.spill_per_thread,
                                       gridDim.x * BLOCK, L.stack_cap};
         extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4, MIX, BIG>(L, st, s_tl, tid & 63u, n);
     } else {

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_megakernel(FrameArgs A, MegaLaunch
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     if constexpr (LDS > 0) {
-        LdsStack<BLOCK, LDS, true> st{s_ent + tid, s_dist + tid, M.spill + gtid, M.spill_per_thread, gridDim.x * BLOCK, M.stack_cap};
+        LdsStack<BLOCK, LDS, true> st{s_ent + tid, s_dist + tid, M.spill, gtid, M.spill_per_thread, gridDim.x * BLOCK, M.stack_cap};
         mega_loop<BLOCK, LDS, L4>(A, M, st, tid & 63u);
     } else {
         PrivStack<128> st;

@@ -480,6 +480,7 @@ wf_status wf_renderer::encode_mesh_refs(wf::Mesh& m)
         e = launch_encode_blas_refs((wf_bvh_node*)m.nodes.p, m.node_count, (GpuTri*)m.tris.p, m.tri_count, (uint32_t*)d_check.p, stream);
     uint32_t bad = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_check.p, 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&m.root_ref, (const char*)m.nodes.p + 12, 4, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "BLAS reference encoding");
     if (bad) return fail(WF_ERROR_INVALID_STATE, "a BLAS leaf's triangle range is out of bounds");
@@ -583,7 +584,10 @@ wf_status wf_renderer::upload_scene()
             g.nodes = (const wf_bvh_node*)m.nodes.p;
             g.tris = (const GpuTri*)m.tris.p;
             g.quads = (const char*)m.quads.p;
-            g.tris4 = g.tris;
+            // 4-wide record 0 holds the root alone, an interior reference
+            // there being a record number (left_first / 2)
+            g.root2 = m.root_ref;
+            g.root4 = (m.root_ref & kLeafBit) ? m.root_ref : m.root_ref >> 1;
             gi.push_back(g);
             GpuShadeInstance sh;
             std::memset(&sh, 0, sizeof(sh));

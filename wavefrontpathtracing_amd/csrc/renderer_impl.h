@@ -21,6 +21,7 @@ struct Mesh {
     bool alive = false;
     uint32_t version = 0;
     uint32_t tri_count = 0, node_count = 0, max_leaf = 0;
+    uint32_t root_ref = 0;     // the BLAS root's traversal reference (node 0's left_first word)
     float blas_min[3] = {0, 0, 0}, blas_max[3] = {0, 0, 0};
     std::vector<float> pos9;   // host copy of positions (emitter sampling)
     DevBuf nodes, tris, shade;

@@ -23,6 +23,22 @@ __device__ __forceinline__ float hmax(float a, float b) { return fmaxf(a, b); }
 
 struct Ray3 { float ox, oy, oz, dx, dy, dz, ix, iy, iz; };
 
+// 1/x rounded to nearest, as the IEEE division 1.0f / x gives it.  For
+// |x| in [2^-126, 2^126] the hardware reciprocal estimate (within 1 ulp) is
+// refined by one Newton-Raphson step with two fused multiply-adds,
+// r' = r + r (1 - x r) (Markstein's reciprocal iteration); zeros,
+// infinities, NaN and denormal or very large |x| take the division.
+// wf_selftest_reciprocal compares the two on every float.
+__device__ __forceinline__ float recip(float x)
+{
+    const float ax = fabsf(x);
+    if (ax >= 0x1p-126f && ax <= 0x1p126f) {
+        const float r = __builtin_amdgcn_rcpf(x);
+        return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+    }
+    return 1.0f / x;
+}
+
 __device__ __forceinline__ float slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
                                       const Ray3& r, float t)
 {
@@ -52,7 +68,7 @@ __device__ __forceinline__ bool tri_test(const float4& a, const float4& b, const
     const float pz = r.dx * e2y - r.dy * e2x;
     const float det = (e1x * px + e1y * py) + e1z * pz;
     if (det < kIntersectEps) return false;
-    const float inv = 1.0f / det;
+    const float inv = recip(det);
     const float sx = r.ox - a.x, sy = r.oy - a.y, sz = r.oz - a.z;
     const float v = ((sx * px + sy * py) + sz * pz) * inv;
     if (v < 0.0f || v > 1.0f) return false;
@@ -133,7 +149,8 @@ template <int BLOCK, int LDS, bool DIST = false>
 struct LdsStack {
     uint32_t* ent;           // this thread's level 0
     uint16_t* dist;          // entry distances (DIST), same levels
-    uint2* spill;            // this thread's first spill slot
+    uint2* spill;            // the grid's spill area
+    uint32_t spill_id;       // this thread's column in it (its index in the grid)
     uint32_t spill_n;        // spill levels
     uint32_t spill_stride;   // threads of the grid
     uint32_t cap;
@@ -155,8 +172,9 @@ struct LdsStack {
         if (sp >= cap) return false;
         const uint32_t k = sp - LDS;
         if (k >= spill_n) return false;
-        if (DIST) gstu2(spill + (size_t)k * spill_stride, make_uint2(e, __float_as_uint(d)));
-        else gstu(spill + (size_t)k * spill_stride, e);
+        uint2* q = spill + ((size_t)k * spill_stride + spill_id);
+        if (DIST) gstu2(q, make_uint2(e, __float_as_uint(d)));
+        else gstu(q, e);
         return true;
     }
     __device__ __forceinline__ bool push(uint32_t sp, uint32_t e, float d)
@@ -173,7 +191,7 @@ struct LdsStack {
             d = DIST ? half_up_to_float(dist[(sp + 1u) * BLOCK]) : 0.0f;
             return;
         }
-        const uint2* q = spill + (size_t)(sp - LDS) * spill_stride;
+        const uint2* q = spill + ((size_t)(sp - LDS) * spill_stride + spill_id);
         if (DIST) { const uint2 v = gldu2(q); e = v.x; d = __uint_as_float(v.y); }
         else { e = gldu(q); d = 0.0f; }
     }
@@ -197,6 +215,8 @@ struct PrivStack {
 
 enum LaneStatus { kRun = 0, kDone = 1, kOverflow = 2 };
 
+constexpr uint32_t kNoBlas = 0xFFFFFFFFu;
+
 // Per-lane traversal state.  `cur` is the lane's node (encoding above):
 // interior nodes are visited from `nodes` (the TLAS pair array, or the BLAS
 // node array, whose children are adjacent; 4-wide records in layout 1), a
@@ -206,8 +226,10 @@ enum LaneStatus { kRun = 0, kDone = 1, kOverflow = 2 };
 struct TraceLane {
     Ray3 wr, cr;
     float t;
-    uint32_t cur, sp, base_sp;
-    bool in_blas;
+    uint32_t cur, sp;
+    float cur_d;               // entry distance `cur` was pushed with, when it was popped (else -inf)
+    uint32_t base_sp;          // stack depth at the BLAS entry; kNoBlas while in the TLAS
+    __device__ __forceinline__ bool in_blas() const { return base_sp != kNoBlas; }
     const char* nodes;
     const GpuTri* tris;
     uint32_t inst_id, h_inst, h_prim;
@@ -224,9 +246,9 @@ __device__ __forceinline__ bool trace_begin(TraceLane& s, const TraceScene& sc, 
     s.t = ray.tmax;
     s.wr.ox = ray.origin[0]; s.wr.oy = ray.origin[1]; s.wr.oz = ray.origin[2];
     s.wr.dx = ray.direction[0]; s.wr.dy = ray.direction[1]; s.wr.dz = ray.direction[2];
-    s.wr.ix = 1.0f / s.wr.dx; s.wr.iy = 1.0f / s.wr.dy; s.wr.iz = 1.0f / s.wr.dz;
+    s.wr.ix = recip(s.wr.dx); s.wr.iy = recip(s.wr.dy); s.wr.iz = recip(s.wr.dz);
     s.cr = s.wr;
-    s.sp = 0; s.base_sp = 0; s.in_blas = false;
+    s.sp = 0; s.base_sp = kNoBlas; s.cur_d = -INFINITY;
     s.nodes = reinterpret_cast<const char*>(L4 ? sc.tquads : sc.tpairs);
     if (COUNT) { cnt[0] = 1; cnt[1] = 0; cnt[2] = 0; cnt[3] = 0; }
     const float4 ra = gld4(&sc.tnodes[root]);
@@ -248,9 +270,9 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
                                           uint32_t visits, bool any, uint32_t* cnt)
 {
     static_assert(!L4 || (!COUNT && !TLDS), "the 4-wide layout has no counting or LDS-TLAS variant");
-    // the LDS stack without entry distances: a visit writes the far child at
-    // the top unconditionally and pops with the entry it read first
-    constexpr bool FAST = Stack::kLds && !SKIP;
+    // the LDS stack: a visit writes the far child at the top unconditionally
+    // and pops with the entry it read first
+    constexpr bool FAST = Stack::kLds;
     constexpr uint32_t LDSD = Stack::kLdsDepth;
     int status = kRun;
     // pushes below this depth go to LDS with no further check
@@ -259,20 +281,21 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
     const char* tlas_nodes = reinterpret_cast<const char*>(L4 ? sc.tquads : sc.tpairs);
     // the BLAS part of the stack is used up: back to the world ray and the TLAS
     auto leave_blas = [&]() {
-        s.in_blas = false;
+        s.base_sp = kNoBlas;
         s.cr = s.wr;
         s.nodes = tlas_nodes;
     };
     // pop until the next node is known (or the stack is empty)
     auto pop = [&]() {
         for (;;) {
-            if (s.in_blas && s.sp == s.base_sp) leave_blas();
+            if (s.sp == s.base_sp) leave_blas();
             if (s.sp == 0) { status = kDone; return; }
             uint32_t e; float dd;
             --s.sp;
             st.get(s.sp, e, dd);
-            if (SKIP && !(e & kLeafBit) && dd >= s.t) { if (COUNT) cnt[s.in_blas ? 1 : 0] += 2; continue; }
+            if (SKIP && !(e & kLeafBit) && dd >= s.t) { if (COUNT) cnt[s.in_blas() ? 1 : 0] += 2; continue; }
             s.cur = e;
+            s.cur_d = -INFINITY;
             return;
         }
     };
@@ -309,20 +332,27 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
         // interior node: near child next, far child pushed (swap only when
         // dist_left > dist_right)
         uint32_t pre = 0u;
-        if constexpr (FAST) pre = st.top(s.sp);   // read before the node loads return
+        float pre_d = -INFINITY;
+        if constexpr (FAST) {
+            pre = st.top(s.sp);   // read before the node loads return
+            if constexpr (SKIP) pre_d = st.top_dist(s.sp);
+        }
         float4 a, b, c, d;
-        if (TLDS && !s.in_blas) {
+        if (TLDS && !s.in_blas()) {
             const float4* q = s_tl + 2u * s.cur;
             a = q[0]; b = q[1]; c = q[2]; d = q[3];
         } else {
             const char* cp = s.nodes + (size_t)s.cur * 32u;
             a = gld4(cp); b = gld4(cp + 16); c = gld4(cp + 32); d = gld4(cp + 48);
         }
-        if (COUNT) cnt[s.in_blas ? 1 : 0] += 2;
+        if (COUNT) cnt[s.in_blas() ? 1 : 0] += 2;
         const float dl = slab(a.x, a.y, a.z, b.x, b.y, b.z, s.cr, s.t);
         const float dr = slab(c.x, c.y, c.z, d.x, d.y, d.z, s.cr, s.t);
         const bool sw = dl > dr;
-        const float dn = sw ? dr : dl, df = sw ? dl : dr;
+        // a node popped with an entry distance no closer than the current hit
+        // is not opened: its children's entries are no closer, so both miss
+        const bool shut = SKIP && FAST && s.cur_d >= s.t;
+        const float dn = shut ? kRayMaxT : (sw ? dr : dl), df = shut ? kRayMaxT : (sw ? dl : dr);
         const uint32_t en = __float_as_uint(sw ? c.w : a.w), ef = __float_as_uint(sw ? a.w : c.w);
         if constexpr (FAST) {
             // selects for the common cases; branches only for a push that
@@ -331,8 +361,9 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
             const uint32_t sp0 = s.sp;
             const bool hit = dn != kRayMaxT, pushf = df != kRayMaxT;   // no hit: no push either
             s.cur = hit ? en : pre;
+            if constexpr (SKIP) s.cur_d = hit ? -INFINITY : pre_d;
             s.sp = sp0 + (pushf ? 1u : 0u) - (hit ? 0u : 1u);
-            if (!hit && s.in_blas && sp0 == s.base_sp) leave_blas();
+            if (!hit && sp0 == s.base_sp) leave_blas();
             if (sp0 >= fast_lim || (!hit && sp0 == 0u)) {
                 // a push into the spill area or past the cap, a pop of a
                 // spilled entry, or the stack is empty
@@ -345,6 +376,7 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
                 } else if (!hit && sp0 > LDSD) {
                     float dd;
                     st.get(sp0 - 1u, s.cur, dd);
+                    if constexpr (SKIP) s.cur_d = dd;
                 }
             }
         } else {
@@ -356,18 +388,26 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
             }
         }
     }
-    if (status == kRun && (s.cur & kLeafBit) && !s.in_blas) {
+    if (status == kRun && (s.cur & kLeafBit) && !s.in_blas()) {
         // TLAS leaf: enter the instance (trace_ray_bvh_instance); the BLAS
         // root is entered without a box test
         if (COUNT) { cnt[2]++; cnt[1]++; }
         const uint32_t id = s.cur & kRefMask;
         const GpuInstance* in = sc.inst + id;
         const float4 m0 = gld4(&in->m[0]), m1 = gld4(&in->m[4]), m2 = gld4(&in->m[8]);
-        const float4 pp = L4 ? gld4(&in->quads) : gld4(&in->nodes);   // (nodes or 4-wide records, triangles)
-        const uint64_t u0 = (uint64_t)__float_as_uint(pp.x) | ((uint64_t)__float_as_uint(pp.y) << 32);
+        const float4 pp = gld4(&in->nodes);   // (nodes, triangles)
         const uint64_t u1 = (uint64_t)__float_as_uint(pp.z) | ((uint64_t)__float_as_uint(pp.w) << 32);
-        const char* bn = reinterpret_cast<const char*>(u0);
-        s.nodes = bn;
+        uint64_t u0;
+        uint32_t root;
+        if constexpr (L4) {
+            const float4 pq = gld4(&in->quads);   // (4-wide records, root2, root4)
+            u0 = (uint64_t)__float_as_uint(pq.x) | ((uint64_t)__float_as_uint(pq.y) << 32);
+            root = __float_as_uint(pq.w);
+        } else {
+            u0 = (uint64_t)__float_as_uint(pp.x) | ((uint64_t)__float_as_uint(pp.y) << 32);
+            root = gldu(&in->root2);
+        }
+        s.nodes = reinterpret_cast<const char*>(u0);
         s.tris = reinterpret_cast<const GpuTri*>(u1);
         const Ray3& w = s.wr;
         // m[c*3+r]: m0 = (c0r0 c0r1 c0r2 c1r0), m1 = (c1r1 c1r2 c2r0 c2r1), m2 = (c2r2 c3r0 c3r1 c3r2)
@@ -377,13 +417,14 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
         s.cr.dx = (m0.x * w.dx + m0.w * w.dy) + (m1.z * w.dz + m2.y * 0.0f);
         s.cr.dy = (m0.y * w.dx + m1.x * w.dy) + (m1.w * w.dz + m2.z * 0.0f);
         s.cr.dz = (m0.z * w.dx + m1.y * w.dy) + (m2.x * w.dz + m2.w * 0.0f);
-        s.cr.ix = 1.0f / s.cr.dx; s.cr.iy = 1.0f / s.cr.dy; s.cr.iz = 1.0f / s.cr.dz;
+        s.cr.ix = recip(s.cr.dx); s.cr.iy = recip(s.cr.dy); s.cr.iz = recip(s.cr.dz);
         s.inst_id = id;
-        s.cur = gldu(bn + 12);    // the root's reference (its left_first word)
+        s.cur = root;
+        s.cur_d = -INFINITY;
         s.base_sp = s.sp;
-        s.in_blas = true;
+
     }
-    if (status == kRun && (s.cur & kLeafBit) && s.in_blas) {
+    if (status == kRun && (s.cur & kLeafBit) && s.in_blas()) {
         // BLAS leaf: triangles in index order; a hit replaces the closest
         // only if t < ray.t
         const uint32_t first = s.cur & kRefMask;

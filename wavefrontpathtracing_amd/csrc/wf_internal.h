@@ -27,15 +27,16 @@ struct alignas(16) GpuTri {
 static_assert(sizeof(GpuTri) == 48, "GpuTri is 48 B");
 
 // Instance record read at every TLAS leaf: rows 0..2 of world_to_local
-// (column-major, m[c*3+r]) and the mesh's device arrays: the BLAS nodes and
-// triangles as one 16-byte load, and the 4-wide records with the triangles as
-// another (null until the 4-wide layout is in use).
+// (column-major, m[c*3+r]), the mesh's BLAS nodes and triangles as one
+// 16-byte load, and the 4-wide records (null until that layout is in use)
+// with the BLAS root's reference in each layout (the root is entered without
+// a box test, so no load of the root node is needed).
 struct alignas(16) GpuInstance {
     float m[12];
     const wf_bvh_node* nodes;
     const GpuTri* tris;
     const char* quads;
-    const GpuTri* tris4;
+    uint32_t root2, root4;    // the BLAS root's reference: 2-wide nodes, 4-wide records
 };
 static_assert(sizeof(GpuInstance) == 80, "GpuInstance is 80 B");
 
@@ -123,9 +124,9 @@ constexpr uint32_t kTlasLdsPairs = 1024;
 // Traversal configuration (the autotuner's axes).
 struct ExtendConfig {
     // defaults: the autotuner's choice on the incoherent 256-instance workload
-    uint32_t block = 128;       // workgroup size: 64, 128, 256
-    uint32_t lds_stack = 12;    // LDS stack entries per lane (0 = private/scratch stack)
-    uint32_t skip_popped = 0;   // skip popped interior nodes whose entry distance >= t
+    uint32_t block = 256;       // workgroup size: 64, 128, 256
+    uint32_t lds_stack = 16;    // LDS stack entries per lane (0 = private/scratch stack)
+    uint32_t skip_popped = 1;   // skip popped interior nodes whose entry distance >= t
     uint32_t stack_limit = 0;   // cap on stack entries per lane (0 = capacity)
     uint32_t refill_min = 16;   // idle lanes of a wave that trigger a refill
     uint32_t tlas_lds = 0;      // copy the TLAS records to LDS when they fit

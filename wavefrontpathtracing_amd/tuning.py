@@ -14,7 +14,7 @@ from . import _lib as L
 
 # the library's default configuration (the autotuner's choice on the incoherent
 # 256-instance workload); a candidate replaces it only when measurably faster
-DEFAULT = dict(block=128, lds_stack=12, skip_popped=0, bvh_layout=0, refill_min=16, tlas_in_lds=0, interior_visits=4)
+DEFAULT = dict(block=256, lds_stack=16, skip_popped=1, bvh_layout=0, refill_min=16, tlas_in_lds=0, interior_visits=4)
 
 
 def set_extend_config(renderer, block: int = DEFAULT["block"], lds_stack: int = DEFAULT["lds_stack"],
