@@ -351,6 +351,7 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
         const bool sw = dl > dr;
         // a node popped with an entry distance no closer than the current hit
         // is not opened: its children's entries are no closer, so both miss
+        // (the loads are issued anyway: a branch around them costs more)
         const bool shut = SKIP && FAST && s.cur_d >= s.t;
         const float dn = shut ? kRayMaxT : (sw ? dr : dl), df = shut ? kRayMaxT : (sw ? dl : dr);
         const uint32_t en = __float_as_uint(sw ? c.w : a.w), ef = __float_as_uint(sw ? a.w : c.w);
