@@ -403,10 +403,13 @@ def bench_inverse(steps: int):
 
 
 def batch_candidates(headline):
-    """The configurations the batch tuner times: the default, the four
-    fastest on the headline rays, and the fastest there with a scratch
-    (private) stack, with the 4-wide layout, and with and without the
-    popped-interior skip (2-wide, LDS stack)."""
+    """The configurations the batch tuner times (the same list for the batch
+    and for the per-scene loop): the default, the four fastest on the
+    headline rays, and the fastest there with a scratch (private) stack,
+    with the 4-wide layout, and with and without the popped-interior skip
+    (2-wide, LDS stack); then the default and the two fastest there at
+    refill thresholds of 24 and 32 idle lanes (launches of short rays claim
+    queue entries often, and fewer, larger claims can pay)."""
     from wavefrontpathtracing_amd import tuning
     picks = [dict(tuning.DEFAULT)]
     ranked = [c for c, _ in headline] if headline else list(tuning.candidates())
@@ -415,6 +418,8 @@ def batch_candidates(headline):
                  lambda c: c["bvh_layout"] == 0 and c["lds_stack"] > 0 and c["skip_popped"] == 1,
                  lambda c: c["bvh_layout"] == 0 and c["lds_stack"] > 0 and c["skip_popped"] == 0):
         picks += [c for c in ranked if pred(c)][:1]
+    for c in [dict(tuning.DEFAULT)] + ranked[:2]:
+        picks += [dict(c, refill_min=m) for m in (24, 32)]
     out, seen = [], set()
     for c in picks:
         key = tuning.label(c)
@@ -463,7 +468,7 @@ def bench_batched(frames: int, tune: bool, headline=None, fixed=None):
         best, table = tuning.autotune(r, lambda: batched_frame()[0], configs=batch_candidates(headline), repeats=1)
         tuned = {"winner": best, "seconds": round(time.perf_counter() - t_tune, 1),
                  "table_ms_per_frame": {tuning.label(c): round(t * 1e3, 1) for c, t in table},
-                 "labels": "block/lds_stack/skip_popped/bvh_layout/interior_visits"}
+                 "labels": "block/lds_stack/skip_popped/bvh_layout/interior_visits/refill_min"}
     batched_frame()
     bt, brays = 0.0, 0
     for _ in range(frames):
@@ -666,9 +671,15 @@ def main():
 
         t_tune = time.perf_counter()
         cfgs = [dict(c, interior_visits=v) for c in tuning.candidates() for v in (2, 4)]
-        best, table = tuning.autotune(r, timed, configs=cfgs, repeats=3)
+        _, table = tuning.autotune(r, timed, configs=cfgs, repeats=3)
+        # the refill threshold for the fastest four (the default stays unless beaten by 1 %)
+        cfgs2 = [dict(tuning.DEFAULT)] + [dict(c, refill_min=m) for c, _ in table[:4] for m in (16, 24, 32)]
+        best, table2 = tuning.autotune(r, timed, configs=cfgs2, repeats=3)
+        seen = {tuning.label(c) for c, _ in table2}
+        table = sorted(table2 + [(c, t) for c, t in table if tuning.label(c) not in seen], key=lambda x: x[1])
         autotuned = {"workload": "the headline rays", "candidates": len(table), "seconds": round(time.perf_counter() - t_tune, 1),
-                     "winner": best, "table_mrays": {tuning.label(c): round(len(rays) / t / 1e6, 1) for c, t in table}}
+                     "winner": best, "table_mrays": {tuning.label(c): round(len(rays) / t / 1e6, 1) for c, t in table},
+                     "labels": "block/lds_stack/skip_popped/bvh_layout/interior_visits/refill_min"}
 
     for _ in range(args.warmup):
         step()

@@ -48,9 +48,9 @@ def candidates():
 
 
 def label(cfg: dict) -> str:
-    """Short name of a configuration: block/lds/skip/layout/visits."""
+    """Short name of a configuration: block/lds/skip/layout/visits/refill."""
     c = {**DEFAULT, **cfg}
-    return f"{c['block']}/{c['lds_stack']}/{c['skip_popped']}/{c['bvh_layout']}/{c['interior_visits']}"
+    return f"{c['block']}/{c['lds_stack']}/{c['skip_popped']}/{c['bvh_layout']}/{c['interior_visits']}/{c['refill_min']}"
 
 
 def autotune(renderer, time_fn, configs=None, repeats: int = 3, margin: float = 0.01):
