@@ -233,7 +233,7 @@ WF_API wf_status wf_renderer_set_tlas_builder(wf_renderer* r, uint32_t builder);
 WF_API wf_status wf_renderer_set_tlas_metric(wf_renderer* r, uint32_t metric);
 
 /* Traversal kernel configuration (no reference counterpart: the autotuner's
- * axes; defaults 256, 16, 1, BVH2, no cap, 16, 0, 4, 0).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
+ * axes; defaults 256, 16, 1, BVH2, no cap, 16, 0, 4, 0, 64).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
  * scratch), 4, 8, 12 or 16 (LDS, spilling to global memory past that depth);
  * skip_popped_interior drops popped interior nodes whose entry distance is
  * not below the current hit distance; bvh_layout 0 = BVH2; stack_limit caps
@@ -249,11 +249,15 @@ WF_API wf_status wf_renderer_set_tlas_metric(wf_renderer* r, uint32_t metric);
  * big_claims selects how a wave of a closest-hit launch that also carries
  * shadow rays claims queue entries: 0 = by the queues' capacity (at least
  * 2^24 entries, as in a large batch: up to 256 entries per atomic; fewer:
- * one entry per idle lane), 1 = always the large claims, 2 = never. */
+ * one entry per idle lane), 1 = always the large claims, 2 = never.
+ * claim_segments (a power of two in 1..64, 0 = default 64) caps how many
+ * cursors an extend launch claims its queue from: 64-entry chunks of the
+ * queue are dealt round-robin to that many segments (fewer for queues under
+ * 512 entries per segment), each with a cursor of its own. */
 typedef struct wf_traversal_config {
     uint32_t workgroup_size; uint32_t lds_stack_entries; uint32_t skip_popped_interior; uint32_t bvh_layout;
     uint32_t stack_limit; uint32_t refill_min; uint32_t tlas_in_lds; uint32_t interior_visits;
-    uint32_t big_claims;
+    uint32_t big_claims; uint32_t claim_segments;
 } wf_traversal_config;
 WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, const wf_traversal_config* config);
 WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* r, wf_traversal_config* out);

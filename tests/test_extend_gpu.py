@@ -250,6 +250,45 @@ def test_stack_overflow_on_the_device_path(small):
     _assert_bitexact(np.frombuffer(d_hits.cpu().numpy().tobytes(), ref.dtype), ref, "device path after the cap is lifted")
 
 
+@pytest.mark.parametrize("segs", [1, 2, 8, 64])
+def test_queue_segments_cover_every_ray(small, segs):
+    """The extend launch deals its queue in 64-entry chunks to up to `segs`
+    cursors: at queue lengths on both sides of the chunk and segment
+    boundaries, every ray is traced exactly as the oracle does and no output
+    keeps the sentinel the buffer was filled with."""
+    import torch
+    sc, r, orc = small
+    from wavefrontpathtracing_amd import tuning
+    bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
+    sr = scenes.sphere_rays(17, bmin, bmax, 70001)
+    ref = orc.trace_closest(sr, nthreads=8)
+    d_rays = torch.from_numpy(np.frombuffer(sr.tobytes(), np.uint8).copy()).cuda()
+    d_hits = torch.empty(len(sr) * ref.dtype.itemsize, dtype=torch.uint8, device="cuda")
+    tuning.set_extend_config(r, claim_segments=segs, refill_min=8)
+    try:
+        assert tuning.get_extend_config(r)["claim_segments"] == segs
+        for n in (1, 63, 64, 65, 255, 256, 257, 511, 512, 513, 4095, 4096, 4097, 32767, 32769, 70001):
+            d_hits.fill_(0xAB)
+            torch.cuda.synchronize()
+            r.trace_closest_device(d_rays.data_ptr(), n, d_hits.data_ptr())
+            torch.cuda.synchronize()   # the renderer's stream too
+            got = np.frombuffer(d_hits.cpu().numpy().tobytes(), ref.dtype)
+            _assert_bitexact(got[:n], ref[:n], f"{n} rays, {segs} segments")
+            assert np.all(np.frombuffer(d_hits.cpu().numpy().tobytes()[n * ref.dtype.itemsize:], np.uint8) == 0xAB)
+    finally:
+        tuning.set_extend_config(r)
+
+
+def test_claim_segments_must_be_a_power_of_two(small):
+    sc, r, orc = small
+    from wavefrontpathtracing_amd import tuning
+    for bad in (3, 48, 128):
+        with pytest.raises(WfError):
+            tuning.set_extend_config(r, claim_segments=bad)
+    tuning.set_extend_config(r)
+    assert tuning.get_extend_config(r)["claim_segments"] == 64
+
+
 @pytest.mark.parametrize("block,lds,skip,tlds,refill,visits,layout", [
     (64, 8, 1, 1, 16, 1, 0), (128, 0, 1, 1, 1, 2, 0), (256, 4, 0, 1, 64, 3, 0), (128, 16, 0, 0, 16, 8, 0),
     (256, 8, 1, 0, 32, 4, 0), (64, 0, 1, 0, 48, 1, 0),

@@ -94,7 +94,8 @@ class FrameStats(C.Structure):
 class TraversalConfig(C.Structure):
     _fields_ = [("workgroup_size", C.c_uint32), ("lds_stack_entries", C.c_uint32), ("skip_popped_interior", C.c_uint32),
                 ("bvh_layout", C.c_uint32), ("stack_limit", C.c_uint32), ("refill_min", C.c_uint32),
-                ("tlas_in_lds", C.c_uint32), ("interior_visits", C.c_uint32), ("big_claims", C.c_uint32)]
+                ("tlas_in_lds", C.c_uint32), ("interior_visits", C.c_uint32), ("big_claims", C.c_uint32),
+                ("claim_segments", C.c_uint32)]
 
 
 class Pcg32(C.Structure):

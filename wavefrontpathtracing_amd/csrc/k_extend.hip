@@ -13,9 +13,10 @@
 //   * ray.t is shared between the world and object-space rays.
 //
 // MI355X mechanics (not in the reference):
-//   * persistent waves pull 64-ray batches from a global atomic cursor, so
-//     the grid is sized to the machine and the queue length is read on the
-//     device (no indirect dispatch on HIP);
+//   * persistent waves claim rays for their idle lanes from a queue dealt in
+//     up to 64 segments with a cursor each (claims.h), so the grid is sized
+//     to the machine, the queue length is read on the device (no indirect
+//     dispatch on HIP) and claims are not serialised on one atomic;
 //   * one combined TLAS+BLAS stack of 32-bit node references lives in LDS,
 //     strided by workgroup size so that lanes hit distinct banks, and spills
 //     to a per-thread global region past the LDS depth;
@@ -54,7 +55,7 @@ constexpr uint64_t kBigLaunch = 16ull << 20;  // queue capacity from which a mix
 // time and advances it one traversal pass per loop iteration (trace_pass).
 // A lane whose ray finishes writes its result and goes idle; once at least
 // `refill_min` lanes of the wave are idle, the wave claims that many rays
-// from the global cursor with one atomic (ballot + mbcnt give each idle lane
+// from the queue with one atomic (ballot + mbcnt give each idle lane
 // its index), and the new rays start with their root box test in the same
 // iteration in which the other lanes make their pass (and then make their
 // own first pass).  The loop ends when every lane is idle and the queue is
@@ -74,15 +75,19 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
     const uint64_t ntot = (uint64_t)n + n2;
     bool lane_any = ANY;
 
+    // The queue is dealt from segments with their own cursors (claims.h).
     // BIG (launches of tens of millions of entries): the wave claims up to
     // kClaim entries per atomic and keeps the unstarted ones in [lb, le); the
-    // claim shrinks with the entries left after the wave's previous claim,
-    // (left) / (16 x waves in the grid), so ranges still held when the queue
-    // runs out stay small and the tail is not lengthened.  Otherwise each
-    // refill claims exactly the idle lanes' entries.
-    uint64_t lb = 0, le = 0, hint = 0;
-    bool last_claim = false;
-    const uint64_t share = 16ull * gridDim.x * (BLOCK / 64);
+    // claim shrinks with the segment's entries left after the wave's previous
+    // claim there, (left) / (16 x waves per segment), so ranges still held
+    // when the queue runs out stay small and the tail is not lengthened.
+    // Otherwise each refill claims exactly the idle lanes' entries.
+    // (BIG claims are rare enough for one cursor, which keeps the band of
+    // rays in flight narrowest)
+    SegQueue wq;
+    wq.init(L.work + kWorkClaimBase, ntot, blockIdx.x * (BLOCK / 64) + threadIdx.x / 64u, BIG ? 1u : L.claim_segs);
+    uint32_t lb = 0, le = 0;   // positions of the wave's current segment
+    const uint32_t share = BIG ? (16u * gridDim.x * (BLOCK / 64)) / wq.nseg + 1u : 1u;
     // the ray's result
     auto finish = [&](int status) {
         if (status == kOverflow) {
@@ -114,31 +119,23 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
             const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
             uint64_t q = ~0ull;
-            if constexpr (BIG) {
-                if (lb >= le) {
-                    uint32_t g = n_idle;
-                    const uint64_t big = (ntot > hint ? ntot - hint : 0ull) / share;
-                    if (big > g) g = big < kClaim ? (uint32_t)big : kClaim;
-                    uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(&L.work[0], g);
-                    base = __builtin_amdgcn_readlane(base, leader);   // wave-uniform (scalar registers)
-                    lb = base;
-                    le = (uint64_t)base + g < ntot ? (uint64_t)base + g : (ntot > base ? ntot : (uint64_t)base);
-                    last_claim = (uint64_t)base + g >= ntot;
-                    hint = (uint64_t)base + g;
+            if (lb >= le) {
+                uint32_t g = n_idle;
+                if constexpr (BIG) {
+                    const uint32_t big = wq.left_hint() / share;
+                    if (big > g) g = big < kClaim ? big : kClaim;
                 }
-                const uint32_t take = (uint32_t)(le - lb < (uint64_t)n_idle ? le - lb : (uint64_t)n_idle);
-                const uint64_t first = lb;
-                lb += take;
-                if (last_claim && lb >= le) drained = true;
-                if (ray_i == ~0u && rank < take && first + rank < ntot) q = first + rank;
-            } else {
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(&L.work[0], n_idle);
-                base = __builtin_amdgcn_readlane(base, leader);
-                if ((uint64_t)base + n_idle >= ntot) drained = true;
-                if (ray_i == ~0u && (uint64_t)base + rank < ntot) q = (uint64_t)base + rank;
+                uint32_t first = 0;
+                const uint32_t got = wq.claim(g, lane, leader, first);
+                lb = first;
+                le = first + got;
+                if (got == 0u) drained = true;
             }
+            const uint32_t take = le - lb < n_idle ? le - lb : n_idle;
+            const uint32_t first = lb;
+            lb += take;
+            if (lb >= le && wq.empty == ~0ull) drained = true;
+            if (ray_i == ~0u && rank < take) q = wq.entry(first + rank);
             if (q != ~0ull) {
                 if constexpr (MIX) {
                     const bool second = q >= n;
@@ -189,6 +186,13 @@ __global__ __launch_bounds__(BLOCK, 8) void k_extend(ExtendLaunch L)
         st.cap = L.stack_cap;
         extend_loop<BLOCK, LDS, SKIP, COUNT, ANY, TLDS, L4, MIX, BIG>(L, st, s_tl, tid & 63u, n);
     }
+}
+
+__global__ void k_clear_claims(uint32_t* work, uint32_t reset_overflow)
+{
+    const uint32_t i = threadIdx.x;
+    if (i < kClaimSegs) work[kWorkClaimBase + i * kClaimStride] = 0u;
+    if (i == 0u && reset_overflow) work[1] = 0u;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n)
@@ -288,6 +292,12 @@ hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStre
     void* args[] = {&arg};
     const size_t dyn = tlds ? (size_t)L.scene.n_tpairs * 64u : 0u;
     return hipLaunchKernel(k, dim3(tlds ? L.grid_tlds : L.grid), dim3(cfg.block), args, dyn, stream);
+}
+
+hipError_t launch_clear_claims(uint32_t* work, bool reset_overflow, hipStream_t stream)
+{
+    k_clear_claims<<<1, kClaimSegs, 0, stream>>>(work, reset_overflow ? 1u : 0u);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t stream)

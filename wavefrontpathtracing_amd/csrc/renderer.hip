@@ -125,7 +125,7 @@ wf_status wf_renderer::ensure_trace_scratch()
     const size_t threads = (size_t)std::max(ext_grid, mega_grid) * ext_cfg.block;
     if ((e = d_spill.ensure(std::max<size_t>(threads * spill_per_thread * sizeof(uint2), 16))) != hipSuccess)
         return hip_fail(e, "spill stack allocation");
-    if ((e = d_work.ensure(64)) != hipSuccess) return hip_fail(e, "work counter allocation");
+    if ((e = d_work.ensure(kWorkWords * sizeof(uint32_t))) != hipSuccess) return hip_fail(e, "work counter allocation");
     return WF_OK;
 }
 
@@ -247,6 +247,8 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     if (c->refill_min > 64) return r->fail(WF_ERROR_INVALID_ARGUMENT, "refill_min must be in [0, 64]");
     if (c->interior_visits > 8) return r->fail(WF_ERROR_INVALID_ARGUMENT, "interior_visits must be in [0, 8]");
     if (c->big_claims > 2) return r->fail(WF_ERROR_INVALID_ARGUMENT, "big_claims must be 0 (by capacity), 1 (always) or 2 (never)");
+    if (c->claim_segments > kClaimSegs || (c->claim_segments & (c->claim_segments - 1u)))
+        return r->fail(WF_ERROR_INVALID_ARGUMENT, "claim_segments must be 0 or a power of two up to 64");
     const ExtendConfig old = r->ext_cfg;
     r->ext_cfg.block = c->workgroup_size;
     r->ext_cfg.lds_stack = c->lds_stack_entries;
@@ -257,6 +259,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     r->ext_cfg.visits = c->interior_visits ? c->interior_visits : ExtendConfig{}.visits;
     r->ext_cfg.layout = c->bvh_layout;
     r->ext_cfg.big = c->big_claims;
+    r->ext_cfg.claim_segs = c->claim_segments ? c->claim_segments : kClaimSegs;
     if (r->ext_cfg.layout != old.layout) r->tlas_valid = false;   // instance records point at the layout's arrays
     r->drop_graph();
     const wf_status s = r->ensure_trace_scratch();
@@ -276,6 +279,7 @@ extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* 
     out->tlas_in_lds = r->ext_cfg.tlas_lds;
     out->interior_visits = r->ext_cfg.visits;
     out->big_claims = r->ext_cfg.big;
+    out->claim_segments = r->ext_cfg.claim_segs;
     return WF_OK;
 }
 
@@ -711,9 +715,9 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
                                   const OcclusionQueue* second, bool reset_overflow)
 {
     if (second && (any || d_counters)) return fail(WF_ERROR_INVALID_ARGUMENT, "a second queue needs a closest-hit launch");
-    // word 0 is the work cursor; word 1 counts overflows, cleared here only
-    // when the caller asks (a frame counts them over all of its launches)
-    hipError_t e = launch_fill_u32((uint32_t*)d_work.p, 0u, reset_overflow ? 2 : 1, stream);
+    // the queue cursors start at zero; word 1 counts overflows, cleared here
+    // only when the caller asks (a frame counts them over all of its launches)
+    hipError_t e = launch_clear_claims((uint32_t*)d_work.p, reset_overflow, stream);
     if (e != hipSuccess) return hip_fail(e, "work counter reset");
     ExtendLaunch L{};
     L.scene.tnodes = (const wf_tlas_node*)d_tnodes.p;
@@ -737,6 +741,7 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     L.any_hit = any ? 1 : 0;
     L.refill_min = ext_cfg.refill_min;
     L.visits = ext_cfg.visits;
+    L.claim_segs = ext_cfg.claim_segs;
     if (second) {
         L.rays2 = second->rays;
         L.ray_root2 = second->roots;

@@ -121,20 +121,19 @@ __device__ __forceinline__ void gstu(void* p, uint32_t v) { *reinterpret_cast<ui
 __device__ __forceinline__ void gstu2(void* p, uint2 v) { *reinterpret_cast<uint2*>(p) = v; }
 #endif
 
-// In LDS the popped-interior skip also keeps each entry's distance as a
-// half-precision value rounded toward -inf: the distance only decides the
-// skip (skip when distance >= t); a rounded-down value never skips a node the
-// exact value would keep, and a node it keeps anyway is visited, its children
-// both miss, and the traversal pops on exactly as the reference does.  Spilled
-// entries keep the full float.
-__device__ __forceinline__ uint16_t half_down(float d)
+// In LDS the popped-interior skip also keeps each entry's distance in 16
+// bits, rounded toward -inf: the distance only decides the skip (skip when
+// distance >= t); a rounded-down value never skips a node the exact value
+// would keep, and a node it keeps anyway is visited, its children both miss,
+// and the traversal pops on exactly as the reference does.  The 16 bits are
+// the upper half of the float's pattern (bfloat16): for d >= 0, dropping the
+// lower half rounds toward zero, which is down; a negative distance is kept
+// as -inf (it can never reach t > 0).  Spilled entries keep the full float.
+__device__ __forceinline__ uint16_t dist_down(float d)
 {
-    const _Float16 h = (_Float16)d;                        // round to nearest even
-    uint16_t b = __builtin_bit_cast(uint16_t, h);
-    if ((float)h > d) b = (b & 0x8000u) ? (uint16_t)(b + 1u) : (b == 0u ? (uint16_t)0x8001u : (uint16_t)(b - 1u));
-    return b;
+    return d >= 0.0f ? (uint16_t)(__float_as_uint(d) >> 16) : (uint16_t)0xFF80u;
 }
-__device__ __forceinline__ float half_up_to_float(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+__device__ __forceinline__ float dist_to_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 // LDS stack of one thread: levels of BLOCK words (lanes of a wave hit
 // distinct banks).  Level 0 is a floor and level LDS + 1 a scratch level;
@@ -159,12 +158,12 @@ struct LdsStack {
     __device__ __forceinline__ uint32_t level(uint32_t sp) const { return (sp < (uint32_t)LDS ? sp : (uint32_t)LDS) * BLOCK; }
     // the top entry (valid for 0 < sp <= LDS)
     __device__ __forceinline__ uint32_t top(uint32_t sp) const { return ent[level(sp)]; }
-    __device__ __forceinline__ float top_dist(uint32_t sp) const { return DIST ? half_up_to_float(dist[level(sp)]) : 0.0f; }
+    __device__ __forceinline__ float top_dist(uint32_t sp) const { return DIST ? dist_to_float(dist[level(sp)]) : 0.0f; }
     // the word at slot sp when sp < LDS, else the scratch level
     __device__ __forceinline__ void put(uint32_t sp, uint32_t e, float d)
     {
         ent[level(sp) + BLOCK] = e;
-        if (DIST) dist[level(sp) + BLOCK] = half_down(d);
+        if (DIST) dist[level(sp) + BLOCK] = dist_down(d);
     }
     // a push at sp >= LDS: the spill area, or false when the stack is full
     __device__ __forceinline__ bool push_spill(uint32_t sp, uint32_t e, float d)
@@ -188,7 +187,7 @@ struct LdsStack {
     {
         if (sp < (uint32_t)LDS) {
             e = ent[(sp + 1u) * BLOCK];
-            d = DIST ? half_up_to_float(dist[(sp + 1u) * BLOCK]) : 0.0f;
+            d = DIST ? dist_to_float(dist[(sp + 1u) * BLOCK]) : 0.0f;
             return;
         }
         const uint2* q = spill + ((size_t)(sp - LDS) * spill_stride + spill_id);

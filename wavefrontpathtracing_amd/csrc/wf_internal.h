@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "claims.h"
 #include "wf_host.h"
 
 namespace wf {
@@ -133,6 +134,7 @@ struct ExtendConfig {
     uint32_t visits = 4;        // interior visits per pass of the traversal loop
     uint32_t layout = 0;        // 0: 2-wide pair records, 1: 4-wide records
     uint32_t big = 0;           // large claims in mixed launches: 0 by capacity, 1 always, 2 never
+    uint32_t claim_segs = 64;   // most queue segments (cursors) of an extend launch
 };
 
 // Launch the closest-hit kernel.  d_count (device) overrides n when non-null.
@@ -145,7 +147,7 @@ struct ExtendLaunch {
     wf_hit* hits;              // AoS output (may be null when hit_* SoA used)
     uint32_t* visible;         // occlusion output (any-hit mode)
     uint32_t* counters;        // 4 x u32 per ray, or null
-    uint32_t* work;            // [0] next ray, [1] stack overflows
+    uint32_t* work;            // [1] stack overflows; queue cursors from word kWorkClaimBase
     uint2* spill;              // global overflow stack
     uint32_t spill_per_thread;
     uint32_t stack_cap;        // entries a lane may hold (LDS + spill, or private)
@@ -154,6 +156,7 @@ struct ExtendLaunch {
     int any_hit;
     uint32_t refill_min;       // idle lanes of a wave that trigger a refill (1..64)
     uint32_t visits;           // interior visits per pass of the traversal loop (1..8)
+    uint32_t claim_segs;       // most queue segments (a power of two, 1..64)
     // optional second queue, traced for occlusion after the first (closest-hit
     // launches only): rays2[0 .. *d_count2 or n2) with their roots, visible2 out
     const wf_ray* rays2;
@@ -163,6 +166,8 @@ struct ExtendLaunch {
     uint32_t* visible2;
 };
 hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStream_t stream);
+// zero the queue cursors of the work buffer (and its overflow word)
+hipError_t launch_clear_claims(uint32_t* work, bool reset_overflow, hipStream_t stream);
 hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, size_t dyn_lds_bytes, int* blocks_per_cu);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t stream);
 

@@ -20,13 +20,14 @@ DEFAULT = dict(block=256, lds_stack=16, skip_popped=1, bvh_layout=0, refill_min=
 def set_extend_config(renderer, block: int = DEFAULT["block"], lds_stack: int = DEFAULT["lds_stack"],
                       skip_popped: int = None, bvh_layout: int = DEFAULT["bvh_layout"], stack_limit: int = 0,
                       refill_min: int = DEFAULT["refill_min"], tlas_in_lds: int = DEFAULT["tlas_in_lds"],
-                      interior_visits: int = DEFAULT["interior_visits"], big_claims: int = 0) -> None:
+                      interior_visits: int = DEFAULT["interior_visits"], big_claims: int = 0,
+                      claim_segments: int = 0) -> None:
     """skip_popped defaults to the library default, or to 1 with the 4-wide
     layout (which runs with the popped-interior skip only)."""
     if skip_popped is None:
         skip_popped = 1 if bvh_layout == 1 else DEFAULT["skip_popped"]
     c = L.TraversalConfig(block, lds_stack, skip_popped, bvh_layout, stack_limit, refill_min, tlas_in_lds,
-                          interior_visits, big_claims)
+                          interior_visits, big_claims, claim_segments)
     L.check(L.lib().wf_renderer_set_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "set_traversal_config")
 
 
@@ -35,7 +36,8 @@ def get_extend_config(renderer) -> dict:
     L.check(L.lib().wf_renderer_get_traversal_config(renderer.handle, C.byref(c)), renderer.handle, "get_traversal_config")
     return dict(block=c.workgroup_size, lds_stack=c.lds_stack_entries, skip_popped=c.skip_popped_interior,
                 bvh_layout=c.bvh_layout, stack_limit=c.stack_limit, refill_min=c.refill_min,
-                tlas_in_lds=c.tlas_in_lds, interior_visits=c.interior_visits, big_claims=c.big_claims)
+                tlas_in_lds=c.tlas_in_lds, interior_visits=c.interior_visits, big_claims=c.big_claims,
+                claim_segments=c.claim_segments)
 
 
 def candidates():
