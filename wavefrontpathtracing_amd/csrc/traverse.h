@@ -39,6 +39,23 @@ __device__ __forceinline__ float recip(float x)
     return 1.0f / x;
 }
 
+// The three reciprocals of a direction with one branch: the fast form for
+// all three, and IEEE division for all three in the lanes where any component
+// lies outside [2^-126, 2^126] (both forms agree wherever the fast one is
+// taken, so the results are recip()'s).
+__device__ __forceinline__ void recip3(float x, float y, float z, float& rx, float& ry, float& rz)
+{
+    auto in_range = [](float v) { const float a = fabsf(v); return a >= 0x1p-126f && a <= 0x1p126f; };   // false for NaN
+    const bool fast = in_range(x) && in_range(y) && in_range(z);
+    const float fx = __builtin_amdgcn_rcpf(x), fy = __builtin_amdgcn_rcpf(y), fz = __builtin_amdgcn_rcpf(z);
+    rx = __builtin_fmaf(__builtin_fmaf(-x, fx, 1.0f), fx, fx);
+    ry = __builtin_fmaf(__builtin_fmaf(-y, fy, 1.0f), fy, fy);
+    rz = __builtin_fmaf(__builtin_fmaf(-z, fz, 1.0f), fz, fz);
+    if (!fast) {
+        rx = 1.0f / x; ry = 1.0f / y; rz = 1.0f / z;
+    }
+}
+
 __device__ __forceinline__ float slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
                                       const Ray3& r, float t)
 {
@@ -245,7 +262,7 @@ __device__ __forceinline__ bool trace_begin(TraceLane& s, const TraceScene& sc, 
     s.t = ray.tmax;
     s.wr.ox = ray.origin[0]; s.wr.oy = ray.origin[1]; s.wr.oz = ray.origin[2];
     s.wr.dx = ray.direction[0]; s.wr.dy = ray.direction[1]; s.wr.dz = ray.direction[2];
-    s.wr.ix = recip(s.wr.dx); s.wr.iy = recip(s.wr.dy); s.wr.iz = recip(s.wr.dz);
+    recip3(s.wr.dx, s.wr.dy, s.wr.dz, s.wr.ix, s.wr.iy, s.wr.iz);
     s.cr = s.wr;
     s.sp = 0; s.base_sp = kNoBlas; s.cur_d = -INFINITY;
     s.nodes = reinterpret_cast<const char*>(L4 ? sc.tquads : sc.tpairs);
@@ -417,7 +434,7 @@ __device__ __forceinline__ int trace_pass(TraceLane& s, const TraceScene& sc, St
         s.cr.dx = (m0.x * w.dx + m0.w * w.dy) + (m1.z * w.dz + m2.y * 0.0f);
         s.cr.dy = (m0.y * w.dx + m1.x * w.dy) + (m1.w * w.dz + m2.z * 0.0f);
         s.cr.dz = (m0.z * w.dx + m1.y * w.dy) + (m2.x * w.dz + m2.w * 0.0f);
-        s.cr.ix = recip(s.cr.dx); s.cr.iy = recip(s.cr.dy); s.cr.iz = recip(s.cr.dz);
+        recip3(s.cr.dx, s.cr.dy, s.cr.dz, s.cr.ix, s.cr.iy, s.cr.iz);
         s.inst_id = id;
         s.cur = root;
         s.cur_d = -INFINITY;
