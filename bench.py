@@ -412,7 +412,9 @@ def batch_candidates(headline):
     queue entries often, and fewer, larger claims can pay)."""
     from wavefrontpathtracing_amd import tuning
     picks = [dict(tuning.DEFAULT)]
-    ranked = [c for c, _ in headline] if headline else list(tuning.candidates())
+    # frames do not order their queues by cost: that axis stays at its default here
+    ranked = [{**c, "cost_order": tuning.DEFAULT["cost_order"]} for c, _ in headline] if headline \
+        else list(tuning.candidates())
     picks += ranked[:4]
     for pred in (lambda c: c["lds_stack"] == 0, lambda c: c["bvh_layout"] == 1,
                  lambda c: c["bvh_layout"] == 0 and c["lds_stack"] > 0 and c["skip_popped"] == 1,
@@ -468,7 +470,7 @@ def bench_batched(frames: int, tune: bool, headline=None, fixed=None):
         best, table = tuning.autotune(r, lambda: batched_frame()[0], configs=batch_candidates(headline), repeats=1)
         tuned = {"winner": best, "seconds": round(time.perf_counter() - t_tune, 1),
                  "table_ms_per_frame": {tuning.label(c): round(t * 1e3, 1) for c, t in table},
-                 "labels": "block/lds_stack/skip_popped/bvh_layout/interior_visits/refill_min"}
+                 "labels": "block/lds_stack/skip_popped/bvh_layout/interior_visits/refill_min/cost_order"}
     batched_frame()
     bt, brays = 0.0, 0
     for _ in range(frames):
@@ -672,14 +674,17 @@ def main():
         t_tune = time.perf_counter()
         cfgs = [dict(c, interior_visits=v) for c in tuning.candidates() for v in (2, 4)]
         _, table = tuning.autotune(r, timed, configs=cfgs, repeats=3)
-        # the refill threshold for the fastest four (the default stays unless beaten by 1 %)
+        # the refill threshold for the fastest four, and the callers' order
+        # (no cost order) for the default and the fastest two; the default
+        # stays unless beaten by 1 %
         cfgs2 = [dict(tuning.DEFAULT)] + [dict(c, refill_min=m) for c, _ in table[:4] for m in (16, 24, 32)]
+        cfgs2 += [dict(c, cost_order=2) for c in [dict(tuning.DEFAULT)] + [c for c, _ in table[:2]]]
         best, table2 = tuning.autotune(r, timed, configs=cfgs2, repeats=3)
         seen = {tuning.label(c) for c, _ in table2}
         table = sorted(table2 + [(c, t) for c, t in table if tuning.label(c) not in seen], key=lambda x: x[1])
         autotuned = {"workload": "the headline rays", "candidates": len(table), "seconds": round(time.perf_counter() - t_tune, 1),
                      "winner": best, "table_mrays": {tuning.label(c): round(len(rays) / t / 1e6, 1) for c, t in table},
-                     "labels": "block/lds_stack/skip_popped/bvh_layout/interior_visits/refill_min"}
+                     "labels": "block/lds_stack/skip_popped/bvh_layout/interior_visits/refill_min/cost_order"}
 
     for _ in range(args.warmup):
         step()

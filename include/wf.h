@@ -233,7 +233,7 @@ WF_API wf_status wf_renderer_set_tlas_builder(wf_renderer* r, uint32_t builder);
 WF_API wf_status wf_renderer_set_tlas_metric(wf_renderer* r, uint32_t metric);
 
 /* Traversal kernel configuration (no reference counterpart: the autotuner's
- * axes; defaults 256, 16, 1, BVH2, no cap, 16, 0, 4, 0, 64).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
+ * axes; defaults 256, 16, 1, BVH2, no cap, 16, 0, 4, 0, 64, on).  workgroup_size 64/128/256; lds_stack_entries 0 (private stack in
  * scratch), 4, 8, 12 or 16 (LDS, spilling to global memory past that depth);
  * skip_popped_interior drops popped interior nodes whose entry distance is
  * not below the current hit distance; bvh_layout 0 = BVH2; stack_limit caps
@@ -253,11 +253,18 @@ WF_API wf_status wf_renderer_set_tlas_metric(wf_renderer* r, uint32_t metric);
  * claim_segments (a power of two in 1..64, 0 = default 64) caps how many
  * cursors an extend launch claims its queue from: 64-entry chunks of the
  * queue are dealt round-robin to that many segments (fewer for queues under
- * 512 entries per segment), each with a cursor of its own. */
+ * 512 entries per segment), each with a cursor of its own.
+ * cost_order (0 = default on, 1 = on, 2 = off; get returns 1 or 2) lets the
+ * trace calls on caller rays (wf_trace_closest, _occluded, _counters,
+ * _device) claim the rays predicted to traverse longest first: a pre-pass
+ * buckets the rays by their chord through the scene bounds, weighted by its
+ * distance from the bounds' centre, so long traversal chains start early
+ * instead of trailing the launch.  Results are the same either way; queues
+ * of at most two rays per persistent lane are traced in the callers' order. */
 typedef struct wf_traversal_config {
     uint32_t workgroup_size; uint32_t lds_stack_entries; uint32_t skip_popped_interior; uint32_t bvh_layout;
     uint32_t stack_limit; uint32_t refill_min; uint32_t tlas_in_lds; uint32_t interior_visits;
-    uint32_t big_claims; uint32_t claim_segments;
+    uint32_t big_claims; uint32_t claim_segments; uint32_t cost_order;
 } wf_traversal_config;
 WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, const wf_traversal_config* config);
 WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* r, wf_traversal_config* out);
@@ -385,6 +392,14 @@ WF_API wf_status wf_make_bvh_instance(const float transform[16], const float bla
  * count and *out_first_bits the bit pattern of the smallest mismatching input
  * (0xFFFFFFFF when none). */
 WF_API wf_status wf_selftest_reciprocal(uint64_t* out_mismatches, uint32_t* out_first_bits);
+/* Self-check of the cost-ordered queue (traversal config cost_order; no
+ * reference counterpart): for n device rays and the box [bmin, bmax], writes
+ * the queue order the trace calls would use (d_order, n device words: ray
+ * indices, predicted-longest bucket first, callers' order within a bucket)
+ * and each ray's cost bucket 0..15 (d_bucket, n device words).
+ * Synchronises the device. */
+WF_API wf_status wf_selftest_cost_order(const wf_ray* d_rays, uint32_t n, const float bmin[3], const float bmax[3],
+                                        uint32_t* d_order, uint32_t* d_bucket);
 
 #ifdef __cplusplus
 }

@@ -95,7 +95,7 @@ class TraversalConfig(C.Structure):
     _fields_ = [("workgroup_size", C.c_uint32), ("lds_stack_entries", C.c_uint32), ("skip_popped_interior", C.c_uint32),
                 ("bvh_layout", C.c_uint32), ("stack_limit", C.c_uint32), ("refill_min", C.c_uint32),
                 ("tlas_in_lds", C.c_uint32), ("interior_visits", C.c_uint32), ("big_claims", C.c_uint32),
-                ("claim_segments", C.c_uint32)]
+                ("claim_segments", C.c_uint32), ("cost_order", C.c_uint32)]
 
 
 class Pcg32(C.Structure):
@@ -168,6 +168,7 @@ SIGNATURES = {
     "wf_bvh_build_device": (_st, [_vp, _u32, C.POINTER(BuildOptions), _vp, _u64, _vp, C.POINTER(_u32), _vp]),
     "wf_bvh_build_device_noalloc": (_st, [_vp, _u32, C.POINTER(BuildOptions), _vp, _u64, _vp, C.POINTER(_u32), _vp]),
     "wf_selftest_reciprocal": (_st, [C.POINTER(_u64), C.POINTER(_u32)]),
+    "wf_selftest_cost_order": (_st, [_vp, _u32, _vp, _vp, _vp, _vp]),
     "wf_tlas_build": (_st, [_vp, _u32, _vp, _u64, C.POINTER(_u64), C.POINTER(_u32)]),
     "wf_tlas_build_metric": (_st, [_vp, _u32, _u32, _vp, _u64, C.POINTER(_u64), C.POINTER(_u32)]),
     "wf_make_bvh_instance": (_st, [_vp, _vp, _vp, _u32, _vp]),

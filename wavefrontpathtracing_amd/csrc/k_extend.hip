@@ -145,9 +145,9 @@ __device__ __forceinline__ void extend_loop(const ExtendLaunch& L, Stack& st, co
                     const uint32_t root = second ? (L.ray_root2 ? L.ray_root2[i] : 0u) : (L.ray_root ? L.ray_root[i] : 0u);
                     if (!trace_begin<COUNT, L4>(s, sc, second ? L.rays2[i] : L.rays[i], root, cnt)) status = kDone;
                 } else {
-                    ray_i = (uint32_t)q;
-                    const uint32_t root = L.ray_root ? L.ray_root[q] : 0u;
-                    if (!trace_begin<COUNT, L4>(s, sc, L.rays[q], root, cnt)) status = kDone;
+                    ray_i = L.order ? L.order[q] : (uint32_t)q;
+                    const uint32_t root = L.ray_root ? L.ray_root[ray_i] : 0u;
+                    if (!trace_begin<COUNT, L4>(s, sc, L.rays[ray_i], root, cnt)) status = kDone;
                 }
             }
         }

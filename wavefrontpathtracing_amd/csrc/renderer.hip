@@ -249,6 +249,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     if (c->big_claims > 2) return r->fail(WF_ERROR_INVALID_ARGUMENT, "big_claims must be 0 (by capacity), 1 (always) or 2 (never)");
     if (c->claim_segments > kClaimSegs || (c->claim_segments & (c->claim_segments - 1u)))
         return r->fail(WF_ERROR_INVALID_ARGUMENT, "claim_segments must be 0 or a power of two up to 64");
+    if (c->cost_order > 2) return r->fail(WF_ERROR_INVALID_ARGUMENT, "cost_order must be 0 (default), 1 (on) or 2 (off)");
     const ExtendConfig old = r->ext_cfg;
     r->ext_cfg.block = c->workgroup_size;
     r->ext_cfg.lds_stack = c->lds_stack_entries;
@@ -260,6 +261,7 @@ extern "C" WF_API wf_status wf_renderer_set_traversal_config(wf_renderer* r, con
     r->ext_cfg.layout = c->bvh_layout;
     r->ext_cfg.big = c->big_claims;
     r->ext_cfg.claim_segs = c->claim_segments ? c->claim_segments : kClaimSegs;
+    r->ext_cfg.cost_order = c->cost_order == 0 ? ExtendConfig{}.cost_order : (c->cost_order == 1 ? 1u : 0u);
     if (r->ext_cfg.layout != old.layout) r->tlas_valid = false;   // instance records point at the layout's arrays
     r->drop_graph();
     const wf_status s = r->ensure_trace_scratch();
@@ -280,6 +282,7 @@ extern "C" WF_API wf_status wf_renderer_get_traversal_config(const wf_renderer* 
     out->interior_visits = r->ext_cfg.visits;
     out->big_claims = r->ext_cfg.big;
     out->claim_segments = r->ext_cfg.claim_segs;
+    out->cost_order = r->ext_cfg.cost_order ? 1u : 2u;
     return WF_OK;
 }
 
@@ -712,8 +715,9 @@ extern "C" WF_API wf_status wf_renderer_get_scene_bounds(const wf_renderer* r, f
 
 wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
                                   uint32_t* d_visible, uint32_t* d_counters, bool any, const uint32_t* d_roots,
-                                  const OcclusionQueue* second, bool reset_overflow)
+                                  const OcclusionQueue* second, bool reset_overflow, const uint32_t* d_order)
 {
+    if (d_order && (second || d_roots)) return fail(WF_ERROR_INVALID_ARGUMENT, "a cost-ordered queue has one scene and one queue");
     if (second && (any || d_counters)) return fail(WF_ERROR_INVALID_ARGUMENT, "a second queue needs a closest-hit launch");
     // the queue cursors start at zero; word 1 counts overflows, cleared here
     // only when the caller asks (a frame counts them over all of its launches)
@@ -726,6 +730,7 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     L.scene.inst = (const GpuInstance*)d_inst.p;
     L.scene.tquads = (const float4*)d_tquads.p;
     L.rays = d_rays;
+    L.order = d_order;
     L.ray_root = d_roots;
     L.d_count = d_count;
     L.n = n;
@@ -751,6 +756,22 @@ wf_status wf_renderer::run_extend(const wf_ray* d_rays, const uint32_t* d_count,
     }
     e = launch_extend(L, ext_cfg, stream);
     if (e != hipSuccess) return hip_fail(e, "extend launch");
+    return WF_OK;
+}
+
+wf_status wf_renderer::cost_order(const wf_ray* d_rays, uint32_t n, const uint32_t** out)
+{
+    *out = nullptr;
+    // a queue no longer than two rays per persistent lane starts (nearly)
+    // every ray at once: there is no order to choose
+    if (!ext_cfg.cost_order || slots.empty() || (uint64_t)n < 2ull * ext_grid * ext_cfg.block) return WF_OK;
+    hipError_t e = d_order.ensure(4ull * n);
+    if (e == hipSuccess) e = d_order_scratch.ensure(4ull * order_scratch_words(n));
+    if (e != hipSuccess) return hip_fail(e, "cost-ordered queue allocation");
+    e = launch_cost_order(d_rays, n, slots[0].bounds_min, slots[0].bounds_max, (uint32_t*)d_order_scratch.p,
+                          (uint32_t*)d_order.p, stream);
+    if (e != hipSuccess) return hip_fail(e, "cost-ordered queue");
+    *out = (const uint32_t*)d_order.p;
     return WF_OK;
 }
 
@@ -783,8 +804,11 @@ static wf_status trace_host(wf_renderer* r, const wf_ray* rays, uint32_t n, wf_h
     if (e == hipSuccess) e = hipMemcpyAsync(r->d_tmp_rays.p, rays, sizeof(wf_ray) * (size_t)n, hipMemcpyHostToDevice, r->stream);
     if (e != hipSuccess) return r->hip_fail(e, "ray upload");
     if ((s = r->reset_overflows()) != WF_OK) return s;
+    const uint32_t* order = nullptr;
+    if ((s = r->cost_order((const wf_ray*)r->d_tmp_rays.p, n, &order)) != WF_OK) return s;
     s = r->run_extend((const wf_ray*)r->d_tmp_rays.p, nullptr, n, any ? nullptr : (wf_hit*)r->d_tmp_hits.p,
-                      any ? (uint32_t*)r->d_tmp_hits.p : nullptr, counters ? (uint32_t*)r->d_tmp_aux.p : nullptr, any);
+                      any ? (uint32_t*)r->d_tmp_hits.p : nullptr, counters ? (uint32_t*)r->d_tmp_aux.p : nullptr, any,
+                      nullptr, nullptr, false, order);
     if (s != WF_OK) return s;
     e = hipMemcpyAsync(any ? (void*)visible : (void*)hits, r->d_tmp_hits.p, out_bytes, hipMemcpyDeviceToHost, r->stream);
     if (e == hipSuccess && counters) e = hipMemcpyAsync(counters, r->d_tmp_aux.p, 16ull * n, hipMemcpyDeviceToHost, r->stream);
@@ -820,7 +844,9 @@ extern "C" WF_API wf_status wf_trace_closest_device(wf_renderer* r, const wf_ray
     wf_status s = r->upload_scene();
     if (s != WF_OK) return s;
     if (n == 0) return r->reset_overflows();
-    return r->run_extend(d_rays, nullptr, n, d_hits, nullptr, nullptr, false, nullptr, nullptr, true);
+    const uint32_t* order = nullptr;
+    if ((s = r->cost_order(d_rays, n, &order)) != WF_OK) return s;
+    return r->run_extend(d_rays, nullptr, n, d_hits, nullptr, nullptr, false, nullptr, nullptr, true, order);
 }
 
 extern "C" WF_API wf_status wf_renderer_read_stack_overflows(wf_renderer* r, uint32_t* out)

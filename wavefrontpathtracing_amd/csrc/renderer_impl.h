@@ -111,6 +111,7 @@ struct wf_renderer {
     uint32_t n_tpairs = 0;
     uint32_t spill_per_thread = 0;
     wf::DevBuf d_work, d_spill, d_tmp_rays, d_tmp_hits, d_tmp_aux;
+    wf::DevBuf d_order, d_order_scratch;   // cost-ordered queue of caller rays and its pre-pass scratch
     wf::DevBuf d_check;   // one word: the BLAS reference encoding's range check
 
     // wavefront buffers (allocated on first render)
@@ -139,7 +140,12 @@ struct wf_renderer {
     };
     wf_status run_extend(const wf_ray* d_rays, const uint32_t* d_count, uint32_t n, wf_hit* d_hits,
                          uint32_t* d_visible, uint32_t* d_counters, bool any, const uint32_t* d_roots = nullptr,
-                         const OcclusionQueue* second = nullptr, bool reset_overflow = false);
+                         const OcclusionQueue* second = nullptr, bool reset_overflow = false,
+                         const uint32_t* d_order = nullptr);
+    // the cost-ordered queue of n caller rays in d_order (when the traversal
+    // configuration asks for it and the launch is long enough to gain from
+    // it), else *out = nullptr
+    wf_status cost_order(const wf_ray* d_rays, uint32_t n, const uint32_t** out);
     wf_status read_overflows(uint32_t* out);
     wf_status reset_overflows();
     wf_status ensure_frame_buffers(size_t n_paths, size_t n_pixels);

@@ -135,12 +135,14 @@ struct ExtendConfig {
     uint32_t layout = 0;        // 0: 2-wide pair records, 1: 4-wide records
     uint32_t big = 0;           // large claims in mixed launches: 0 by capacity, 1 always, 2 never
     uint32_t claim_segs = 64;   // most queue segments (cursors) of an extend launch
+    uint32_t cost_order = 1;    // launches on caller rays claim the predicted-longest rays first
 };
 
 // Launch the closest-hit kernel.  d_count (device) overrides n when non-null.
 struct ExtendLaunch {
     TraceScene scene;
     const wf_ray* rays;
+    const uint32_t* order;     // queue entry -> ray index (a cost-ordered queue), or null for the identity
     const uint32_t* ray_root;  // per-ray TLAS root node (batched scenes), or null for root 0
     const uint32_t* d_count;
     uint32_t n;
@@ -170,5 +172,14 @@ hipError_t launch_extend(const ExtendLaunch& L, const ExtendConfig& cfg, hipStre
 hipError_t launch_clear_claims(uint32_t* work, bool reset_overflow, hipStream_t stream);
 hipError_t extend_occupancy(const ExtendConfig& cfg, bool tlas_lds, size_t dyn_lds_bytes, int* blocks_per_cu);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint32_t n, hipStream_t stream);
+
+// Cost-ordered queue of caller rays (k_order.hip): order[0, n) lists the ray
+// indices with the predicted-longest traversals first, in the callers' order
+// within each of kOrderBuckets cost buckets.  scratch (16-byte aligned)
+// needs order_scratch_words(n) words; [bmin, bmax] is the scene's root box.
+constexpr uint32_t kOrderBuckets = 16;
+uint32_t order_scratch_words(uint32_t n);
+hipError_t launch_cost_order(const wf_ray* rays, uint32_t n, const float bmin[3], const float bmax[3], uint32_t* scratch,
+                             uint32_t* order, hipStream_t stream);
 
 } // namespace wf
