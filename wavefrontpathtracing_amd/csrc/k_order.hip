@@ -95,16 +95,28 @@ __global__ __launch_bounds__(kOrderThreads) void k_order_hist(const wf_ray* rays
 // In place: hist becomes the exclusive prefix sum over (bucket, block) in
 // bucket-major order, i.e. the first queue position of each block's rays of
 // each bucket.  One block of 1024 threads; thread t scans a contiguous range
-// of the words, the range sums are scanned with wave shuffles.
+// of the words, the range sums are scanned with wave shuffles.  Up to
+// kScanLds words (launches of up to 4 M rays) are staged through LDS with
+// 16-byte loads and stores (total is a multiple of 16: 16 buckets x blocks).
+constexpr uint32_t kScanLds = 8192;
+
 __global__ __launch_bounds__(1024) void k_order_scan(uint32_t* hist, uint32_t total)
 {
     __shared__ uint32_t s_wave[16];
+    __shared__ uint4 s_h4[kScanLds / 4];
+    uint32_t* s_h = reinterpret_cast<uint32_t*>(s_h4);
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const bool staged = total <= kScanLds;
+    uint32_t* h = staged ? s_h : hist;
+    if (staged) {
+        for (uint32_t j = t; j < total / 4u; j += 1024u) s_h4[j] = reinterpret_cast<const uint4*>(hist)[j];
+        __syncthreads();
+    }
     const uint32_t per = (total + 1023u) / 1024u;
     const uint32_t a = t * per < total ? t * per : total;
     const uint32_t e = a + per < total ? a + per : total;
     uint32_t sum = 0;
-    for (uint32_t j = a; j < e; ++j) sum += hist[j];
+    for (uint32_t j = a; j < e; ++j) sum += h[j];
     uint32_t inc = sum;
 #pragma unroll
     for (uint32_t off = 1; off < 64u; off <<= 1) {
@@ -116,9 +128,13 @@ __global__ __launch_bounds__(1024) void k_order_scan(uint32_t* hist, uint32_t to
     uint32_t run = inc - sum;
     for (uint32_t w = 0; w < wave; ++w) run += s_wave[w];
     for (uint32_t j = a; j < e; ++j) {
-        const uint32_t v = hist[j];
-        hist[j] = run;
+        const uint32_t v = h[j];
+        h[j] = run;
         run += v;
+    }
+    if (staged) {
+        __syncthreads();
+        for (uint32_t j = t; j < total / 4u; j += 1024u) reinterpret_cast<uint4*>(hist)[j] = s_h4[j];
     }
 }
 
