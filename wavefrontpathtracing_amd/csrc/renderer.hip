@@ -179,6 +179,7 @@ extern "C" WF_API wf_status wf_renderer_exit(wf_renderer* r)
     r->d_lrem.release(); r->d_delta.release(); r->d_shadow_unit.release(); r->d_shadow_light.release();
     r->d_adj_dl.release(); r->d_adj_tex.release(); r->d_adj_em.release();
     r->d_work.release(); r->d_check.release(); r->d_spill.release(); r->d_tmp_rays.release(); r->d_tmp_hits.release(); r->d_tmp_aux.release();
+    r->d_order.release(); r->d_order_scratch.release();
     for (int i = 0; i < kNumStageEvents; ++i) if (r->ev[i]) (void)hipEventDestroy(r->ev[i]);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
