@@ -134,6 +134,35 @@ def test_cost_ordered_traces_equal_the_oracle(block):
         r.exit()
 
 
+def test_cost_ordered_occlusion_and_counters_equal_the_oracle():
+    """The other trace calls on caller rays at a size where the order is
+    used: occlusion (any hit) and the counting trace, whose per-ray counters
+    are written at each ray's own index like its hit."""
+    sc = scenes.small_tlas_scene()
+    r = Renderer(sc.width, sc.height)
+    try:
+        scenes.load(r, sc)
+        r.build_tlas()
+        orc = O.OracleScene(sc)
+        bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
+        n = 1100003
+        rays = scenes.sphere_rays(31, bmin, bmax, n)
+        rays["tmax"][::3] = np.float32(0.5)   # some occlusion queries end early
+        want_vis = orc.trace_any(rays, nthreads=16)
+        want_hits, want_cnt = orc.trace_closest(rays, nthreads=16, counters=True)
+        for order in (1, 2):
+            tuning.set_extend_config(r, cost_order=order)
+            vis = r.trace_occluded(rays)
+            assert np.array_equal(vis, want_vis), f"cost_order {order}: {(vis != want_vis).sum()} occlusion results differ"
+            hits, cnt = r.trace_closest_counters(rays)
+            assert len(O.hits_equal(hits, want_hits)) == 0
+            got = cnt.view(np.uint32).reshape(n, 4)
+            want = want_cnt.view(np.uint32).reshape(n, 4)
+            assert np.array_equal(got, want), f"cost_order {order}: counters differ for {(got != want).any(1).sum()} rays"
+    finally:
+        r.exit()
+
+
 def test_cost_order_setting_is_checked():
     sc = scenes.small_tlas_scene()
     r = Renderer(sc.width, sc.height)
