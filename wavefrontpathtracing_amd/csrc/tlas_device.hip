@@ -6,9 +6,12 @@
 // The merge sequence is the reference's and stays sequential: A's best match
 // B, B's best match C, a merge when C == A, otherwise the walk moves on, with
 // swap-remove of the merged pair from the candidate list.  Each best-match
-// search (every remaining candidate against one node) is the parallel part: a
-// 1024-lane argmin that keeps the reference's first-index tie-break, so the
-// nodes, their numbering and the list order are those of the host build.
+// search (every remaining candidate against one node) is the parallel part: an
+// argmin over the workgroup that keeps the reference's first-index tie-break,
+// so the nodes, their numbering and the list order are those of the host
+// build.  The walk makes a few searches per merge, each ending in barriers,
+// so small instance counts run in one wave (no cross-wave reduction) and
+// only large ones in 1024 lanes.
 // The candidate list (node ids and bounds) lives in LDS up to kTlasLdsMax
 // entries and in device memory beyond.
 #include "bvh_device.h"
@@ -20,15 +23,15 @@
 namespace wf {
 namespace {
 
-constexpr uint32_t kTWG = 1024;
 constexpr uint32_t kTlasLdsMax = 2048;
+constexpr uint32_t kTlasWaveMax = 1024;   // instance counts built by one wave
 
 __device__ __forceinline__ float tmin_(float x, float y) { return (y < x) ? y : x; }
 __device__ __forceinline__ float tmax_(float x, float y) { return (x < y) ? y : x; }
 __device__ __forceinline__ float half_area3(float ex, float ey, float ez) { return ex * ey + ey * ez + ez * ex; }
 
 // status[0] = node count, status[1] = 1 if the walk found no match or did not end
-template <bool LDS, bool UNION>
+template <bool LDS, bool UNION, uint32_t kTWG>
 __global__ __launch_bounds__(kTWG) void k_tlas_build(const wf_bvh_instance* __restrict__ inst, uint32_t m, wf_tlas_node* nodes,
                                                      float* g_list, uint32_t* g_ids, uint32_t* status)
 {
@@ -148,12 +151,16 @@ wf_status device_build_tlas(const wf_bvh_instance* d_inst, uint32_t m, wf_tlas_n
     if (e == hipSuccess && m > kTlasLdsMax) e = list.ensure(24ull * m);
     if (e == hipSuccess && m > kTlasLdsMax) e = ids.ensure(4ull * m);
     if (e != hipSuccess) { err = "TLAS builder scratch allocation failed"; return WF_ERROR_OUT_OF_MEMORY; }
-    if (m <= kTlasLdsMax) {
-        if (uni) k_tlas_build<true, true><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, nullptr, nullptr, (uint32_t*)status.p);
-        else k_tlas_build<true, false><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, nullptr, nullptr, (uint32_t*)status.p);
+    uint32_t* stp = (uint32_t*)status.p;
+    if (m <= kTlasWaveMax) {
+        if (uni) k_tlas_build<true, true, 64><<<1, 64, 0, st>>>(d_inst, m, d_nodes, nullptr, nullptr, stp);
+        else k_tlas_build<true, false, 64><<<1, 64, 0, st>>>(d_inst, m, d_nodes, nullptr, nullptr, stp);
+    } else if (m <= kTlasLdsMax) {
+        if (uni) k_tlas_build<true, true, 1024><<<1, 1024, 0, st>>>(d_inst, m, d_nodes, nullptr, nullptr, stp);
+        else k_tlas_build<true, false, 1024><<<1, 1024, 0, st>>>(d_inst, m, d_nodes, nullptr, nullptr, stp);
     } else {
-        if (uni) k_tlas_build<false, true><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, (float*)list.p, (uint32_t*)ids.p, (uint32_t*)status.p);
-        else k_tlas_build<false, false><<<1, kTWG, 0, st>>>(d_inst, m, d_nodes, (float*)list.p, (uint32_t*)ids.p, (uint32_t*)status.p);
+        if (uni) k_tlas_build<false, true, 1024><<<1, 1024, 0, st>>>(d_inst, m, d_nodes, (float*)list.p, (uint32_t*)ids.p, stp);
+        else k_tlas_build<false, false, 1024><<<1, 1024, 0, st>>>(d_inst, m, d_nodes, (float*)list.p, (uint32_t*)ids.p, stp);
     }
     uint32_t h[2] = {0, 0};
     e = hipGetLastError();
