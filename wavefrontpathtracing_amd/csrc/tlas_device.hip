@@ -29,6 +29,13 @@ constexpr uint32_t kTlasWaveMax = 1024;   // instance counts built by one wave
 __device__ __forceinline__ float tmin_(float x, float y) { return (y < x) ? y : x; }
 __device__ __forceinline__ float tmax_(float x, float y) { return (x < y) ? y : x; }
 __device__ __forceinline__ float half_area3(float ex, float ey, float ez) { return ex * ey + ey * ez + ez * ex; }
+// a metric's order-preserving key: -0 as +0, negative patterns inverted,
+// positive ones with the sign bit set (for values below FLT_MAX, not NaN)
+__device__ __forceinline__ uint32_t lane_key(float v)
+{
+    const uint32_t u = __float_as_uint(v == 0.0f ? 0.0f : v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
 
 // status[0] = node count, status[1] = 1 if the walk found no match or did not end
 template <bool LDS, bool UNION, uint32_t kTWG>
@@ -38,7 +45,7 @@ __global__ __launch_bounds__(kTWG) void k_tlas_build(const wf_bvh_instance* __re
     constexpr uint32_t CAP = LDS ? kTlasLdsMax : 1;
     __shared__ float s_list[6][CAP];
     __shared__ uint32_t s_ids[CAP];
-    __shared__ float s_ra[kTWG / 64];
+    __shared__ uint32_t s_rk[kTWG / 64];
     __shared__ uint32_t s_ri[kTWG / 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     float* lst[6];
@@ -82,19 +89,22 @@ __global__ __launch_bounds__(kTWG) void k_tlas_build(const wf_bvh_instance* __re
             }
             if (area < best) { best = area; bi = b; }
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            const float ob = __shfl_xor(best, o, 64);
-            const uint32_t oi = __shfl_xor(bi, o, 64);
-            if (ob < best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-        }
-        if (lane == 0) { s_ra[wave] = best; s_ri[wave] = bi; }
+        // The wave's argmin in two hardware wave reductions: the least key,
+        // then the least index holding it.  A key orders like the float
+        // (sign-flipped bit pattern, -0 taken as +0 so that equal metrics tie
+        // and the first index wins); a lane without a candidate (only FLT_MAX
+        // or NaN seen) has the largest key.
+        const uint32_t mine = bi != 0xFFFFFFFFu ? lane_key(best) : 0xFFFFFFFFu;
+        uint32_t key = __ockl_wfred_min_u32(mine);
+        bi = __ockl_wfred_min_u32(mine == key ? bi : 0xFFFFFFFFu);
+        if constexpr (kTWG == 64) return bi;
+        if (lane == 0) { s_rk[wave] = key; s_ri[wave] = bi; }
         __syncthreads();
-        best = s_ra[0];
+        key = s_rk[0];
         bi = s_ri[0];
         for (uint32_t w = 1; w < kTWG / 64; ++w) {
-            const float ob = s_ra[w];
-            const uint32_t oi = s_ri[w];
-            if (ob < best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            const uint32_t ok = s_rk[w], oi = s_ri[w];
+            if (ok < key || (ok == key && oi < bi)) { key = ok; bi = oi; }
         }
         __syncthreads();
         return bi;
