@@ -7,7 +7,7 @@ Workload: the 256-instance TLAS scene over 4 BLAS (value-noise heightfield
 scene bounds, direction uniform on the sphere, seed 3).  One step = one
 closest-hit pass over all rays, inputs resident in HBM.
 
-Prints one JSON line (driver contract) with a roofline object for the extend
+Prints one JSON line (metric, value, unit, steps, config ...) with a roofline object for the extend
 kernel (algorithmic bytes from the reference algorithm's per-ray traversal
 counters, measured with the library's counting kernel) and the CPU oracle
 timed on the host cores on a bounded sample.
