@@ -6,8 +6,8 @@ cost buckets (its chord through the scene bounds, weighted by its distance
 from their centre) and lists the ray indices bucket by bucket, keeping the
 callers' order within a bucket.  Checked here:
   * the order is a permutation of the rays, bucket-major and stable within a
-    bucket, for a ragged ray count and for rays that miss the bounds, have a
-    zero or NaN direction or an empty interval;
+    bucket, for ragged ray counts on both sides of 2^21 and for rays that miss
+    the bounds, have a zero or NaN direction or an empty interval;
   * the buckets agree with a float64 restatement of the predictor except at
     bucket boundaries (the pre-pass uses the hardware reciprocal and square
     root estimates: a scheduling heuristic, not a result);
@@ -65,12 +65,15 @@ def _special_rays(bmin, bmax, n):
     return out
 
 
-def test_cost_order_is_a_stable_bucket_sort():
+@pytest.mark.parametrize("n", [100003, 3000017])
+def test_cost_order_is_a_stable_bucket_sort(n):
+    """100,003 rays: the scan of the per-block histograms (16 words per 4,096
+    rays) is staged through LDS; 3,000,017 rays (more than 2^21): it runs on
+    the histograms in global memory."""
     import torch
     sc = scenes.small_tlas_scene()
     orc = O.OracleScene(sc)
     bmin, bmax = orc.tnodes[0]["aabb_min"], orc.tnodes[0]["aabb_max"]
-    n = 100003
     rays = scenes.sphere_rays(23, bmin, bmax, n)
     rays[:400] = _special_rays(bmin, bmax, 400)
     d_rays = torch.from_numpy(np.frombuffer(rays.tobytes(), np.uint8).copy()).cuda()
