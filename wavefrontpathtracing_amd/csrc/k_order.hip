@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kOrderThreads) void k_order_hist(const wf_ray* rays
 // bucket-major order, i.e. the first queue position of each block's rays of
 // each bucket.  One block of 1024 threads; thread t scans a contiguous range
 // of the words, the range sums are scanned with wave shuffles.  Up to
-// kScanLds words (launches of up to 4 M rays) are staged through LDS with
+// kScanLds words (launches of up to 2^21 rays) are staged through LDS with
 // 16-byte loads and stores (total is a multiple of 16: 16 buckets x blocks).
 constexpr uint32_t kScanLds = 8192;
 
@@ -219,12 +219,16 @@ OrderBox make_box(const float bmin[3], const float bmax[3])
     return b;
 }
 
+// chunks of n rays, rounded up in 64 bits (n + kOrderChunk - 1 wraps in 32
+// bits for n within a chunk of 2^32)
+uint32_t order_blocks(uint32_t n) { return (uint32_t)(((uint64_t)n + kOrderChunk - 1u) / kOrderChunk); }
+
 } // namespace
 
 uint32_t order_scratch_words(uint32_t n)
 {
     // the bucket bytes (whole chunks) and the histogram
-    const uint32_t nblocks = (n + kOrderChunk - 1u) / kOrderChunk;
+    const uint32_t nblocks = order_blocks(n);
     return nblocks * (kOrderChunk / 4u) + kOrderBuckets * nblocks;
 }
 
@@ -233,7 +237,7 @@ hipError_t launch_cost_order(const wf_ray* rays, uint32_t n, const float bmin[3]
 {
     if (n == 0) return hipSuccess;
     const OrderBox b = make_box(bmin, bmax);
-    const uint32_t nblocks = (n + kOrderChunk - 1u) / kOrderChunk;
+    const uint32_t nblocks = order_blocks(n);
     uint8_t* bucket = reinterpret_cast<uint8_t*>(scratch);
     uint32_t* hist = scratch + nblocks * (kOrderChunk / 4u);
     k_order_hist<<<nblocks, kOrderThreads, 0, stream>>>(rays, n, b, bucket, hist, nblocks);
@@ -246,7 +250,7 @@ hipError_t launch_cost_buckets(const wf_ray* rays, uint32_t n, const float bmin[
                                hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
-    k_order_buckets<<<(n + 255u) / 256u, 256, 0, stream>>>(rays, n, make_box(bmin, bmax), bucket);
+    k_order_buckets<<<(uint32_t)(((uint64_t)n + 255u) / 256u), 256, 0, stream>>>(rays, n, make_box(bmin, bmax), bucket);
     return hipGetLastError();
 }
 
