@@ -37,8 +37,10 @@ static wf_status adjoint_run(wf_renderer* r, const float* d_dl, int32_t tex_slot
     const uint32_t grid = (uint32_t)r->cu_count * 8u;
     wf_status s;
     if ((e = hipEventRecord(r->ev[0], r->stream)) != hipSuccess) return r->hip_fail(e, "event");
-    if (tex_texels && (e = launch_fill_u32((uint32_t*)d_grad_tex, 0u, (uint32_t)(4 * tex_texels), r->stream)) != hipSuccess)
-        return r->hip_fail(e, "gradient clear");
+    // in chunks of 2^28 words: a fill's count is 32-bit
+    for (size_t off = 0, words = 4 * tex_texels, chunk = size_t(1) << 28; off < words; off += chunk)
+        if ((e = launch_fill_u32((uint32_t*)d_grad_tex + off, 0u, (uint32_t)std::min(chunk, words - off), r->stream)) != hipSuccess)
+            return r->hip_fail(e, "gradient clear");
     if ((e = launch_fill_u32((uint32_t*)d_grad_em, 0u, (uint32_t)n_inst, r->stream)) != hipSuccess) return r->hip_fail(e, "gradient clear");
     if ((s = r->reset_overflows()) != WF_OK) return s;
     if ((e = launch_adjoint_clear(A, grid, r->stream)) != hipSuccess) return r->hip_fail(e, "adjoint clear launch");
