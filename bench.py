@@ -431,13 +431,14 @@ def batch_candidates(headline):
     return out
 
 
-def bench_batched(frames: int, tune: bool, headline=None, fixed=None):
+def bench_batched(frames: int, tune: bool, headline=None, fixed=None, batch_only: bool = False):
     """64 uneven scenes (soup of 10K..2M triangles in a lit box room, 256^2 /
     512^2 / 1024^2 pixels), 4 spp, depth 4, next-event estimation: the whole
     batch as one frame of one launch per stage (captured hipGraph) against
     the plain loop of one frame per scene.  Both produce the same images
     (the batch path numbering equals the per-scene one).  `fixed` (a
-    traversal configuration) replaces the tuner for both."""
+    traversal configuration) replaces the tuner for both.  `batch_only`
+    stops after the batched frames (for profiling them)."""
     from wavefrontpathtracing_amd import Renderer, scenes, tuning
     t0 = time.perf_counter()
     sl = scenes.batch_scenes()
@@ -477,6 +478,10 @@ def bench_batched(frames: int, tune: bool, headline=None, fixed=None):
         dt, nr = batched_frame()
         bt += dt
         brays += nr
+    if batch_only:
+        r.exit()
+        return {"batched_mrays": round(brays / bt / 1e6, 2), "batched_ms_per_frame": round(bt / frames * 1e3, 2),
+                "rays_per_frame": brays // frames, "frames": frames, "fixed_traversal": fixed, "batch_only": True}
     batch_imgs = [r.read_scene_energy(k, sc.width, sc.height) for k, sc in enumerate(sl)]
     # the same images, one scene at a time (no graph), with the traversal
     # configuration tuned for that loop
@@ -613,6 +618,9 @@ def main():
                     help="fixed traversal configuration, e.g. block=64,lds_stack=16,interior_visits=4 (implies --no-tune)")
     ap.add_argument("--batch-traversal", default="",
                     help="fixed traversal configuration of the batched section alone (its tuner then does not run)")
+    ap.add_argument("--batch-frames-only", action="store_true",
+                    help="batched section: only the batched frames, no per-scene, reference-loop or megakernel "
+                         "comparison (for profiling the batched frame)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -813,7 +821,8 @@ def main():
     if not args.no_batch:
         out["batched"] = bench_batched(frames=2, tune=not args.no_tune and not args.traversal,
                                        headline=None if autotuned is None else table,
-                                       fixed=parse_traversal(args.batch_traversal))
+                                       fixed=parse_traversal(args.batch_traversal),
+                                       batch_only=args.batch_frames_only)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(frames=3)
     if not args.no_inverse:
